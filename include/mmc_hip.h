@@ -1,0 +1,156 @@
+/*
+ * mmc_hip.h -- C ABI of the MI355X photon-transport library (libmmc_hip.so).
+ *
+ * The reference selects a transport backend with one call of the shape
+ *     void mmc_run_XX(mcconfig* cfg, tetmesh* mesh, raytracer* tracer)
+ * (mmc_run_cl: src/mmc_cl_host.h:126, mmc_run_cu: src/mmc_cu_host.h:58,
+ *  mmc_run_mp: src/mmc_host.h:54), dispatched in src/mmc.c:82-103.
+ * This header exposes the same operation with plain pointers and sizes: a
+ * caller fills mmc_hip_mesh from tetmesh/raytracer after mmc_prep()
+ * (src/mmc_host.c:136-165) and mmc_hip_sim from mcconfig, and receives the
+ * fluence field, detected-photon records and energy totals that mmc_run_cl
+ * writes into cfg->exportfield / exportdetected / energytot / energyesc.
+ *
+ * All element and node ids are 1-based, as in tetmesh.  Nothing in this ABI
+ * refers to torch or to HIP types.
+ */
+#ifndef MMC_HIP_H
+#define MMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMC_HIP_ABI_VERSION 1
+
+/* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
+#define MMC_RT_PLUCKER   0
+#define MMC_RT_BLBADOUEL 3
+/* source types (enum TSrcType, src/mmc_utils.h:101) */
+#define MMC_SRC_PENCIL   0
+#define MMC_SRC_PLANAR   4
+#define MMC_SRC_PATTERN  5
+/* output types (enum TOutputType, src/mmc_utils.h:106) */
+#define MMC_OUT_FLUX     0
+#define MMC_OUT_FLUENCE  1
+#define MMC_OUT_ENERGY   2
+/* boundary conditions (enum TBoundary, src/mmc_utils.h:116) */
+#define MMC_BC_NOREFLECT 0
+#define MMC_BC_REFLECT   1
+#define MMC_BC_ABSORBEXT 2
+#define MMC_BC_MIRROR    3
+
+/* per-photon termination codes in mmc_hip_photon.reason */
+#define MMC_END_EXIT     1   /* left the mesh through an exterior face */
+#define MMC_END_LOST     2   /* no exit face after 3 nudges (eid = 0x7FFFFFFF) */
+#define MMC_END_ROULETTE 3   /* killed by Russian roulette */
+#define MMC_END_STEPLIMIT 4  /* exceeded the per-photon step bound (a photon cycling at a vertex) */
+#define MMC_MAX_STEPS    (1u << 26)  /* per-photon bound on ray-tetrahedron tests */
+
+/* Mesh after mmc_prep: the fields of tetmesh (src/mmc_mesh.h:87-122) and
+ * raytracer->n (src/mmc_mesh.h:135-141) that the transport reads. */
+typedef struct mmc_hip_mesh {
+    int32_t nn, ne, prop, nf;
+    const float* node;       /* nn*3 */
+    const int32_t* elem;     /* ne*4, positively oriented (mesh_getvolume) */
+    const int32_t* type;     /* ne, 0 = background, 1..prop */
+    const int32_t* facenb;   /* ne*4, exterior faces numbered -1,-2,... (tracer_prep) */
+    const float* planes;     /* ne*16 per element {nx[4],ny[4],nz[4],d[4]} (tracer->n) */
+    const float* med;        /* (prop+1)*4 {mua,mus,g,n}; row 0 = background (mesh->med) */
+    const int32_t* srcelem;  /* candidate launch elements for area sources (mesh->srcelem) */
+    int32_t srcelemlen;
+} mmc_hip_mesh;
+
+/* One simulation: the mcconfig fields (src/mmc_utils.h:210-345) the
+ * transport path reads, plus the per-photon seed of this build. */
+typedef struct mmc_hip_sim {
+    int64_t nphoton;
+    uint64_t seed;           /* photon i starts from mmc_hip_photon_seeds(seed, i) */
+    int64_t photon_offset;   /* id of the first photon (respin / chunked runs) */
+    int32_t method;          /* MMC_RT_PLUCKER or MMC_RT_BLBADOUEL */
+    int32_t basisorder;      /* 0 element field, 1 nodal field */
+    int32_t outputtype;
+    int32_t isreflect, isspecular, voidtime;
+    int32_t srctype, srcnum;
+    int32_t e0;              /* initial element (cfg->e0) */
+    float bary0[4];          /* its barycentric coordinates (cfg->bary0) */
+    float srcpos[4], srcdir[4], srcparam1[4], srcparam2[4];
+    const float* srcpattern; /* srcparam1.w*srcparam2.w weights (pattern source) */
+    float tstart, tend, tstep;
+    int32_t maxgate;
+    float nout, minenergy, roulettesize;
+    int32_t issavedet, issaveexit;
+    int32_t detnum;
+    const float* detpos;     /* detnum*4 {x,y,z,r} */
+    int64_t maxdetphoton;
+    const float* med;        /* optional (prop+1)*4 media replacing mesh->med */
+} mmc_hip_sim;
+
+/* Per-photon end state, written for photon ids in [recfirst, recfirst+reccount). */
+typedef struct mmc_hip_photon {
+    float p[3]; float w;     /* final position and weight (0 after roulette death) */
+    float v[3]; float t;     /* final direction, time of flight (s) */
+    int32_t eid, reason, nraytet, detid;
+    float eabs, rgain, w0, nscat;  /* absorbed energy, roulette gain, launch weight, scatterings */
+} mmc_hip_photon;
+
+typedef struct mmc_hip_result {
+    double* field;           /* datalen*maxgate, datalen = nn (basisorder 1) or ne; accumulated (+=) */
+    double launchweight;     /* total launched weight   (cfg->energytot) */
+    double absorbweight;     /* total absorbed weight */
+    double escapeweight;     /* final weight of photons that ended by exit or lost */
+    double roulettegain;     /* net weight created (>0) or removed (<0) by roulette */
+    double raytet;           /* ray-tetrahedron tests (reporter->raytet) */
+    float* detdata;          /* maxdetphoton*reclen records, or NULL */
+    int64_t* detphoton;      /* photon id of each saved record, or NULL */
+    int64_t detected;        /* detected photons (may exceed maxdetphoton) */
+    int32_t reclen;          /* floats per record: 1 + 2*prop + 6*issaveexit + 1 */
+    mmc_hip_photon* photons; /* reccount records, or NULL */
+    int64_t recfirst, reccount;
+    double kernel_ms;        /* transport kernels only */
+    double total_ms;         /* including uploads, reductions and copies */
+} mmc_hip_result;
+
+/* Launch shape for the persistent kernel; zeros pick the defaults. */
+typedef struct mmc_hip_tune {
+    int32_t block;           /* threads per workgroup (multiple of 64) */
+    int32_t blocks_per_cu;   /* resident workgroups per CU */
+    int32_t chunk;           /* photons claimed per lane refill round (1 = per photon) */
+    int32_t compensated;     /* 1: TwoSum-compensated FP32 atomics (hi+lo buffers) */
+} mmc_hip_tune;
+
+typedef struct mmc_hip_meshres* mmc_hip_mesh_handle;
+
+/* ---- library / device ---------------------------------------------------- */
+int32_t mmc_hip_abi_version(void);
+const char* mmc_hip_last_error(void);
+/* Fills name (len bytes) and returns the CU count, or -1 without a device. */
+int32_t mmc_hip_device_info(char* name, int32_t len, int64_t* hbm_bytes);
+
+/* Per-photon xorshift128+ states (2 words per photon) for photons
+ * [first, first+n) of a simulation seeded with `seed`.  Host only. */
+void mmc_hip_photon_seeds(uint64_t seed, int64_t first, int64_t n, uint64_t* out);
+
+/* ---- plain drop-in (replaces mmc_run_cl, src/mmc_cl_host.c:87-1721) ------- */
+/* Uploads the mesh, runs the simulation, accumulates into res, frees all
+ * device memory.  Returns 0 or a negative error (see mmc_hip_last_error). */
+int32_t mmc_hip_run(const mmc_hip_mesh* mesh, const mmc_hip_sim* sim,
+                    const mmc_hip_tune* tune, mmc_hip_result* res);
+
+/* ---- resident meshes and batching (no reference counterpart; the batch
+ *      follows the per-slot output offsets of src/mmc_core.cl:859-861) ------ */
+int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_handle* out);
+int32_t mmc_hip_mesh_release(mmc_hip_mesh_handle h);
+int64_t mmc_hip_mesh_bytes(mmc_hip_mesh_handle h);
+/* Runs nsim simulations on one resident mesh in a single persistent launch. */
+int32_t mmc_hip_run_batch(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_t nsim,
+                          const mmc_hip_tune* tune, mmc_hip_result* res);
+/* Device bytes the outputs of a simulation occupy (admission accounting). */
+int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+/*
+ * mmc_host.h -- host-side mesh preparation, normalization and output writers
+ * of the MI355X photon tracer (part of libmmc_hip.so; no GPU calls).
+ *
+ * These restate the reference's mmc_prep() chain (src/mmc_host.c:136-165):
+ * mesh_getvolume (src/mmc_mesh.c:910-950), mesh_getfacenb
+ * (src/mmc_highorder.cpp:128-159), tracer_build (src/mmc_mesh.c:1496-1602),
+ * tracer_prep (src/mmc_mesh.c:1241-1483), mesh_initelem / mesh_barycentric
+ * (src/mmc_mesh.c:1060-1206) and mesh_normalize (src/mmc_mesh.c:2154-2279),
+ * plus the writers mesh_saveweight (src/mmc_mesh.c:1726-1800), mcx_savedata
+ * (src/mmc_utils.c:919-993) and mcx_savedetphoton (src/mmc_utils.c:4309-4342).
+ */
+#ifndef MMC_HOST_H
+#define MMC_HOST_H
+
+#include <stdint.h>
+#include "mmc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* .mch header, field-compatible with `history` (src/mmc_utils.h:168-184) */
+typedef struct mmc_mch_header {
+    char magic[4];
+    uint32_t version, maxmedia, detnum, colcount, totalphoton, detected, savedphoton;
+    float unitinmm;
+    uint32_t seedbyte;
+    float normalizer;
+    uint32_t srcnum;
+    int32_t respin;
+    uint32_t savedetflag;
+    int32_t reserved[2];
+} mmc_mch_header;
+
+/* Element volumes, node volumes and orientation fix (swaps nodes 3,4 of
+ * negatively oriented elements in place). */
+void mmc_host_volume(int32_t nn, const float* node, int32_t ne, int32_t* elem,
+                     const int32_t* type, float* evol, float* nvol);
+/* Face-neighbour table: facenb[i*4+j] = element sharing face j of element i
+ * (faces {0,1,2},{0,1,3},{0,2,3},{1,2,3}), 0 on the exterior. */
+int32_t mmc_host_facenb(int32_t nn, int32_t ne, const int32_t* elem, int32_t* facenb);
+/* Face planes: per element {nx[4],ny[4],nz[4],d[4]}, unit outward normals. */
+void mmc_host_planes(int32_t ne, const float* node, const int32_t* elem, float* planes);
+/* Barycentric test of srcpos in element e0 (1-based): 0 and bary if inside. */
+int32_t mmc_host_barycentric(const float* node, const int32_t* elem, int32_t e0,
+                             const float* srcpos, float* bary);
+/* First element (index order) enclosing srcpos, 0 if none. */
+int32_t mmc_host_initelem(int32_t ne, const float* node, const int32_t* elem,
+                          const float* srcpos, float* bary);
+/* Effective reflection coefficient (mesh_getreff). */
+double mmc_host_reff(double n_in, double n_out);
+/* Surface-node volume correction for nodal output; facenb must still hold 0
+ * on exterior faces. */
+void mmc_host_surface_nvol(int32_t nn, int32_t ne, const int32_t* elem, const int32_t* type,
+                           const int32_t* facenb, const float* med, int32_t prop,
+                           int32_t isreflect, float* nvol);
+/* Numbers exterior faces -1,-2,... in element-major order; returns the count. */
+int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb);
+/* Normalizes a field in place and returns the factor (mesh_normalize). */
+double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
+                          const int32_t* elem, const int32_t* type, const float* med,
+                          const float* evol, const float* nvol, double* field,
+                          double absorbweight, double launchweight);
+
+/* Writers.  Return 0 on success. */
+int32_t mmc_host_save_dat(const char* fname, const double* field, int32_t datalen,
+                          int32_t maxgate);
+int32_t mmc_host_save_bin(const char* fname, const double* field, int64_t count);
+int32_t mmc_host_save_mch(const char* fname, const mmc_mch_header* his, const float* data,
+                          int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+/*
+ * mmc_kernel.h -- device data layout shared by the transport kernels and the
+ * host runtime of the MI355X photon tracer.
+ */
+#ifndef MMC_KERNEL_H
+#define MMC_KERNEL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mmc_hip.h"
+
+namespace mmc {
+
+constexpr int kBlock = 256;          /* default threads per workgroup (4 waves) */
+constexpr int kMaxProp = 15;         /* media labels 1..kMaxProp per mesh */
+constexpr int kLaneSlots = 8;        /* per-lane LDS accumulators (energy, Kahan terms) */
+
+/* Per-simulation parameters read on every step; staged in LDS (32 B). */
+enum { kFlagRoulette = 1, kFlagEnergy = 2, kFlagSaveDet = 4 };
+struct SimHot {
+    float tstart, tend, rtstep, minenergy;
+    float roulettesize;
+    int32_t maxgate, flags, pad0;
+    uint64_t fieldoff;               /* float offset of this simulation's field */
+    uint64_t pad1;
+};
+
+/* Per-simulation parameters read at launch / exit / deposit (global, cached). */
+struct SimCold {
+    float4 srcpos, srcdir, srcparam1, srcparam2, bary0;
+    float tstep, nout, minenergy, roulettesize;
+    int32_t e0, srctype, detnum, detbase;
+    int32_t patbase, patnx, patny, srcnum;
+    int32_t isreflect, isspecular, voidtime, outputtype;
+    int32_t issavedet, issaveexit, pad0, pad1;
+    int64_t nphoton, photon_first;
+    uint64_t seed;
+    uint64_t fieldoff;               /* float offset of this simulation's field */
+};
+
+/* Accumulators per simulation (FP64, final reduction only). */
+enum SimAcc { kAccLaunch = 0, kAccAbsorb, kAccEscape, kAccRGain, kAccRaytet, kAccCount };
+
+struct KArgs {
+    /* mesh, structure of arrays */
+    const float4* planes;            /* ne*4: nx, ny, nz, d */
+    const int4* facenb;              /* ne */
+    const int4* elem;                /* ne, 1-based */
+    const int32_t* type;             /* ne */
+    const float4* node;              /* nn, w unused */
+    const float4* enode;             /* ne*3: the 12 node coordinates of each element (Plucker) */
+    const int32_t* srcelem;
+    int32_t srcelemlen;
+    int32_t nn, ne, prop, nsim;
+    /* simulations */
+    const SimHot* hot;               /* nsim */
+    const SimCold* cold;             /* nsim */
+    const float4* media;             /* nsim*(prop+1) */
+    const int64_t* cum;              /* nsim+1 cumulative photon counts */
+    const float4* detpos;
+    const float* pattern;
+    /* outputs */
+    float* field;
+    float* field_lo;                 /* compensation buffer (TwoSum atomics) or null */
+    double* simacc;                  /* nsim*kAccCount */
+    float* detdata;
+    int64_t* detphoton;
+    unsigned long long* detcount;    /* nsim+1 counters: [0] global slot counter */
+    int64_t maxdet;
+    int32_t reclen, savedet;
+    mmc_hip_photon* rec;
+    int64_t recfirst, reccount;      /* global photon index window */
+    /* work queue */
+    unsigned long long* queue;
+    int64_t total;
+    int32_t chunk;
+    uint32_t maxsteps;               /* per-photon bound on ray-tetrahedron tests */
+};
+
+/* Per-photon xorshift128+ state: splitmix64 finalizer over (seed, photon id).
+ * Host and device use the same rule, so a CPU run can replay any photon. */
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id, uint64_t& s0,
+                                                     uint64_t& s1) {
+    uint64_t z = seed + (2 * id + 1) * 0x9E3779B97F4A7C15ull;
+    s0 = mix64(z);
+    s1 = mix64(z + 0x9E3779B97F4A7C15ull);
+    if ((s0 | s1) == 0) s1 = 1;
+}
+
+template <bool NODAL, bool COMP>
+__global__ void transport(KArgs a);
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n);
+__global__ void fill_zero_u64(unsigned long long* p, int n);
+
+}  // namespace mmc
+
+#endif

@@ -1,0 +1,704 @@
+/*
+ * mmc_kernel.hip -- persistent photon-transport kernel for gfx950 (MI355X).
+ *
+ * One photon per lane.  Every iteration of the persistent loop performs one
+ * ray-tetrahedron test per live lane followed by the bookkeeping the
+ * reference's onephoton() does after that test (src/mmc_raytrace.c:1772-2126):
+ * face crossing, Fresnel reflection, exit/detector capture, roulette and
+ * scattering.  The nested loops of onephoton are flattened into a per-lane
+ * state machine (ctx = OUTER / INNER / INNERFIX) so that every lane of a
+ * wavefront runs its tracer in the same iteration.  Lanes whose photon ended
+ * are refilled from a device work queue: a wave ballot counts the dead lanes,
+ * one lane claims that many photon ids with a single atomic, and each dead
+ * lane takes base + (number of dead lanes below it).
+ *
+ * Mesh arrays are structure-of-arrays in HBM.  Per-tissue optical properties
+ * and per-simulation parameters are staged in LDS; per-lane energy tallies and
+ * partial-path buffers live in LDS too.  Fluence deposits are coalesced in a
+ * register while the photon stays in one element and time gate (as
+ * src/mmc_raytrace.c:1562-1612 does) and then written with FP32 atomics.
+ *
+ * All per-photon arithmetic is FP32, in the reference CPU's operation order
+ * (compiled with -ffp-contract=off); FP64 appears only in the per-simulation
+ * energy totals written when a lane changes simulation or exits.
+ */
+#include <hip/hip_runtime.h>
+
+#include "mmc_fp32math.h"
+#include "mmc_kernel.h"
+
+namespace mmc {
+
+#define EPSF       1e-6f                      /* src/mmc_rand_common.h:46 */
+#define R_C0F      3.335640951981520e-12f     /* src/mmc_const.h:45 */
+#define R_MIN_MUSF 1e9f                       /* src/mmc_mesh.h:72 */
+#define ID_UNDEF   0x7FFFFFFF                 /* src/mmc_mesh.h:66 */
+#define MAX_TRIAL  3                          /* src/mmc_raytrace.h:44 */
+#define FIX_PHOTON 1e-3f                      /* src/mmc_raytrace.h:45 */
+
+enum { CTX_OUTER = 0, CTX_INNER = 1, CTX_INNERFIX = 2 };
+
+/* face tables (src/mmc_raytrace.c:75-152, src/mmc_mesh.c:59-103) */
+__constant__ int c_fc[4][3] = {{0, 4, 2}, {3, 5, 4}, {2, 5, 1}, {1, 3, 0}};
+__constant__ int c_nc[4][3] = {{3, 0, 1}, {3, 1, 2}, {2, 0, 3}, {1, 0, 2}};
+__constant__ int c_out[4][3] = {{0, 3, 1}, {3, 2, 1}, {0, 2, 3}, {0, 1, 2}};
+__constant__ int c_facemap[4] = {2, 0, 1, 3};
+__constant__ int c_ifacemap[4] = {1, 2, 0, 3};
+
+/* plane index -> facenb column, and its inverse */
+__device__ __forceinline__ int faceorder(int i) { return (0x1320 >> (4 * (3 - i))) & 0xF; }
+__device__ __forceinline__ int ifaceorder(int f) { return (0x3021 >> (4 * (3 - f))) & 0xF; }
+
+__device__ __forceinline__ int pick4(int4 v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ float pick4f(float4 v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+/* packed face neighbour: (element << 4) | media label of that element */
+__device__ __forceinline__ int nb_id(int packed) { return packed >> 4; }
+__device__ __forceinline__ int nb_type(int packed) { return packed & 15; }
+
+/* ---- xorshift128+ (src/mmc_rand_xorshift128p.c:55-70) ------------------- */
+__device__ __forceinline__ float rng_next(uint64_t& s0r, uint64_t& s1r) {
+    uint64_t s1 = s0r;
+    const uint64_t s0 = s1r;
+    s0r = s0;
+    s1 ^= s1 << 23;
+    s1r = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+    uint32_t lo = (uint32_t)(s1r + s0);
+    return __uint_as_float(0x3F800000u | (lo >> 9)) - 1.0f;
+}
+__device__ __forceinline__ float rng_scatlen(uint64_t& a, uint64_t& b) {
+    return -mmc_logf(rng_next(a, b) + EPSF);
+}
+
+/* SSE4.1 dpps over xyz: (ax*bx + ay*by) + (az*bz + 0) */
+__device__ __forceinline__ float dpps(float ax, float ay, float az, float bx, float by, float bz) {
+    float t2 = ax * bx + ay * by;
+    float t3 = az * bz + 0.0f;
+    return t2 + t3;
+}
+
+/* Kahan add into a lane-private LDS pair */
+__device__ __forceinline__ void kahan_lds(float* s, float* c, float v) {
+    float y = v - *c;
+    float t = *s + y;
+    *c = (t - *s) - y;
+    *s = t;
+}
+
+template <bool COMP>
+__device__ __forceinline__ void deposit(const KArgs& a, uint64_t idx, float v) {
+    if (!COMP) {
+        atomicAdd(a.field + idx, v);
+    } else {
+        /* TwoSum against the value the atomic returned: hi + lo holds the
+         * exact running sum up to the rounding of lo itself */
+        float old = atomicAdd(a.field + idx, v);
+        float s = old + v;
+        float bp = s - old;
+        float err = (old - (s - bp)) + (v - bp);
+        if (err != 0.f) atomicAdd(a.field_lo + idx, err);
+    }
+}
+
+/*
+ * The transport kernel.  NODAL selects the Plucker tracer with linear nodal
+ * deposits (src/mmc_raytrace.c:227-508); otherwise the branch-less Badouel
+ * tracer with element deposits (src/mmc_raytrace.c:1414-1709).
+ */
+template <bool NODAL, bool COMP>
+__global__ void __launch_bounds__(kBlock) transport(KArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nsim = a.nsim;
+    const int nmed = a.prop + 1;
+    SimHot* s_hot = (SimHot*)smem;
+    float4* s_med = (float4*)(s_hot + nsim);
+    int64_t* s_cum = (int64_t*)(s_med + (size_t)nsim * nmed);
+    float* s_lane = (float*)(s_cum + nsim + 1);
+    const int nthr = blockDim.x;
+    const int tid = threadIdx.x;
+    /* lane-private LDS: [slot][thread] */
+    auto LS = [&](int slot) -> float* { return s_lane + (size_t)slot * nthr + tid; };
+    const int ppbase = kLaneSlots;                   /* nscat[prop], ppath[prop] */
+    const int slot_w0 = kLaneSlots + 2 * a.prop;
+    const int slot_rg = slot_w0 + 1;
+    const int slot_ns = slot_w0 + 2;
+    const int nslots = slot_w0 + 3;
+
+    for (int i = tid; i < nsim; i += nthr) s_hot[i] = a.hot[i];
+    for (int i = tid; i < nsim * nmed; i += nthr) s_med[i] = a.media[i];
+    for (int i = tid; i <= nsim; i += nthr) s_cum[i] = a.cum[i];
+    for (int k = 0; k < nslots; k++) *LS(k) = 0.f;
+    __syncthreads();
+
+    const int lane = __lane_id();
+    const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    /* ---- lane state ---- */
+    bool alive = false;
+    bool exhausted = false;
+    int sim = -1;
+    int64_t gid = 0;
+    float px = 0.f, py = 0.f, pz = 0.f, vx = 0.f, vy = 0.f, vz = 1.f;
+    int eid = 0, ctype = 0;
+    float w = 0.f, tof = 0.f, slen = 0.f, Lmove = 0.f;
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+    uint32_t oldidx = 0xFFFFFFFFu;
+    float oldw = 0.f;
+    uint64_t r0 = 0, r1 = 0;
+    int ctx = CTX_OUTER, fixcount = 0;
+    float eabs = 0.f, eabs_c = 0.f;
+    uint32_t raytet = 0, raytet_start = 0;
+
+    auto flush_sim = [&](int s) {
+        if (s < 0) return;
+        double* acc = a.simacc + (size_t)s * kAccCount;
+        float v;
+        v = *LS(0) - *LS(1); if (v != 0.f) atomicAdd(acc + kAccLaunch, (double)v);
+        v = *LS(2) - *LS(3); if (v != 0.f) atomicAdd(acc + kAccAbsorb, (double)v);
+        v = *LS(4) - *LS(5); if (v != 0.f) atomicAdd(acc + kAccEscape, (double)v);
+        v = *LS(6);          if (v != 0.f) atomicAdd(acc + kAccRGain, (double)v);
+        if (raytet) atomicAdd(acc + kAccRaytet, (double)raytet);
+        for (int k = 0; k < kLaneSlots; k++) *LS(k) = 0.f;
+        raytet = 0;
+    };
+
+    while (true) {
+        /* ------------------------------------------------ refill dead lanes */
+        uint64_t deadmask = __ballot(!alive);
+        if (deadmask && !exhausted) {
+            int ndead = __popcll(deadmask);
+            int leader = __ffsll((unsigned long long)deadmask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)ndead);
+            uint32_t blo = __shfl((uint32_t)base, leader);
+            uint32_t bhi = __shfl((uint32_t)(base >> 32), leader);
+            base = ((unsigned long long)bhi << 32) | blo;
+            if ((int64_t)(base + ndead) >= a.total) exhausted = true;
+            if (!alive) {
+                int64_t g = (int64_t)base + __popcll(deadmask & lanemask_lt);
+                if (g < a.total) {
+                    /* locate the simulation of photon g */
+                    int lo = 0, hi = nsim - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_cum[mid] <= g) lo = mid; else hi = mid - 1;
+                    }
+                    if (lo != sim) {
+                        flush_sim(sim);
+                        sim = lo;
+                    }
+                    gid = g;
+                    alive = true;
+                    const SimCold& C = a.cold[sim];
+                    uint64_t pid = (uint64_t)(C.photon_first + (g - s_cum[sim]));
+                    photon_seed(C.seed, pid, r0, r1);
+                    /* launchphoton (src/mmc_raytrace.c:2346-2671) */
+                    px = C.srcpos.x; py = C.srcpos.y; pz = C.srcpos.z;
+                    vx = C.srcdir.x; vy = C.srcdir.y; vz = C.srcdir.z;
+                    b0 = C.bary0.x; b1 = C.bary0.y; b2 = C.bary0.z; b3 = C.bary0.w;
+                    eid = C.e0;
+                    w = 1.f; tof = 0.f; Lmove = 0.f;
+                    oldidx = 0xFFFFFFFFu; oldw = 0.f;
+                    ctx = CTX_OUTER; fixcount = 0;
+                    eabs = 0.f; eabs_c = 0.f;
+                    raytet_start = raytet;
+                    slen = rng_scatlen(r0, r1);
+                    if (C.srctype == MMC_SRC_PLANAR || C.srctype == MMC_SRC_PATTERN) {
+                        float rx = rng_next(r0, r1);
+                        float ry = rng_next(r0, r1);
+                        px = C.srcpos.x + rx * C.srcparam1.x + ry * C.srcparam2.x;
+                        py = C.srcpos.y + rx * C.srcparam1.y + ry * C.srcparam2.y;
+                        pz = C.srcpos.z + rx * C.srcparam1.z + ry * C.srcparam2.z;
+                        if (C.srctype == MMC_SRC_PATTERN) {
+                            int xs = C.patnx, ys = C.patny;
+                            int iy = min((int)(ry * ys), ys - 1);
+                            int ix = min((int)(rx * xs), xs - 1);
+                            w = (C.srcnum == 1) ? a.pattern[C.patbase + iy * xs + ix] : 1.f;
+                        }
+                        px = EPSF * vx + 1.f * px;
+                        py = EPSF * vy + 1.f * py;
+                        pz = EPSF * vz + 1.f * pz;
+                        /* enclosing element: e0 first, then the candidate list */
+                        int found = 0;
+                        for (int is = -1; is < a.srcelemlen && !found; is++) {
+                            int ce = (is < 0) ? eid : a.srcelem[is];
+                            if (ce <= 0) continue;
+                            int4 ee = a.elem[ce - 1];
+                            float bc[4];
+                            for (int f = 0; f < 4; f++) {
+                                float4 A = a.node[pick4(ee, c_out[f][0]) - 1];
+                                float4 B = a.node[pick4(ee, c_out[f][1]) - 1];
+                                float4 Cc = a.node[pick4(ee, c_out[f][2]) - 1];
+                                float abx = B.x - A.x, aby = B.y - A.y, abz = B.z - A.z;
+                                float acx = Cc.x - A.x, acy = Cc.y - A.y, acz = Cc.z - A.z;
+                                float sx = px - A.x, sy = py - A.y, sz = pz - A.z;
+                                float nx = aby * acz - abz * acy;
+                                float ny = abz * acx - abx * acz;
+                                float nz = abx * acy - aby * acx;
+                                bc[c_facemap[f]] = -(sx * nx + sy * ny + sz * nz);
+                            }
+                            if (bc[0] < -1e-4f || bc[1] < -1e-4f || bc[2] < -1e-4f || bc[3] < -1e-4f)
+                                continue;
+                            float s = 0.f;
+                            s += bc[0]; s += bc[1]; s += bc[2]; s += bc[3];
+                            eid = ce;
+                            b0 = bc[0] / s; b1 = bc[1] / s; b2 = bc[2] / s; b3 = bc[3] / s;
+                            found = 1;
+                        }
+                        if (!found) eid = ID_UNDEF;   /* reported as a lost photon */
+                    }
+                    ctype = (eid > 0 && eid != ID_UNDEF) ? a.type[eid - 1] : 0;
+                    *LS(slot_w0) = w;
+                    *LS(slot_rg) = 0.f;
+                    *LS(slot_ns) = 0.f;
+                    if (a.savedet)
+                        for (int k = 0; k < 2 * a.prop; k++) *LS(ppbase + k) = 0.f;
+                    kahan_lds(LS(0), LS(1), w);
+                    if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */
+                }
+            }
+        }
+        if (__ballot(alive) == 0) break;
+        if (!alive) continue;
+
+        const SimHot hot = s_hot[sim];
+        const float4* med = s_med + (size_t)sim * nmed;
+        int reason = 0;
+        int faceid = -1, isend = 0, undef = 1, nexteid = 0, faceidx = 4;
+        float poutx = 0.f, pouty = 0.f, poutz = 0.f;
+
+        if (ctx >= 0 && raytet - raytet_start >= a.maxsteps) {
+            ctx = -1;                      /* bounded lifetime: ends as a lost photon */
+            reason = MMC_END_STEPLIMIT;
+        }
+        if (ctx >= 0) {
+            /* ============================ ray-tetrahedron step ============ */
+            const int e = eid - 1;
+            const int4 fnb = a.facenb[e];
+            const float4 m = med[ctype];
+            const float mua = m.x, mus = m.y, nidx = m.w;
+            const float rc = nidx * R_C0F;
+            const float currweight = w;
+            raytet++;
+            if (!NODAL) {
+                const float4* P = a.planes + 4 * (size_t)e;
+                const float4 NX = P[0], NY = P[1], NZ = P[2], DD = P[3];
+                float T[4];
+                #pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float nx = pick4f(NX, i), ny = pick4f(NY, i), nz = pick4f(NZ, i), d = pick4f(DD, i);
+                    float t = nx * px;
+                    t = t + ny * py;
+                    t = t + nz * pz;
+                    t = d - t;
+                    float s = nx * vx;
+                    s = s + ny * vy;
+                    s = s + nz * vz;
+                    t = t / s;
+                    T[i] = (s > 0.f) ? t : 1e10f;
+                }
+                float m02 = (T[0] < T[2]) ? T[0] : T[2];
+                float m13 = (T[1] < T[3]) ? T[1] : T[3];
+                float bx = (m02 < m13) ? m02 : m13;
+                faceidx = (T[3] == bx) ? 3 : (T[2] == bx) ? 2 : (T[1] == bx) ? 1 : (T[0] == bx) ? 0 : 4;
+                faceid = (faceidx < 4) ? faceorder(faceidx) : -1;
+                if (faceid >= 0 && bx >= 0.f) {
+                    nexteid = pick4(fnb, faceid);
+                    float dlen = (mus <= EPSF) ? R_MIN_MUSF : slen / mus;
+                    isend = (bx > dlen);
+                    Lmove = isend ? dlen : bx;
+                    poutx = vx * bx + px;
+                    pouty = vy * bx + py;
+                    poutz = vz * bx + pz;
+                    undef = 0;
+                    if ((int)((tof + Lmove * rc - hot.tstart) * hot.rtstep) > hot.maxgate - 1) {
+                        faceid = -2;
+                        undef = 1;
+                        Lmove = (hot.tend - tof) / (nidx * R_C0F) - 1e-4f;
+                    }
+                    w *= mmc_expf(-mua * Lmove);
+                    slen -= Lmove * mus;
+                    float ww = currweight - w;
+                    tof += Lmove * rc;
+                    int g = (int)((tof - hot.tstart) * hot.rtstep);
+                    int tshift = max(min(g, hot.maxgate - 1), 0) * a.ne;
+                    if (mua > 0.f) {
+                        float y = ww - eabs_c, tt = eabs + y;
+                        eabs_c = (tt - eabs) - y;
+                        eabs = tt;
+                    }
+                    if (!(hot.flags & kFlagEnergy))
+                        ww = (mua < EPSF) ? (currweight * Lmove) : ((currweight - w) / mua);
+                    px = px + vx * Lmove;
+                    py = py + vy * Lmove;
+                    pz = pz + vz * Lmove;
+                    uint32_t newidx = (uint32_t)(e + tshift);
+                    const uint64_t foff = hot.fieldoff;
+                    if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
+                    if (newidx != oldidx) {
+                        deposit<COMP>(a, foff + oldidx, oldw);
+                        oldidx = newidx;
+                        oldw = ww;
+                    } else {
+                        oldw += ww;
+                    }
+                    if (faceid == -2 || !isend) {
+                        deposit<COMP>(a, foff + newidx, oldw);
+                        oldw = 0.f;
+                    }
+                }
+                if (faceid == -2) slen = 0.f;
+            } else {
+                /* Plucker test over the 6 edges (src/mmc_raytrace.c:227-508) */
+                const int4 ee = a.elem[e];
+                const float4 q0 = a.enode[3 * (size_t)e], q1 = a.enode[3 * (size_t)e + 1],
+                             q2 = a.enode[3 * (size_t)e + 2];
+                const float nxs[4] = {q0.x, q0.w, q1.z, q2.y};
+                const float nys[4] = {q0.y, q1.x, q1.w, q2.z};
+                const float nzs[4] = {q0.z, q1.y, q2.x, q2.w};
+                float p1x = px + vx, p1y = py + vy, p1z = pz + vz;
+                float cx = py * p1z - pz * p1y;
+                float cy = pz * p1x - px * p1z;
+                float cz = px * p1y - py * p1x;
+                float wv[6];
+                #pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    const int i0 = (i < 3) ? 0 : (i < 5 ? 1 : 2);
+                    const int i1 = (i == 0) ? 1 : (i == 1 || i == 3) ? 2 : 3;
+                    float ax = nxs[i0], ay = nys[i0], az = nzs[i0];
+                    float bxx = nxs[i1], byy = nys[i1], bzz = nzs[i1];
+                    float dx = bxx - ax, dy = byy - ay, dz = bzz - az;
+                    float mx = ay * bzz - az * byy, my = az * bxx - ax * bzz, mz = ax * byy - ay * bxx;
+                    wv[i] = dpps(vx, vy, vz, mx, my, mz) + dpps(cx, cy, cz, dx, dy, dz);
+                }
+                float bo[4] = {0.f, 0.f, 0.f, 0.f};
+                float Lp0 = 0.f;
+                Lmove = 0.f;
+                for (int i = 0; i < 4; i++) {
+                    if (i >= 2) wv[c_fc[i][1]] = -wv[c_fc[i][1]];
+                    float wa = wv[c_fc[i][0]], wb = wv[c_fc[i][1]], wc = wv[c_fc[i][2]];
+                    if ((__float_as_uint(wa) & 0x80000000u) & (__float_as_uint(wb) & 0x80000000u) &
+                        (__float_as_uint(wc) ^ 0x80000000u)) {
+                        float Rv = 1.f / (-wa - wb + wc);
+                        int na = c_nc[i][0], nb = c_nc[i][1], nc = c_nc[i][2];
+                        float ba = -wa * Rv, bb = -wb * Rv, bcc = wc * Rv;
+                        bo[na] = ba; bo[nb] = bb; bo[nc] = bcc;
+                        poutx = ba * nxs[na] + bb * nxs[nb] + bcc * nxs[nc];
+                        pouty = ba * nys[na] + bb * nys[nb] + bcc * nys[nc];
+                        poutz = ba * nzs[na] + bb * nzs[nb] + bcc * nzs[nc];
+                        undef = 0;
+                        float dx = poutx - px, dy = pouty - py, dz = poutz - pz;
+                        Lp0 = sqrtf(dx * dx + dy * dy + dz * dz);
+                        float dlen = (mus <= EPSF) ? R_MIN_MUSF : slen / mus;
+                        faceidx = i;
+                        faceid = faceorder(i);
+                        nexteid = pick4(fnb, faceid);
+                        isend = (Lp0 > dlen);
+                        Lmove = isend ? dlen : Lp0;
+                        break;
+                    }
+                }
+                if (!undef) {
+                    if ((int)((tof + Lmove * rc - hot.tstart) * hot.rtstep) >=
+                        (int)((hot.tend - hot.tstart) * hot.rtstep)) {
+                        faceid = -2;
+                        undef = 1;
+                        Lmove = (hot.tend - tof) / (nidx * R_C0F) - 1e-4f;
+                    }
+                    w *= mmc_expf(-mua * Lmove);
+                    slen -= Lmove * mus;
+                    px += Lmove * vx;
+                    py += Lmove * vy;
+                    pz += Lmove * vz;
+                    if (Lp0 > EPSF) {
+                        tof += Lmove * rc;
+                        float ww = currweight - w;
+                        if (mua > 0.f || !(hot.flags & kFlagEnergy)) {
+                            float ratio = Lmove / Lp0;
+                            if (mua > 0.f) {
+                                float y = ww - eabs_c, tt = eabs + y;
+                                eabs_c = (tt - eabs) - y;
+                                eabs = tt;
+                            }
+                            if (!(hot.flags & kFlagEnergy))
+                                ww = (mua < EPSF) ? (currweight * Lmove) : ((currweight - w) / mua);
+                            int g = (int)((tof - hot.tstart) * hot.rtstep);
+                            uint64_t base = hot.fieldoff + (uint64_t)max(min(g, hot.maxgate - 1), 0) * a.nn;
+                            ww *= 0.5f;
+                            if (isend) {
+                                bo[0] = (1.f - ratio) * b0 + ratio * bo[0];
+                                bo[1] = (1.f - ratio) * b1 + ratio * bo[1];
+                                bo[2] = (1.f - ratio) * b2 + ratio * bo[2];
+                                bo[3] = (1.f - ratio) * b3 + ratio * bo[3];
+                            }
+                            deposit<COMP>(a, base + (uint32_t)(ee.x - 1), ww * (b0 + bo[0]));
+                            deposit<COMP>(a, base + (uint32_t)(ee.y - 1), ww * (b1 + bo[1]));
+                            deposit<COMP>(a, base + (uint32_t)(ee.z - 1), ww * (b2 + bo[2]));
+                            deposit<COMP>(a, base + (uint32_t)(ee.w - 1), ww * (b3 + bo[3]));
+                        }
+                        if (isend) {
+                            b0 = bo[0]; b1 = bo[1]; b2 = bo[2]; b3 = bo[3];
+                        } else if (nb_id(nexteid) > 0 && faceidx < 4) {
+                            /* carry the exit-face barycentrics into the next element */
+                            int4 ne4 = a.elem[nb_id(nexteid) - 1];
+                            float nbv[4] = {0.f, 0.f, 0.f, 0.f};
+                            for (int j = 0; j < 3; j++) {
+                                int node = pick4(ee, c_nc[faceidx][j]);
+                                float bv = bo[c_nc[faceidx][j]];
+                                if (ne4.x == node) nbv[0] = bv;
+                                else if (ne4.y == node) nbv[1] = bv;
+                                else if (ne4.z == node) nbv[2] = bv;
+                                else if (ne4.w == node) nbv[3] = bv;
+                            }
+                            b0 = nbv[0]; b1 = nbv[1]; b2 = nbv[2]; b3 = nbv[3];
+                        }
+                    }
+                    if (faceid == -2) slen = 0.f;
+                }
+            }
+
+            /* ======================= onephoton bookkeeping after the test == */
+            const bool savedet = (hot.flags & kFlagSaveDet) != 0;
+            bool next_iter = false;
+            if (ctx == CTX_OUTER) {
+                if (undef) {
+                    if (fixcount++ < MAX_TRIAL) {
+                        ctx = CTX_OUTER;
+                        next_iter = true;
+                    } else {
+                        eid = ID_UNDEF;
+                        faceid = -1;
+                    }
+                }
+                if (!next_iter && savedet && Lmove > 0.f && faceid >= 0 && ctype > 0)
+                    *LS(ppbase + a.prop - 1 + ctype) += Lmove;
+            } else {
+                if (savedet && Lmove > 0.f && ctype > 0) *LS(ppbase + a.prop - 1 + ctype) += Lmove;
+                if (ctx == CTX_INNER) fixcount = 0;
+                if (undef) {
+                    if (fixcount++ < MAX_TRIAL) {
+                        ctx = CTX_INNERFIX;
+                        next_iter = true;
+                    } else {
+                        eid = ID_UNDEF;
+                    }
+                }
+            }
+            if (next_iter) {
+                /* fixphoton: nudge 0.1% toward the centroid (src/mmc_raytrace.c:200-213) */
+                int4 ee = a.elem[eid - 1];
+                float4 n0 = a.node[ee.x - 1], n1 = a.node[ee.y - 1], n2 = a.node[ee.z - 1], n3 = a.node[ee.w - 1];
+                float cxx = 0.f, cyy = 0.f, czz = 0.f;
+                cxx = cxx + n0.x; cyy = cyy + n0.y; czz = czz + n0.z;
+                cxx = cxx + n1.x; cyy = cyy + n1.y; czz = czz + n1.z;
+                cxx = cxx + n2.x; cyy = cyy + n2.y; czz = czz + n2.z;
+                cxx = cxx + n3.x; cyy = cyy + n3.y; czz = czz + n3.z;
+                px += (cxx * 0.25f - px) * FIX_PHOTON;
+                py += (cyy * 0.25f - py) * FIX_PHOTON;
+                pz += (czz * 0.25f - pz) * FIX_PHOTON;
+                continue;
+            }
+
+            /* ---- crossing into the neighbour (inner while of onephoton) ---- */
+            if (eid != ID_UNDEF && faceid >= 0 && !isend) {
+                const SimCold& C = a.cold[sim];
+                px = poutx; py = pouty; pz = poutz;
+                int oldeid = eid;
+                int oldtype = ctype;
+                int nbp = nexteid;
+                int neid = nb_id(nbp);
+                int ntype = neid > 0 ? nb_type(nbp) : 0;
+                float nold = med[oldtype].w;
+                if (C.isreflect && (neid <= 0 || med[ntype].w != nold)) {
+                    if (!(neid <= 0 && ((nold == C.nout && C.isreflect != MMC_BC_MIRROR) ||
+                                        C.isreflect == MMC_BC_ABSORBEXT))) {
+                        /* reflectray (src/mmc_raytrace.c:2257-2332) */
+                        int f = ifaceorder(faceid);
+                        const float4* P = a.planes + 4 * (size_t)(oldeid - 1);
+                        float pnx = pick4f(P[0], f), pny = pick4f(P[1], f), pnz = pick4f(P[2], f);
+                        float Icos = fabsf(vx * pnx + vy * pny + vz * pnz);
+                        float n1 = nold;
+                        float n2 = (neid > 0) ? med[ntype].w : C.nout;
+                        float t0 = n1 * n1, t1 = n2 * n2;
+                        float t2 = 1.f - t0 / t1 * (1.f - Icos * Icos);
+                        if (t2 > 0.f && !(neid <= 0 && C.isreflect == MMC_BC_MIRROR)) {
+                            float Re = t0 * Icos * Icos + t1 * t2;
+                            t2 = sqrtf(t2);
+                            float Im = 2.f * n1 * n2 * Icos * t2;
+                            float Rt = (Re - Im) / (Re + Im);
+                            Re = t1 * Icos * Icos + t0 * t2 * t2;
+                            Rt = (Rt + (Re - Im) / (Re + Im)) * 0.5f;
+                            if (rng_next(r0, r1) <= Rt) {
+                                float k = -2.f * Icos;
+                                vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
+                                neid = oldeid;
+                                ntype = oldtype;
+                            } else if (C.isspecular == 2 && neid == 0) {
+                            } else {
+                                float k = -Icos;
+                                vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
+                                float r = n1 / n2;
+                                vx = r * vx + t2 * pnx; vy = r * vy + t2 * pny; vz = r * vz + t2 * pnz;
+                            }
+                        } else {
+                            float k = -2.f * Icos;
+                            vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
+                            neid = oldeid;
+                            ntype = oldtype;
+                        }
+                        float inv = 1.f / sqrtf(vx * vx + vy * vy + vz * vz);
+                        vx = inv * vx; vy = inv * vy; vz = inv * vz;
+                    }
+                }
+                eid = neid;
+                if (eid > 0) {
+                    if (oldtype == 0 && ntype != 0 && !C.voidtime) tof = 0.f;
+                    if (oldtype != 0 && ntype == 0) eid = 0;
+                }
+                if (eid > 0) {
+                    ctype = ntype;
+                    ctx = CTX_INNER;
+                    continue;
+                }
+            }
+
+            /* ---- exit, roulette, scattering ---- */
+            if (eid <= 0 || eid == ID_UNDEF || undef) {
+                reason = (eid == ID_UNDEF) ? MMC_END_LOST : MMC_END_EXIT;
+            } else {
+                if ((hot.flags & kFlagRoulette) && w < hot.minenergy) {
+                    const float rsize = hot.roulettesize;
+                    if (rng_next(r0, r1) * rsize <= 1.f) {
+                        float wb = w;
+                        w *= rsize;
+                        *LS(slot_rg) += w - wb;
+                    } else {
+                        reason = MMC_END_ROULETTE;
+                    }
+                }
+                if (!reason) {
+                    /* mc_next_scatter (src/mmc_mesh.c:1644-1715) */
+                    const float g = med[ctype].z;
+                    float sphi, cphi, stheta, ctheta, tmp0, tmp1;
+                    float nextslen = rng_scatlen(r0, r1);
+                    tmp0 = mmc_twopi_u((uint32_t)(rng_next(r0, r1) * 8388608.0f));
+                    mmc_sincosf(tmp0, &sphi, &cphi);
+                    if (g > EPSF) {
+                        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * rng_next(r0, r1));
+                        tmp0 *= tmp0;
+                        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+                        tmp0 = (tmp0 > 1.f) ? 1.f : tmp0;
+                        tmp0 = (tmp0 < -1.f) ? -1.f : tmp0;
+                        stheta = sqrtf(1.f - tmp0 * tmp0);
+                        ctheta = tmp0;
+                    } else {
+                        float theta = mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
+                        mmc_sincosf(theta, &stheta, &ctheta);
+                    }
+                    float qx, qy, qz;
+                    if (vz > -1.f + EPSF && vz < 1.f - EPSF) {
+                        tmp0 = 1.f - vz * vz;
+                        tmp1 = 1.f / sqrtf(tmp0);
+                        tmp1 = stheta * tmp1;
+                        qx = tmp1 * (vx * vz * cphi - vy * sphi) + vx * ctheta;
+                        qy = tmp1 * (vy * vz * cphi + vx * sphi) + vy * ctheta;
+                        qz = -tmp1 * tmp0 * cphi + vz * ctheta;
+                    } else {
+                        qx = stheta * cphi;
+                        qy = stheta * sphi;
+                        qz = (vz > 0.f) ? ctheta : -ctheta;
+                    }
+                    vx = qx; vy = qy; vz = qz;
+                    slen = nextslen;
+                    if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
+                    *LS(slot_ns) += 1.f;
+                    ctx = CTX_OUTER;
+                    continue;
+                }
+            }
+        } else if (!reason) {
+            reason = MMC_END_LOST;
+        }
+
+        /* ============================================ photon ends here ===== */
+        {
+            const SimCold& C = a.cold[sim];
+            const float w0 = *LS(slot_w0);
+            int exitdet = 0;
+            if (reason == MMC_END_EXIT && a.savedet && C.issavedet && eid <= 0) {
+                for (int i = 0; i < C.detnum; i++) {
+                    float4 d = a.detpos[C.detbase + i];
+                    if ((d.x - px) * (d.x - px) + (d.y - py) * (d.y - py) + (d.z - pz) * (d.z - pz) <
+                        d.w * d.w) {
+                        exitdet = i + 1;
+                        break;
+                    }
+                }
+                if (exitdet) {
+                    unsigned long long slot = atomicAdd(a.detcount, 1ull);
+                    atomicAdd(a.detcount + 1 + sim, 1ull);
+                    if ((int64_t)slot < a.maxdet && a.detdata) {
+                        float* row = a.detdata + (size_t)slot * a.reclen;
+                        row[0] = (float)exitdet;
+                        for (int k = 0; k < 2 * a.prop; k++) row[1 + k] = *LS(ppbase + k);
+                        int col = 1 + 2 * a.prop;
+                        if (C.issaveexit) {
+                            row[col++] = px; row[col++] = py; row[col++] = pz;
+                            row[col++] = vx; row[col++] = vy; row[col++] = vz;
+                        }
+                        row[col] = w0;
+                        if (a.detphoton) a.detphoton[slot] = gid;
+                    }
+                }
+            }
+            const float wend = (reason == MMC_END_ROULETTE) ? 0.f : w;
+            float rg = *LS(slot_rg);
+            if (reason == MMC_END_ROULETTE) rg -= w;
+            kahan_lds(LS(2), LS(3), eabs);
+            kahan_lds(LS(4), LS(5), wend);
+            *LS(6) += rg;
+            if (a.rec && gid >= a.recfirst && gid < a.recfirst + a.reccount) {
+                mmc_hip_photon* R = a.rec + (gid - a.recfirst);
+                R->p[0] = px; R->p[1] = py; R->p[2] = pz; R->w = wend;
+                R->v[0] = vx; R->v[1] = vy; R->v[2] = vz; R->t = tof;
+                R->eid = eid; R->reason = reason;
+                R->nraytet = (int32_t)(raytet - raytet_start);
+                R->detid = exitdet;
+                R->eabs = eabs; R->rgain = rg; R->w0 = w0; R->nscat = *LS(slot_ns);
+            }
+            alive = false;
+        }
+    }
+    flush_sim(sim);
+}
+
+/* FP32 -> FP64 accumulation of a finished field (the final reduction), with
+ * the compensation buffer folded in when present; zeroes the FP32 buffers. */
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        double v = (double)f[i];
+        f[i] = 0.f;
+        if (lo) {
+            v += (double)lo[i];
+            lo[i] = 0.f;
+        }
+        out[i] += v;
+    }
+}
+
+__global__ void fill_zero_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+template __global__ void transport<false, false>(KArgs);
+template __global__ void transport<false, true>(KArgs);
+template __global__ void transport<true, false>(KArgs);
+template __global__ void transport<true, true>(KArgs);
+
+}  // namespace mmc

@@ -1,0 +1,532 @@
+/*
+ * mmc_runtime.hip -- host side of the C ABI in mmc_hip.h: device mesh
+ * residency, simulation tables, the persistent launch, the FP64 final
+ * reduction and the copies back to the caller.
+ *
+ * Plays the role of mmc_run_cl (src/mmc_cl_host.c:87-1721) / mmc_run_cu
+ * (src/mmc_cu_host.cu:203-1554): buffer build, launch, read-back and
+ * accumulation into the caller's double field.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <chrono>
+#include <vector>
+
+#include "mmc_hip.h"
+#include "mmc_kernel.h"
+
+using namespace mmc;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(-100 - (int)e_, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                           \
+    } while (0)
+
+struct mmc_hip_meshres {
+    int nn, ne, prop, nf;
+    float4* planes = nullptr;
+    int4* facenb = nullptr;
+    int4* elem = nullptr;
+    int32_t* type = nullptr;
+    float4* node = nullptr;
+    float4* enode = nullptr;
+    int32_t* srcelem = nullptr;
+    int32_t srcelemlen = 0;
+    std::vector<float> med;       /* (prop+1)*4 default media */
+    int64_t bytes = 0;
+};
+
+extern "C" int32_t mmc_hip_abi_version(void) { return MMC_HIP_ABI_VERSION; }
+extern "C" const char* mmc_hip_last_error(void) { return g_err; }
+
+extern "C" int32_t mmc_hip_device_info(char* name, int32_t len, int64_t* hbm_bytes) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_err(-1, "no HIP device");
+        return -1;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return -1;
+    if (name && len > 0) snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+    return p.multiProcessorCount;
+}
+
+extern "C" void mmc_hip_photon_seeds(uint64_t seed, int64_t first, int64_t n, uint64_t* out) {
+    for (int64_t i = 0; i < n; i++) photon_seed(seed, (uint64_t)(first + i), out[2 * i], out[2 * i + 1]);
+}
+
+template <typename T>
+static int dalloc(T** p, size_t count, int64_t* bytes) {
+    size_t b = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc((void**)p, b);
+    if (e != hipSuccess)
+        return set_err(-2, "hipMalloc of %zu bytes failed: %s", b, hipGetErrorString(e));
+    if (bytes) *bytes += (int64_t)b;
+    return 0;
+}
+
+static void mesh_free(mmc_hip_meshres* m) {
+    if (!m) return;
+    (void)hipFree(m->planes);
+    (void)hipFree(m->facenb);
+    (void)hipFree(m->elem);
+    (void)hipFree(m->type);
+    (void)hipFree(m->node);
+    (void)hipFree(m->enode);
+    (void)hipFree(m->srcelem);
+    delete m;
+}
+
+extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_handle* out) {
+    if (!mesh || !out) return set_err(-1, "null argument");
+    if (mesh->ne <= 0 || mesh->nn <= 0 || !mesh->node || !mesh->elem || !mesh->type ||
+        !mesh->facenb || !mesh->planes || !mesh->med)
+        return set_err(-1, "incomplete mesh");
+    if (mesh->prop < 1 || mesh->prop > kMaxProp)
+        return set_err(-1, "media count %d outside 1..%d", mesh->prop, kMaxProp);
+    if ((int64_t)mesh->ne >= (1ll << 27) || (int64_t)mesh->nf >= (1ll << 27))
+        return set_err(-1, "mesh too large for packed face neighbours (ne=%d)", mesh->ne);
+    const size_t ne = (size_t)mesh->ne, nn = (size_t)mesh->nn;
+    /* validate ids on the host before anything reaches the device */
+    for (size_t i = 0; i < 4 * ne; i++) {
+        if (mesh->elem[i] < 1 || mesh->elem[i] > mesh->nn)
+            return set_err(-1, "elem[%zu]=%d outside 1..nn", i, mesh->elem[i]);
+        if (mesh->facenb[i] > mesh->ne || mesh->facenb[i] < -mesh->nf)
+            return set_err(-1, "facenb[%zu]=%d outside -nf..ne", i, mesh->facenb[i]);
+    }
+    for (size_t i = 0; i < ne; i++)
+        if (mesh->type[i] < 0 || mesh->type[i] > mesh->prop)
+            return set_err(-1, "type[%zu]=%d outside 0..prop", i, mesh->type[i]);
+
+    mmc_hip_meshres* m = new mmc_hip_meshres();
+    m->nn = mesh->nn;
+    m->ne = mesh->ne;
+    m->prop = mesh->prop;
+    m->nf = mesh->nf;
+    m->med.assign(mesh->med, mesh->med + 4 * (mesh->prop + 1));
+
+    std::vector<int4> fnb(ne);
+    std::vector<float4> nd(nn), en(3 * ne);
+    for (size_t i = 0; i < ne; i++) {
+        int v[4];
+        for (int j = 0; j < 4; j++) {
+            int nb = mesh->facenb[4 * i + j];
+            int t = nb > 0 ? mesh->type[nb - 1] : 0;
+            v[j] = (int)((unsigned)nb << 4) | t;
+        }
+        fnb[i] = make_int4(v[0], v[1], v[2], v[3]);
+    }
+    for (size_t i = 0; i < nn; i++)
+        nd[i] = make_float4(mesh->node[3 * i], mesh->node[3 * i + 1], mesh->node[3 * i + 2], 0.f);
+    for (size_t i = 0; i < ne; i++) {
+        float c[12];
+        for (int j = 0; j < 4; j++) {
+            const float* p = mesh->node + 3 * (size_t)(mesh->elem[4 * i + j] - 1);
+            c[3 * j] = p[0];
+            c[3 * j + 1] = p[1];
+            c[3 * j + 2] = p[2];
+        }
+        en[3 * i] = make_float4(c[0], c[1], c[2], c[3]);
+        en[3 * i + 1] = make_float4(c[4], c[5], c[6], c[7]);
+        en[3 * i + 2] = make_float4(c[8], c[9], c[10], c[11]);
+    }
+    int rc = 0;
+    if ((rc = dalloc(&m->planes, 4 * ne, &m->bytes)) || (rc = dalloc(&m->facenb, ne, &m->bytes)) ||
+        (rc = dalloc(&m->elem, ne, &m->bytes)) || (rc = dalloc(&m->type, ne, &m->bytes)) ||
+        (rc = dalloc(&m->node, nn, &m->bytes)) || (rc = dalloc(&m->enode, 3 * ne, &m->bytes)) ||
+        (rc = dalloc(&m->srcelem, (size_t)std::max(mesh->srcelemlen, 1), &m->bytes))) {
+        mesh_free(m);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    e = e ? e : hipMemcpy(m->planes, mesh->planes, 64 * ne, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(m->facenb, fnb.data(), 16 * ne, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(m->elem, mesh->elem, 16 * ne, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(m->type, mesh->type, 4 * ne, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(m->node, nd.data(), 16 * nn, hipMemcpyHostToDevice);
+    e = e ? e : hipMemcpy(m->enode, en.data(), 48 * ne, hipMemcpyHostToDevice);
+    if (!e && mesh->srcelemlen > 0) {
+        for (int i = 0; i < mesh->srcelemlen; i++)
+            if (mesh->srcelem[i] < 1 || mesh->srcelem[i] > mesh->ne) {
+                mesh_free(m);
+                return set_err(-1, "srcelem[%d] outside 1..ne", i);
+            }
+        e = hipMemcpy(m->srcelem, mesh->srcelem, 4 * (size_t)mesh->srcelemlen, hipMemcpyHostToDevice);
+    }
+    m->srcelemlen = mesh->srcelemlen;
+    if (e != hipSuccess) {
+        mesh_free(m);
+        return set_err(-3, "mesh upload failed: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_mesh_release(mmc_hip_mesh_handle h) {
+    mesh_free(h);
+    return 0;
+}
+
+extern "C" int64_t mmc_hip_mesh_bytes(mmc_hip_mesh_handle h) { return h ? h->bytes : 0; }
+
+static int64_t datalen_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
+    return s->basisorder ? m->nn : m->ne;
+}
+
+extern "C" int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* s) {
+    if (!h || !s) return -1;
+    /* FP32 working field + FP64 reduction target */
+    return datalen_of(h, s) * (int64_t)s->maxgate * 12;
+}
+
+/* Host checks of a simulation against the mesh; every id the kernel will
+ * dereference is validated here. */
+static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0) {
+    if (s->nphoton < 0) return set_err(-1, "negative photon count");
+    if (s->method != MMC_RT_PLUCKER && s->method != MMC_RT_BLBADOUEL)
+        return set_err(-1, "unsupported ray-tracer %d (use Plucker or branch-less Badouel)", s->method);
+    if (s->method != method0) return set_err(-1, "all simulations of a batch must use one ray-tracer");
+    if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1))
+        return set_err(-1, "supported pairs: Plucker with nodal output, branch-less Badouel with element output");
+    if (s->srctype != MMC_SRC_PENCIL && s->srctype != MMC_SRC_PLANAR && s->srctype != MMC_SRC_PATTERN)
+        return set_err(-1, "unsupported source type %d", s->srctype);
+    if (s->e0 < 1 || s->e0 > m->ne) return set_err(-1, "initial element %d outside 1..ne", s->e0);
+    if (s->maxgate < 1 || !(s->tstep > 0.f) || !(s->tend > s->tstart))
+        return set_err(-1, "bad time gates");
+    if (s->srctype == MMC_SRC_PATTERN) {
+        int nx = (int)s->srcparam1[3], ny = (int)s->srcparam2[3];
+        if (!s->srcpattern || nx < 1 || ny < 1) return set_err(-1, "pattern source without pattern");
+        if (s->srcnum != 1) return set_err(-1, "only srcnum=1 patterns are supported");
+    }
+    if (s->issavedet && s->detnum > 0 && !s->detpos) return set_err(-1, "detectors without positions");
+    if (s->outputtype != MMC_OUT_FLUX && s->outputtype != MMC_OUT_FLUENCE && s->outputtype != MMC_OUT_ENERGY)
+        return set_err(-1, "unsupported output type %d", s->outputtype);
+    return 0;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                                     const mmc_hip_tune* tune, mmc_hip_result* res) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!m || !sims || nsim < 1 || !res) return set_err(-1, "null argument");
+    for (int i = 0; i < nsim; i++) {
+        int rc = check_sim(m, sims + i, sims[0].method);
+        if (rc) return set_err(rc, "simulation %d: %s", i, std::string(g_err).c_str());
+    }
+    const bool nodal = sims[0].method == MMC_RT_PLUCKER;
+    const int nmed = m->prop + 1;
+    const int block = (tune && tune->block > 0) ? tune->block : kBlock;
+    if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
+    const bool comp = tune && tune->compensated;
+
+    /* ---- simulation tables ---- */
+    std::vector<SimHot> hot(nsim);
+    std::vector<SimCold> cold(nsim);
+    std::vector<float4> media((size_t)nsim * nmed);
+    std::vector<int64_t> cum(nsim + 1, 0);
+    std::vector<float4> dets;
+    std::vector<float> pats;
+    uint64_t fieldlen = 0;
+    std::vector<uint64_t> fieldoff(nsim);
+    int savedet = 0, maxrec = 0;
+    int64_t maxdet = 0;
+    for (int i = 0; i < nsim; i++) {
+        const mmc_hip_sim& s = sims[i];
+        SimHot& h = hot[i];
+        SimCold& c = cold[i];
+        memset(&h, 0, sizeof(h));
+        memset(&c, 0, sizeof(c));
+        h.tstart = s.tstart;
+        h.tend = s.tend;
+        h.rtstep = 1.f / s.tstep;
+        h.minenergy = s.minenergy;
+        h.roulettesize = s.roulettesize;
+        h.maxgate = s.maxgate;
+        /* roulette only for single-gate runs (src/mmc_raytrace.c:2033) */
+        float span = (s.tend - s.tstart) * h.rtstep;
+        h.flags = ((s.minenergy > 0.f && span <= 1.f) ? kFlagRoulette : 0) |
+                  (s.outputtype == MMC_OUT_ENERGY ? kFlagEnergy : 0) |
+                  ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0);
+        fieldoff[i] = fieldlen;
+        h.fieldoff = fieldlen;
+        fieldlen += (uint64_t)datalen_of(m, &s) * s.maxgate;
+        c.srcpos = make_float4(s.srcpos[0], s.srcpos[1], s.srcpos[2], s.srcpos[3]);
+        c.srcdir = make_float4(s.srcdir[0], s.srcdir[1], s.srcdir[2], s.srcdir[3]);
+        c.srcparam1 = make_float4(s.srcparam1[0], s.srcparam1[1], s.srcparam1[2], s.srcparam1[3]);
+        c.srcparam2 = make_float4(s.srcparam2[0], s.srcparam2[1], s.srcparam2[2], s.srcparam2[3]);
+        c.bary0 = make_float4(s.bary0[0], s.bary0[1], s.bary0[2], s.bary0[3]);
+        c.tstep = s.tstep;
+        c.nout = s.nout;
+        c.minenergy = s.minenergy;
+        c.roulettesize = s.roulettesize;
+        c.e0 = s.e0;
+        c.srctype = s.srctype;
+        c.srcnum = s.srcnum;
+        c.isreflect = s.isreflect;
+        c.isspecular = s.isspecular;
+        c.voidtime = s.voidtime;
+        c.outputtype = s.outputtype;
+        c.issavedet = (s.issavedet && s.detnum > 0);
+        c.issaveexit = c.issavedet ? s.issaveexit : 0;
+        c.detnum = c.issavedet ? s.detnum : 0;
+        c.detbase = (int)dets.size();
+        for (int d = 0; d < c.detnum; d++)
+            dets.push_back(make_float4(s.detpos[4 * d], s.detpos[4 * d + 1], s.detpos[4 * d + 2], s.detpos[4 * d + 3]));
+        if (s.srctype == MMC_SRC_PATTERN) {
+            c.patnx = (int)s.srcparam1[3];
+            c.patny = (int)s.srcparam2[3];
+            c.patbase = (int)pats.size();
+            pats.insert(pats.end(), s.srcpattern, s.srcpattern + (size_t)c.patnx * c.patny);
+        }
+        c.nphoton = s.nphoton;
+        c.photon_first = s.photon_offset;
+        c.seed = s.seed;
+        c.fieldoff = fieldlen;
+        const float* md = s.med ? s.med : m->med.data();
+        for (int k = 0; k < nmed; k++)
+            media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
+        cum[i + 1] = cum[i] + s.nphoton;
+        if (c.issavedet) {
+            savedet = 1;
+            maxrec = std::max(maxrec, 1 + 2 * m->prop + 6 * c.issaveexit + 1);
+            maxdet += s.maxdetphoton > 0 ? s.maxdetphoton : 0;
+        }
+        res[i].reclen = c.issavedet ? 1 + 2 * m->prop + 6 * c.issaveexit + 1 : 0;
+    }
+    const int64_t total = cum[nsim];
+    const int reclen = maxrec;
+
+    /* ---- device buffers ---- */
+    DevBuf dhot, dcold, dmed, dcum, ddet, dpat, dfield, dlo, dacc, ddetdata, ddetph, ddetcnt, dq, drec;
+    int64_t dummy = 0;
+    int rc;
+    if ((rc = dalloc((SimHot**)&dhot.p, nsim, &dummy)) || (rc = dalloc((SimCold**)&dcold.p, nsim, &dummy)) ||
+        (rc = dalloc((float4**)&dmed.p, media.size(), &dummy)) || (rc = dalloc((int64_t**)&dcum.p, cum.size(), &dummy)) ||
+        (rc = dalloc((float4**)&ddet.p, dets.size(), &dummy)) || (rc = dalloc((float**)&dpat.p, pats.size(), &dummy)) ||
+        (rc = dalloc((float**)&dfield.p, fieldlen, &dummy)) ||
+        (rc = dalloc((double**)&dacc.p, (size_t)nsim * kAccCount, &dummy)) ||
+        (rc = dalloc((unsigned long long**)&ddetcnt.p, (size_t)nsim + 1, &dummy)) ||
+        (rc = dalloc((unsigned long long**)&dq.p, 1, &dummy)))
+        return rc;
+    if (comp && (rc = dalloc((float**)&dlo.p, fieldlen, &dummy))) return rc;
+    if (savedet && maxdet > 0) {
+        if ((rc = dalloc((float**)&ddetdata.p, (size_t)maxdet * reclen, &dummy)) ||
+            (rc = dalloc((int64_t**)&ddetph.p, (size_t)maxdet, &dummy)))
+            return rc;
+    }
+    /* per-photon records: one window, taken from the first simulation's request */
+    int64_t recfirst = 0, reccount = 0;
+    for (int i = 0; i < nsim; i++)
+        if (res[i].photons && res[i].reccount > 0) {
+            recfirst = cum[i] + res[i].recfirst;
+            reccount = std::min(res[i].reccount, sims[i].nphoton - res[i].recfirst);
+            break;
+        }
+    if (reccount > 0 && (rc = dalloc((mmc_hip_photon**)&drec.p, (size_t)reccount, &dummy))) return rc;
+
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    HIPCHK(hipMemcpyAsync(dhot.p, hot.data(), sizeof(SimHot) * nsim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcold.p, cold.data(), sizeof(SimCold) * nsim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dmed.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dcum.p, cum.data(), sizeof(int64_t) * cum.size(), hipMemcpyHostToDevice, st));
+    if (!dets.empty())
+        HIPCHK(hipMemcpyAsync(ddet.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
+    if (!pats.empty())
+        HIPCHK(hipMemcpyAsync(dpat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+    if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+    HIPCHK(hipMemsetAsync(dacc.p, 0, sizeof(double) * nsim * kAccCount, st));
+    hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)dq.p, 1);
+    hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 64) / 64), dim3(64), 0, st,
+                       (unsigned long long*)ddetcnt.p, nsim + 1);
+
+    KArgs a;
+    memset(&a, 0, sizeof(a));
+    a.planes = m->planes;
+    a.facenb = m->facenb;
+    a.elem = m->elem;
+    a.type = m->type;
+    a.node = m->node;
+    a.enode = m->enode;
+    a.srcelem = m->srcelem;
+    a.srcelemlen = m->srcelemlen;
+    a.nn = m->nn;
+    a.ne = m->ne;
+    a.prop = m->prop;
+    a.nsim = nsim;
+    a.hot = (const SimHot*)dhot.p;
+    a.cold = (const SimCold*)dcold.p;
+    a.media = (const float4*)dmed.p;
+    a.cum = (const int64_t*)dcum.p;
+    a.detpos = (const float4*)ddet.p;
+    a.pattern = (const float*)dpat.p;
+    a.field = (float*)dfield.p;
+    a.field_lo = (float*)dlo.p;
+    a.simacc = (double*)dacc.p;
+    a.detdata = (float*)ddetdata.p;
+    a.detphoton = (int64_t*)ddetph.p;
+    a.detcount = (unsigned long long*)ddetcnt.p;
+    a.maxdet = (savedet && maxdet > 0) ? maxdet : 0;
+    a.reclen = reclen;
+    a.savedet = savedet;
+    a.rec = (mmc_hip_photon*)drec.p;
+    a.recfirst = recfirst;
+    a.reccount = reccount;
+    a.queue = (unsigned long long*)dq.p;
+    a.total = total;
+    a.chunk = 1;
+    a.maxsteps = MMC_MAX_STEPS;
+
+    const int nslots = kLaneSlots + 2 * m->prop + 3;
+    size_t shmem = sizeof(SimHot) * nsim + sizeof(float4) * (size_t)nsim * nmed +
+                   sizeof(int64_t) * (nsim + 1) + sizeof(float) * (size_t)nslots * block;
+    shmem = (shmem + 15) & ~(size_t)15;
+    if (shmem > 160 * 1024) return set_err(-1, "batch of %d simulations needs %zu B of LDS", nsim, shmem);
+
+    const void* kfn = nodal ? (comp ? (const void*)transport<true, true> : (const void*)transport<true, false>)
+                            : (comp ? (const void*)transport<false, true> : (const void*)transport<false, false>);
+    int dev = 0, ncu = 0, occ = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, shmem));
+    int bpc = (tune && tune->blocks_per_cu > 0) ? std::min(tune->blocks_per_cu, std::max(occ, 1)) : std::max(occ, 1);
+    int grid = ncu * bpc;
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    if (total > 0) {
+        if (nodal) {
+            if (comp) hipLaunchKernelGGL((transport<true, true>), dim3(grid), dim3(block), shmem, st, a);
+            else hipLaunchKernelGGL((transport<true, false>), dim3(grid), dim3(block), shmem, st, a);
+        } else {
+            if (comp) hipLaunchKernelGGL((transport<false, true>), dim3(grid), dim3(block), shmem, st, a);
+            else hipLaunchKernelGGL((transport<false, false>), dim3(grid), dim3(block), shmem, st, a);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(e1, st));
+
+    /* ---- final FP64 reduction and copies ---- */
+    std::vector<double> acc((size_t)nsim * kAccCount);
+    std::vector<unsigned long long> cnt(nsim + 1);
+    HIPCHK(hipMemcpyAsync(acc.data(), dacc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cnt.data(), ddetcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0.f;
+    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    DevBuf d64;
+    for (int i = 0; i < nsim; i++) {
+        mmc_hip_result& r = res[i];
+        const double* A = acc.data() + (size_t)i * kAccCount;
+        r.launchweight += A[kAccLaunch];
+        r.absorbweight += A[kAccAbsorb];
+        r.escapeweight += A[kAccEscape];
+        r.roulettegain += A[kAccRGain];
+        r.raytet += A[kAccRaytet];
+        r.detected += (int64_t)cnt[1 + i];
+        r.kernel_ms += kms;
+        if (r.field) {
+            uint64_t n = (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate;
+            if (!d64.p && (rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(n, 1), &dummy))) {
+                /* fall back to a larger allocation per simulation below */
+                return rc;
+            }
+            HIPCHK(hipMemsetAsync(d64.p, 0, sizeof(double) * n, st));
+            int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
+            hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
+                               (float*)dfield.p + fieldoff[i], dlo.p ? (float*)dlo.p + fieldoff[i] : nullptr,
+                               (double*)d64.p, n);
+            std::vector<double> hbuf(n);
+            HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
+            if (i + 1 < nsim) {
+                uint64_t n2 = (uint64_t)datalen_of(m, sims + i + 1) * sims[i + 1].maxgate;
+                if (n2 > n) {
+                    (void)hipFree(d64.p);
+                    d64.p = nullptr;
+                }
+            }
+        }
+    }
+    /* detected photons: saved records are split per simulation by photon id range */
+    if (savedet && maxdet > 0 && cnt[0] > 0) {
+        int64_t nsaved = std::min<int64_t>((int64_t)cnt[0], maxdet);
+        std::vector<float> dd((size_t)nsaved * reclen);
+        std::vector<int64_t> dp((size_t)nsaved);
+        HIPCHK(hipMemcpy(dd.data(), ddetdata.p, sizeof(float) * dd.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dp.data(), ddetph.p, sizeof(int64_t) * dp.size(), hipMemcpyDeviceToHost));
+        std::vector<int64_t> fill(nsim, 0);
+        for (int64_t k = 0; k < nsaved; k++) {
+            /* records carry the global photon index; cum[] maps it to its simulation */
+            int s = (int)(std::upper_bound(cum.begin(), cum.end(), dp[k]) - cum.begin()) - 1;
+            if (s < 0 || s >= nsim) continue;
+            mmc_hip_result& r = res[s];
+            if (!r.detdata || fill[s] >= sims[s].maxdetphoton) continue;
+            memcpy(r.detdata + (size_t)fill[s] * r.reclen, dd.data() + (size_t)k * reclen, sizeof(float) * r.reclen);
+            if (r.detphoton) r.detphoton[fill[s]] = sims[s].photon_offset + (dp[k] - cum[s]);
+            fill[s]++;
+        }
+    }
+    if (reccount > 0) {
+        for (int i = 0; i < nsim; i++)
+            if (res[i].photons && res[i].reccount > 0) {
+                HIPCHK(hipMemcpy(res[i].photons, drec.p, sizeof(mmc_hip_photon) * reccount, hipMemcpyDeviceToHost));
+                break;
+            }
+    }
+    double tms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    for (int i = 0; i < nsim; i++) res[i].total_ms += tms;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_run(const mmc_hip_mesh* mesh, const mmc_hip_sim* sim, const mmc_hip_tune* tune,
+                               mmc_hip_result* res) {
+    auto t0 = std::chrono::steady_clock::now();
+    mmc_hip_mesh_handle h = nullptr;
+    int rc = mmc_hip_mesh_upload(mesh, &h);
+    if (rc) return rc;
+    double before = res->total_ms;
+    rc = mmc_hip_run_batch(h, sim, 1, tune, res);
+    mmc_hip_mesh_release(h);
+    if (!rc) res->total_ms = before + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}

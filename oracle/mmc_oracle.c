@@ -1,0 +1,1044 @@
+/*
+ * mmc_oracle.c -- CPU restatement of the fangq/mmc photon-transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load this library; the product never does.
+ *
+ * This is a clean-room restatement written from the reference's semantics.
+ * It follows the reference CPU build (`sse` target: -DMMC_USE_SSE -msse4.1,
+ * glibc libm, no -ffast-math, commons/Makefile_common.mk:46-49) operation by
+ * operation, including the order of floating-point additions, because the
+ * parity tests compare single photon trajectories.  Build with
+ * -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Citations are relative to the reference root.
+ */
+#define _GNU_SOURCE
+#include "mmc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ORC_PORTABLE_MATH replaces glibc's expf/logf/sincosf/acosf with the
+ * deterministic FP32 routines of include/mmc_fp32math.h (the library the GPU
+ * kernel uses), so that single photons can be compared bit for bit; the
+ * default build keeps glibc, as the reference CPU build does. */
+#ifdef ORC_PORTABLE_MATH
+#include "mmc_fp32math.h"
+#define O_EXPF(x) mmc_expf(x)
+#define O_LOGF(x) mmc_logf(x)
+#define O_SINCOSF(x, s, c) mmc_sincosf(x, s, c)
+#define O_ACOSF(x) mmc_acosf(x)
+#else
+#define O_EXPF(x) expf(x)
+#define O_LOGF(x) logf(x)
+#define O_SINCOSF(x, s, c) sincosf(x, s, c)
+#define O_ACOSF(x) acosf(x)
+#endif
+
+#define EPS        1e-6f              /* src/mmc_rand_common.h:46 */
+#define TWO_PI     (M_PI * 2.0)       /* src/mmc_rand_common.h:45 (double) */
+#define R_C0       3.335640951981520e-12f  /* src/mmc_const.h:45 */
+#define MAX_TRIAL  3                  /* src/mmc_raytrace.h:44 */
+#define FIX_PHOTON 1e-3f              /* src/mmc_raytrace.h:45 */
+#define R_MIN_MUS  1e9f               /* src/mmc_mesh.h:72 */
+#define UNDEF_F    (3.40282347e+38F)  /* MMC_UNDEFINED, src/mmc_mesh.h:65 */
+#define VERY_BIG   1e30f              /* src/mmc_mesh.h:74 */
+
+/* local face / edge tables: src/mmc_raytrace.c:75-152, src/mmc_mesh.c:59-103 */
+static const int O_fc[4][3] = {{0, 4, 2}, {3, 5, 4}, {2, 5, 1}, {1, 3, 0}};
+static const int O_nc[4][3] = {{3, 0, 1}, {3, 1, 2}, {2, 0, 3}, {1, 0, 2}};
+static const int O_out[4][3] = {{0, 3, 1}, {3, 2, 1}, {0, 2, 3}, {0, 1, 2}};
+static const int O_facemap[4] = {2, 0, 1, 3};
+static const int O_ifacemap[4] = {1, 2, 0, 3};
+static const int O_faceorder[5] = {1, 3, 2, 0, -1};
+static const int O_ifaceorder[4] = {3, 0, 2, 1};
+static const int O_pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+/* facelist used by the face-neighbour builder, src/mmc_highorder.cpp:50 */
+static const int O_facelist[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+
+/* ------------------------------------------------------------------------ */
+/* RNG: xorshift128+ (src/mmc_rand_xorshift128p.c:55-83)                     */
+/* ------------------------------------------------------------------------ */
+
+float orc_rand_next(uint64_t t[2]) {
+    uint64_t s1 = t[0];
+    const uint64_t s0 = t[1];
+    t[0] = s0;
+    s1 ^= s1 << 23;
+    t[1] = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+    uint64_t r = t[1] + s0;
+    uint32_t lo = (uint32_t)r;                 /* u[0]: low word on little-endian */
+    uint32_t bits = 0x3F800000U | (lo >> 9);
+    float f;
+    memcpy(&f, &bits, 4);
+    return f - 1.0f;
+}
+
+void orc_rand_seed(const uint32_t seed[4], uint64_t t[2]) {
+    t[0] = (uint64_t)seed[0] << 32 | seed[1];
+    t[1] = (uint64_t)seed[2] << 32 | seed[3];
+}
+
+/* src/mmc_rand_common.h:50-51 */
+float orc_rand_scatlen(uint64_t t[2]) {
+    return -O_LOGF(orc_rand_next(t) + EPS);
+}
+
+/* ------------------------------------------------------------------------ */
+/* small vector helpers with the reference's evaluation order               */
+/* (src/mmc_mesh.h:194-357)                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float x, y, z; } v3;
+
+static inline float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 diff3(v3 a, v3 b) { v3 r = {b.x - a.x, b.y - a.y, b.z - a.z}; return r; }
+static inline v3 cross3(v3 a, v3 b) {
+    v3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+/* vec_mult_add(a,b,sa,sb): sb*b + sa*a (src/mmc_mesh.h:212-216) */
+static inline v3 mult_add(v3 a, v3 b, float sa, float sb) {
+    v3 r = {sb * b.x + sa * a.x, sb * b.y + sa * a.y, sb * b.z + sa * a.z};
+    return r;
+}
+static inline v3 node_at(const orc_mesh* m, int id1) {
+    const float* p = m->node + 3 * (size_t)(id1 - 1);
+    v3 r = {p[0], p[1], p[2]};
+    return r;
+}
+/* SSE4.1 _mm_dp_ps(a,b,0x7F): (ax*bx + ay*by) + (az*bz + 0)  (src/mmc_raytrace.c:264) */
+static inline float dpps(v3 a, v3 b) {
+    float t2 = a.x * b.x + a.y * b.y;
+    float t3 = a.z * b.z + 0.0f;
+    return t2 + t3;
+}
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* ------------------------------------------------------------------------ */
+/* mesh preparation                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* mesh_getvolume, src/mmc_mesh.c:910-950: signed volume, swap nodes 3/4 when
+ * negative, nodal volume = 1/4 of each non-void element. */
+void orc_getvolume(int nn, const float* node, int ne, int* elem, const int* type,
+                   float* evol, float* nvol) {
+    memset(nvol, 0, sizeof(float) * (size_t)nn);
+    for (int i = 0; i < ne; i++) {
+        int* ee = elem + 4 * (size_t)i;
+        const float* n0 = node + 3 * (size_t)(ee[0] - 1);
+        const float* n1 = node + 3 * (size_t)(ee[1] - 1);
+        const float* n2 = node + 3 * (size_t)(ee[2] - 1);
+        const float* n3 = node + 3 * (size_t)(ee[3] - 1);
+        float dx = n2[0] - n3[0], dy = n2[1] - n3[1], dz = n2[2] - n3[2];
+        float v = n1[0] * (n2[1] * n3[2] - n2[2] * n3[1])
+                  - n1[1] * (n2[0] * n3[2] - n2[2] * n3[0])
+                  + n1[2] * (n2[0] * n3[1] - n2[1] * n3[0]);
+        v += -n0[0] * ((n2[1] * n3[2] - n2[2] * n3[1]) + n1[1] * dz - n1[2] * dy);
+        v += +n0[1] * ((n2[0] * n3[2] - n2[2] * n3[0]) + n1[0] * dz - n1[2] * dx);
+        v += -n0[2] * ((n2[0] * n3[1] - n2[1] * n3[0]) + n1[0] * dy - n1[1] * dx);
+        v = -v;
+        if (v < 0.f) {
+            int e1 = ee[3];
+            ee[3] = ee[2];
+            ee[2] = e1;
+            v = -v;
+        }
+        v *= (1.f / 6.f);
+        evol[i] = v;
+        if (type[i] == 0) continue;
+        for (int j = 0; j < 4; j++) nvol[ee[j] - 1] += v * 0.25f;
+    }
+}
+
+/* mesh_getfacenb, src/mmc_highorder.cpp:128-159: facenb[i*4+j] is the element
+ * sharing face facelist[j] of element i (0 when on the exterior).  Restated
+ * with a sort over (sorted triple, owner) records instead of a hash map; the
+ * result is the same pairing because an interior face is shared by exactly
+ * two elements. */
+typedef struct { int a, b, c; int slot; } orc_face;
+static int cmp_face(const void* x, const void* y) {
+    const orc_face* f = (const orc_face*)x;
+    const orc_face* g = (const orc_face*)y;
+    if (f->a != g->a) return f->a < g->a ? -1 : 1;
+    if (f->b != g->b) return f->b < g->b ? -1 : 1;
+    if (f->c != g->c) return f->c < g->c ? -1 : 1;
+    return f->slot < g->slot ? -1 : (f->slot > g->slot);
+}
+void orc_getfacenb(int ne, const int* elem, int* facenb) {
+    size_t nfc = (size_t)ne * 4;
+    orc_face* fl = (orc_face*)malloc(sizeof(orc_face) * nfc);
+    for (int i = 0; i < ne; i++)
+        for (int j = 0; j < 4; j++) {
+            int v[3] = {elem[4 * i + O_facelist[j][0]], elem[4 * i + O_facelist[j][1]],
+                        elem[4 * i + O_facelist[j][2]]};
+            /* sort 3 */
+            if (v[0] > v[1]) { int t = v[0]; v[0] = v[1]; v[1] = t; }
+            if (v[1] > v[2]) { int t = v[1]; v[1] = v[2]; v[2] = t; }
+            if (v[0] > v[1]) { int t = v[0]; v[0] = v[1]; v[1] = t; }
+            orc_face* f = fl + 4 * (size_t)i + j;
+            f->a = v[0]; f->b = v[1]; f->c = v[2]; f->slot = 4 * i + j;
+        }
+    qsort(fl, nfc, sizeof(orc_face), cmp_face);
+    memset(facenb, 0, sizeof(int) * nfc);
+    for (size_t k = 0; k + 1 < nfc; k++) {
+        if (fl[k].a == fl[k + 1].a && fl[k].b == fl[k + 1].b && fl[k].c == fl[k + 1].c) {
+            facenb[fl[k].slot] = fl[k + 1].slot / 4 + 1;
+            facenb[fl[k + 1].slot] = fl[k].slot / 4 + 1;
+            k++;
+        }
+    }
+    free(fl);
+}
+
+/* tracer_build, src/mmc_mesh.c:1496-1602.  planes: per element 16 floats
+ * {nx[4], ny[4], nz[4], d[4]}, face j spanned by out[j], unit outward normal
+ * with Rn2 = 1.f/sqrt(double).  Plucker: d = n1 - n0, m = n0 x n1 per edge. */
+void orc_tracer_build(int ne, const float* node, const int* elem,
+                      float* planes, float* pd, float* pm) {
+    for (int i = 0; i < ne; i++) {
+        const int* ee = elem + 4 * (size_t)i;
+        if (pd && pm) {
+            for (int j = 0; j < 6; j++) {
+                const float* a = node + 3 * (size_t)(ee[O_pairs[j][0]] - 1);
+                const float* b = node + 3 * (size_t)(ee[O_pairs[j][1]] - 1);
+                v3 A = {a[0], a[1], a[2]}, B = {b[0], b[1], b[2]};
+                v3 d = diff3(A, B), mm = cross3(A, B);
+                float* od = pd + 18 * (size_t)i + 3 * j;
+                float* om = pm + 18 * (size_t)i + 3 * j;
+                od[0] = d.x; od[1] = d.y; od[2] = d.z;
+                om[0] = mm.x; om[1] = mm.y; om[2] = mm.z;
+            }
+        }
+        if (planes) {
+            float* vn = planes + 16 * (size_t)i;
+            for (int j = 0; j < 4; j++) {
+                const float* a = node + 3 * (size_t)(ee[O_out[j][0]] - 1);
+                const float* b = node + 3 * (size_t)(ee[O_out[j][1]] - 1);
+                const float* c = node + 3 * (size_t)(ee[O_out[j][2]] - 1);
+                v3 A = {a[0], a[1], a[2]}, B = {b[0], b[1], b[2]}, C = {c[0], c[1], c[2]};
+                v3 AB = diff3(A, B), AC = diff3(A, C);
+                v3 N = cross3(AB, AC);
+                float Rn2 = 1.f / sqrt(dot3(N, N));
+                N.x = Rn2 * N.x; N.y = Rn2 * N.y; N.z = Rn2 * N.z;
+                vn[j] = N.x;
+                vn[j + 4] = N.y;
+                vn[j + 8] = N.z;
+                vn[j + 12] = dot3(N, A);
+            }
+        }
+    }
+}
+
+/* mesh_barycentric, src/mmc_mesh.c:1170-1206 */
+int orc_barycentric(const orc_mesh* m, int e0, const float srcpos[3], float bary[4]) {
+    const int* ee = m->elem + 4 * (size_t)(e0 - 1);
+    v3 S0 = {srcpos[0], srcpos[1], srcpos[2]};
+    float s = 0.f;
+    for (int i = 0; i < 4; i++) {
+        v3 A = node_at(m, ee[O_out[i][0]]), B = node_at(m, ee[O_out[i][1]]),
+           C = node_at(m, ee[O_out[i][2]]);
+        v3 AB = diff3(A, B), AC = diff3(A, C), S = diff3(A, S0);
+        v3 N = cross3(AB, AC);
+        bary[O_facemap[i]] = -dot3(S, N);
+    }
+    for (int i = 0; i < 4; i++) {
+        if (bary[i] < 0.f) return 1;
+        s += bary[i];
+    }
+    for (int i = 0; i < 4; i++) bary[i] /= s;
+    return 0;
+}
+
+/* mesh_initelem, src/mmc_mesh.c:1060-1090: first element (index order) whose
+ * bounding box holds the source and whose barycentric weights are >= 0. */
+int orc_initelem(const orc_mesh* m, const float srcpos[3], float bary[4]) {
+    for (int i = 0; i < m->ne; i++) {
+        double pmin[3] = {VERY_BIG, VERY_BIG, VERY_BIG}, pmax[3] = {-VERY_BIG, -VERY_BIG, -VERY_BIG};
+        const int* ee = m->elem + 4 * (size_t)i;
+        for (int j = 0; j < 4; j++) {
+            const float* p = m->node + 3 * (size_t)(ee[j] - 1);
+            for (int k = 0; k < 3; k++) {
+                pmin[k] = p[k] < pmin[k] ? p[k] : pmin[k];
+                pmax[k] = p[k] > pmax[k] ? p[k] : pmax[k];
+            }
+        }
+        if (srcpos[0] <= pmax[0] && srcpos[0] >= pmin[0] && srcpos[1] <= pmax[1] &&
+            srcpos[1] >= pmin[1] && srcpos[2] <= pmax[2] && srcpos[2] >= pmin[2]) {
+            if (orc_barycentric(m, i + 1, srcpos, bary) == 0) return i + 1;
+        }
+    }
+    return 0;
+}
+
+/* mesh_getreff, src/mmc_mesh.c:2305-2335 */
+double orc_getreff(double n_in, double n_out) {
+    double oc = asin(1.0 / n_in);
+    const double count = 1000.0;
+    const double ostep = (M_PI / (2.0 * count));
+    double r_phi = 0.0, r_j = 0.0;
+    for (int i = 0; i < count; i++) {
+        double o = i * ostep, coso = cos(o), r_fres, tmp, cosop;
+        if (o < oc) {
+            cosop = n_in * sin(o);
+            cosop = sqrt(1. - cosop * cosop);
+            tmp = (n_in * cosop - n_out * coso) / (n_in * cosop + n_out * coso);
+            r_fres = 0.5 * tmp * tmp;
+            tmp = (n_in * coso - n_out * cosop) / (n_in * coso + n_out * cosop);
+            r_fres += 0.5 * tmp * tmp;
+        } else {
+            r_fres = 1.f;
+        }
+        r_phi += 2.0 * sin(o) * coso * r_fres;
+        r_j += 3.0 * sin(o) * coso * coso * r_fres;
+    }
+    r_phi *= ostep;
+    r_j *= ostep;
+    return (r_phi + r_j) / (2.0 - r_phi + r_j);
+}
+
+/* tracer_prep surface-node correction, src/mmc_mesh.c:1344-1386.  Uses the
+ * raw face-neighbour table (0 = exterior), i.e. before exterior numbering. */
+void orc_surface_nvol(int nn, int ne, const int* elem, const int* type, const int* facenb_raw,
+                      const float* med, int prop, int isreflect, float* nvol) {
+    float* Reff = (float*)calloc((size_t)prop + 1, sizeof(float));
+    if (isreflect) {
+        for (int i = 1; i <= prop; i++) {
+            for (int j = 1; j < i; j++) {
+                if (med[4 * j + 3] == med[4 * i + 3]) { Reff[i] = Reff[j]; break; }
+            }
+            if (Reff[i] == 0.f) Reff[i] = (float)orc_getreff(med[4 * i + 3], med[3]);
+        }
+    }
+    for (int i = 0; i < ne; i++) {
+        const int* ee = elem + 4 * (size_t)i;
+        const int* enb = facenb_raw + 4 * (size_t)i;
+        for (int j = 0; j < 4; j++) {
+            if (enb[j] == 0) {
+                for (int k = 0; k < 3; k++) {
+                    int nid = ee[O_out[O_ifaceorder[j]][k]] - 1;
+                    if (nvol[nid] > 0.f && type[i] >= 0)
+                        nvol[nid] *= -(2.f / (1.0 + Reff[type[i]]));
+                }
+            }
+        }
+    }
+    free(Reff);
+    for (int i = 0; i < nn; i++)
+        if (nvol[i] < 0.f) nvol[i] = -nvol[i];
+}
+
+/* tracer_prep exterior numbering, src/mmc_mesh.c:1466-1474 */
+int orc_number_exterior(int ne, int* facenb) {
+    int nf = 0;
+    for (size_t i = 0; i < (size_t)ne * 4; i++)
+        if (facenb[i] == 0) facenb[i] = -(++nf);
+    return nf;
+}
+
+/* ------------------------------------------------------------------------ */
+/* photon transport                                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    v3 p0, vec, pout;
+    float bary0[4];
+    int eid, faceid, isend, nexteid;
+    float weight, photontimer, slen0, slen, Lmove;
+    double Eabsorb;
+    unsigned int posidx, oldidx;
+    double oldweight;
+    float* ppath;
+    int pout_undef;
+} oray;
+
+typedef struct {
+    const orc_cfg* cfg;
+    const orc_mesh* m;
+    double* weight;
+    float rtstep;
+    int framelen;
+    double raytet;
+} octx;
+
+static inline const float* med_of(const orc_mesh* m, int t) { return m->med + 4 * (size_t)t; }
+
+/* deposit helper: #pragma omp atomic on the double output array */
+static inline void add_w(double* w, size_t idx, double v) {
+#pragma omp atomic
+    w[idx] += v;
+}
+
+/* branchless_badouel_raytet, src/mmc_raytrace.c:1414-1709 (element output,
+ * mmMCX, single source, no iMMC). */
+static float orc_badouel(oray* r, octx* c) {
+    const orc_cfg* cfg = c->cfg;
+    const orc_mesh* m = c->m;
+    float baryx;
+    int eid = r->eid - 1;
+    const float* N = m->planes + 16 * (size_t)eid;
+
+    r->pout_undef = 1;
+    r->faceid = -1;
+    r->isend = 0;
+
+    float T[4], S[4];
+    for (int i = 0; i < 4; i++) {
+        float t = N[i] * r->p0.x;
+        t = t + N[4 + i] * r->p0.y;
+        t = t + N[8 + i] * r->p0.z;
+        t = N[12 + i] - t;
+        float s = N[i] * r->vec.x;
+        s = s + N[4 + i] * r->vec.y;
+        s = s + N[8 + i] * r->vec.z;
+        t = t / s;
+        T[i] = (s > 0.f) ? t : 1e10f;
+        S[i] = s;
+    }
+    (void)S;
+    /* _mm_min_ps(T, movehl(T)) then min_ss of lanes 0,1 */
+    float m02 = (T[0] < T[2]) ? T[0] : T[2];
+    float m13 = (T[1] < T[3]) ? T[1] : T[3];
+    baryx = (m02 < m13) ? m02 : m13;
+    /* maskmap: highest tied lane (src/mmc_raytrace.c:152,1467) */
+    int faceidx = 4;
+    for (int i = 0; i < 4; i++)
+        if (T[i] == baryx) faceidx = i;
+    r->faceid = O_faceorder[faceidx];
+
+    if (r->faceid >= 0 && baryx >= 0) {
+        int type = m->type[eid];
+        const float* prop = med_of(m, type);
+        float mua = prop[0], mus = prop[1], n = prop[3];
+        float rc = n * R_C0;
+        float currweight = r->weight;
+        r->nexteid = m->facenb[4 * (size_t)eid + r->faceid];
+        float dlen = (mus <= EPS) ? R_MIN_MUS : r->slen / mus;
+        float Lp0 = baryx;
+        r->isend = (Lp0 > dlen);
+        r->Lmove = (r->isend) ? dlen : Lp0;
+
+        r->pout.x = r->vec.x * baryx + r->p0.x;
+        r->pout.y = r->vec.y * baryx + r->p0.y;
+        r->pout.z = r->vec.z * baryx + r->p0.z;
+        r->pout_undef = 0;
+
+        if ((int)((r->photontimer + r->Lmove * rc - cfg->tstart) * c->rtstep) > cfg->maxgate - 1) {
+            r->faceid = -2;
+            r->pout_undef = 1;
+            r->Lmove = (cfg->tend - r->photontimer) / (n * R_C0) - 1e-4f;
+        }
+        float totalloss = O_EXPF(-mua * r->Lmove);
+        r->weight *= totalloss;
+        r->slen -= r->Lmove * mus;
+
+        {
+            int framelen = c->framelen;
+            float ww = currweight - r->weight;
+            r->photontimer += r->Lmove * rc;
+            int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+            int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * framelen;
+            if (mua > 0.f) r->Eabsorb += ww;
+            if (cfg->outputtype != ORC_OT_ENERGY)
+                ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
+
+            r->p0.x = r->p0.x + r->vec.x * r->Lmove;
+            r->p0.y = r->p0.y + r->vec.y * r->Lmove;
+            r->p0.z = r->p0.z + r->vec.z * r->Lmove;
+
+            if (!cfg->basisorder) {
+                unsigned int newidx = (unsigned int)(eid + tshift);
+                r->oldidx = (r->oldidx == 0xFFFFFFFFu) ? newidx : r->oldidx;
+                if (newidx != r->oldidx) {
+                    add_w(c->weight, r->oldidx, r->oldweight);
+                    r->oldidx = newidx;
+                    r->oldweight = ww;
+                } else {
+                    r->oldweight += ww;
+                }
+                if (r->faceid == -2 || !r->isend) {
+                    add_w(c->weight, newidx, r->oldweight);
+                    r->oldweight = 0.f;
+                }
+            } else {
+                /* basisorder=1 under BLBadouel: w/3 to the exit-face nodes
+                 * (src/mmc_raytrace.c:1674-1690) */
+                const int* ee = m->elem + 4 * (size_t)eid;
+                ww *= 1.f / 3.f;
+                for (int i = 0; i < 3; i++) add_w(c->weight, ee[O_out[faceidx][i]] - 1 + tshift, ww);
+            }
+        }
+    }
+    c->raytet++;
+    if (r->faceid == -2) return 0.f;
+    return r->slen;
+}
+
+/* plucker_raytet, src/mmc_raytrace.c:227-508 (SSE4.1 inner products) */
+static float orc_plucker(oray* r, octx* c) {
+    const orc_cfg* cfg = c->cfg;
+    const orc_mesh* m = c->m;
+    int eid = r->eid - 1, faceidx = -1;
+    float w[6];
+    float baryout[4] = {0.f, 0.f, 0.f, 0.f};
+    float Lp0 = 0.f, dlen = 0.f;
+
+    r->faceid = -1;
+    r->isend = 0;
+    r->Lmove = 0.f;
+    v3 p1 = {r->p0.x + r->vec.x, r->p0.y + r->vec.y, r->p0.z + r->vec.z};
+    v3 pcrx = cross3(r->p0, p1);
+    const int* ee = m->elem + 4 * (size_t)eid;
+    int type = m->type[eid];
+    const float* prop = med_of(m, type);
+    float mua = prop[0], mus = prop[1], n = prop[3];
+    float rc = n * R_C0;
+    float currweight = r->weight;
+
+    for (int i = 0; i < 6; i++) {
+        const float* d = m->pd + 18 * (size_t)eid + 3 * i;
+        const float* mm = m->pm + 18 * (size_t)eid + 3 * i;
+        v3 D = {d[0], d[1], d[2]}, M = {mm[0], mm[1], mm[2]};
+        w[i] = dpps(r->vec, M) + dpps(pcrx, D);
+    }
+    r->pout_undef = 1;
+    for (int i = 0; i < 4; i++) {
+        if (i >= 2) w[O_fc[i][1]] = -w[O_fc[i][1]];
+        uint32_t a = fbits(w[O_fc[i][0]]), b = fbits(w[O_fc[i][1]]), cc = fbits(w[O_fc[i][2]]);
+        if ((a & 0x80000000u) & (b & 0x80000000u) & (cc ^ 0x80000000u)) {
+            float Rv = 1.f / (-w[O_fc[i][0]] - w[O_fc[i][1]] + w[O_fc[i][2]]);
+            baryout[O_nc[i][0]] = -w[O_fc[i][0]] * Rv;
+            baryout[O_nc[i][1]] = -w[O_fc[i][1]] * Rv;
+            baryout[O_nc[i][2]] = w[O_fc[i][2]] * Rv;
+            v3 P1 = node_at(m, ee[O_nc[i][0]]), P2 = node_at(m, ee[O_nc[i][1]]),
+               P3 = node_at(m, ee[O_nc[i][2]]);
+            float b1 = baryout[O_nc[i][0]], b2 = baryout[O_nc[i][1]], b3 = baryout[O_nc[i][2]];
+            r->pout.x = b1 * P1.x + b2 * P2.x + b3 * P3.x;
+            r->pout.y = b1 * P1.y + b2 * P2.y + b3 * P3.y;
+            r->pout.z = b1 * P1.z + b2 * P2.z + b3 * P3.z;
+            r->pout_undef = 0;
+            {
+                float dx = r->pout.x - r->p0.x, dy = r->pout.y - r->p0.y, dz = r->pout.z - r->p0.z;
+                Lp0 = sqrtf(dx * dx + dy * dy + dz * dz);
+            }
+            dlen = (mus <= EPS) ? R_MIN_MUS : r->slen / mus;
+            faceidx = i;
+            r->faceid = O_faceorder[i];
+            r->nexteid = m->facenb[4 * (size_t)eid + r->faceid];
+            r->isend = (Lp0 > dlen);
+            r->Lmove = (r->isend) ? dlen : Lp0;
+            break;
+        }
+    }
+    c->raytet++;
+
+    if (!r->pout_undef) {
+        if ((int)((r->photontimer + r->Lmove * rc - cfg->tstart) * c->rtstep) >=
+            (int)((cfg->tend - cfg->tstart) * c->rtstep)) {
+            r->faceid = -2;
+            r->pout_undef = 1;
+            r->Lmove = (cfg->tend - r->photontimer) / (n * R_C0) - 1e-4f;
+        }
+        r->weight *= O_EXPF(-mua * r->Lmove);
+        r->slen -= r->Lmove * mus;
+        r->p0.x += r->Lmove * r->vec.x;
+        r->p0.y += r->Lmove * r->vec.y;
+        r->p0.z += r->Lmove * r->vec.z;
+
+        if (!cfg->basisorder) {
+            float ww = currweight - r->weight;
+            r->Eabsorb += ww;
+            if (cfg->outputtype != ORC_OT_ENERGY)
+                ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
+            r->photontimer += r->Lmove * rc;
+            int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+            int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * m->ne;
+            add_w(c->weight, (size_t)eid + tshift, ww);
+        } else if (Lp0 > EPS) {
+            r->photontimer += r->Lmove * rc;
+            float ww = currweight - r->weight;
+            if (mua > 0.f || cfg->outputtype != ORC_OT_ENERGY) {
+                float ratio = r->Lmove / Lp0;
+                if (mua > 0.f) r->Eabsorb += ww;
+                if (cfg->outputtype != ORC_OT_ENERGY)
+                    ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
+                int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+                int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * m->nn;
+                ww *= 0.5f;
+                if (r->isend)
+                    for (int i = 0; i < 4; i++)
+                        baryout[i] = (1.f - ratio) * r->bary0[i] + ratio * baryout[i];
+                for (int i = 0; i < 4; i++)
+                    add_w(c->weight, (size_t)(ee[i] - 1 + tshift), ww * (r->bary0[i] + baryout[i]));
+            }
+            if (r->isend) {
+                memcpy(r->bary0, baryout, sizeof(float) * 4);
+            } else if (r->nexteid > 0 && faceidx >= 0) {
+                const int* nextenb = m->elem + 4 * (size_t)(r->nexteid - 1);
+                memset(r->bary0, 0, sizeof(float) * 4);
+                for (int j = 0; j < 3; j++)
+                    for (int k = 0; k < 4; k++)
+                        if (ee[O_nc[faceidx][j]] == nextenb[k]) {
+                            r->bary0[k] = baryout[O_nc[faceidx][j]];
+                            break;
+                        }
+            }
+        }
+        if (r->faceid == -2) return 0.f;
+    }
+    return r->slen;
+}
+
+static inline float orc_trace(oray* r, octx* c) {
+    return (c->cfg->method == ORC_RT_PLUCKER) ? orc_plucker(r, c) : orc_badouel(r, c);
+}
+
+/* fixphoton, src/mmc_raytrace.c:200-213 */
+static void orc_fixphoton(oray* r, const orc_mesh* m) {
+    const int* ee = m->elem + 4 * (size_t)(r->eid - 1);
+    v3 c0 = {0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; i++) {
+        v3 p = node_at(m, ee[i]);
+        c0.x = c0.x + p.x; c0.y = c0.y + p.y; c0.z = c0.z + p.z;
+    }
+    r->p0.x += (c0.x * 0.25f - r->p0.x) * FIX_PHOTON;
+    r->p0.y += (c0.y * 0.25f - r->p0.y) * FIX_PHOTON;
+    r->p0.z += (c0.z * 0.25f - r->p0.z) * FIX_PHOTON;
+}
+
+/* mc_next_scatter, src/mmc_mesh.c:1644-1715 (non-SSE-math build) */
+float orc_scatter(float g, float dir[3], uint64_t t[2]) {
+    float sphi, cphi, tmp0, stheta, ctheta, tmp1;
+    float nextslen = orc_rand_scatlen(t);
+    tmp0 = (float)(TWO_PI * orc_rand_next(t));
+    O_SINCOSF(tmp0, &sphi, &cphi);
+    if (g > EPS) {
+        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * orc_rand_next(t));
+        tmp0 *= tmp0;
+        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+        if (tmp0 > 1.f) tmp0 = 1.f;
+        if (tmp0 < -1.f) tmp0 = -1.f;
+        stheta = (float)sqrt(1.f - tmp0 * tmp0);
+        ctheta = tmp0;
+    } else {
+        float theta = O_ACOSF(2.f * orc_rand_next(t) - 1.f);
+        O_SINCOSF(theta, &stheta, &ctheta);
+    }
+    float px, py, pz;
+    if (dir[2] > -1.f + EPS && dir[2] < 1.f - EPS) {
+        tmp0 = 1.f - dir[2] * dir[2];
+        tmp1 = 1.f / sqrtf(tmp0);
+        tmp1 = stheta * tmp1;
+        px = tmp1 * (dir[0] * dir[2] * cphi - dir[1] * sphi) + dir[0] * ctheta;
+        py = tmp1 * (dir[1] * dir[2] * cphi + dir[0] * sphi) + dir[1] * ctheta;
+        pz = -tmp1 * tmp0 * cphi + dir[2] * ctheta;
+    } else {
+        px = stheta * cphi;
+        py = stheta * sphi;
+        pz = (dir[2] > 0.f) ? ctheta : -ctheta;
+    }
+    dir[0] = px;
+    dir[1] = py;
+    dir[2] = pz;
+    return nextslen;
+}
+
+/* reflectray, src/mmc_raytrace.c:2257-2332 (no iMMC) */
+float orc_reflect(const orc_mesh* m, int method, float nout, int isreflect, int isspecular,
+                  float dir[3], int oldeid, int* eid, int faceid, uint64_t t[2]) {
+    (void)method;
+    int offs = (oldeid - 1) << 2;
+    int f = O_ifaceorder[faceid];
+    const float* N = m->planes + 4 * (size_t)offs;
+    v3 pn = {N[f], N[f + 4], N[f + 8]};
+    v3 c0 = {dir[0], dir[1], dir[2]};
+    float Icos = fabs(dot3(c0, pn));
+    float n1 = (oldeid != *eid) ? med_of(m, m->type[oldeid - 1])[3] : nout;
+    float n2 = (*eid > 0) ? med_of(m, m->type[*eid - 1])[3] : nout;
+    float tmp0 = n1 * n1, tmp1 = n2 * n2;
+    float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+    float ret = 1.f;
+    if (tmp2 > 0.f && !(*eid <= 0 && isreflect == ORC_BC_MIRROR)) {
+        float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+        tmp2 = sqrtf(tmp2);
+        float Im = 2.f * n1 * n2 * Icos * tmp2;
+        float Rtotal = (Re - Im) / (Re + Im);
+        Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+        Rtotal = (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+        if (oldeid == *eid) return Rtotal;
+        if (orc_rand_next(t) <= Rtotal) {
+            c0 = mult_add(pn, c0, -2.f * Icos, 1.f);
+            *eid = oldeid;
+        } else if (isspecular == 2 && *eid == 0) {
+        } else {
+            c0 = mult_add(pn, c0, -Icos, 1.f);
+            c0 = mult_add(pn, c0, tmp2, n1 / n2);
+        }
+    } else {
+        c0 = mult_add(pn, c0, -2.f * Icos, 1.f);
+        *eid = oldeid;
+    }
+    tmp0 = 1.f / sqrtf(dot3(c0, c0));
+    dir[0] = tmp0 * c0.x;
+    dir[1] = tmp0 * c0.y;
+    dir[2] = tmp0 * c0.z;
+    return ret;
+}
+
+/* launchphoton, src/mmc_raytrace.c:2346-2671 for pencil, planar and pattern
+ * sources.  Returns nonzero if no candidate element encloses the start. */
+static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
+    const orc_cfg* cfg = c->cfg;
+    const orc_mesh* m = c->m;
+    r->slen = orc_rand_scatlen(ran);
+    if (cfg->srctype == ORC_ST_PENCIL) {
+        if (r->eid > 0) return 0;
+    } else if (cfg->srctype == ORC_ST_PLANAR || cfg->srctype == ORC_ST_PATTERN) {
+        float rx = orc_rand_next(ran);
+        float ry = orc_rand_next(ran);
+        r->p0.x = cfg->srcpos[0] + rx * cfg->srcparam1[0] + ry * cfg->srcparam2[0];
+        r->p0.y = cfg->srcpos[1] + rx * cfg->srcparam1[1] + ry * cfg->srcparam2[1];
+        r->p0.z = cfg->srcpos[2] + rx * cfg->srcparam1[2] + ry * cfg->srcparam2[2];
+        r->weight = 1.f;
+        if (cfg->srctype == ORC_ST_PATTERN) {
+            int xsize = (int)cfg->srcparam1[3];
+            int ysize = (int)cfg->srcparam2[3];
+            int iy = (int)(ry * ysize), ix = (int)(rx * xsize);
+            iy = iy < ysize - 1 ? iy : ysize - 1;
+            ix = ix < xsize - 1 ? ix : xsize - 1;
+            r->posidx = (unsigned int)(iy * xsize + ix);
+            r->weight = (cfg->srcnum == 1) ? cfg->srcpattern[r->posidx] : 1.f;
+        }
+    }
+    r->p0 = mult_add(r->p0, r->vec, 1.f, EPS);
+
+    for (int is = -1; is < m->srcelemlen; is++) {
+        const int* elems;
+        if (is < 0) {
+            if (r->eid >= 0) elems = m->elem + 4 * (size_t)(r->eid - 1);
+            else continue;
+        } else {
+            elems = m->elem + 4 * (size_t)(m->srcelem[is] - 1);
+        }
+        float bary[4];
+        int include = 1;
+        for (int i = 0; i < 4; i++) {
+            v3 A = node_at(m, elems[O_out[i][0]]), B = node_at(m, elems[O_out[i][1]]),
+               C = node_at(m, elems[O_out[i][2]]);
+            v3 AB = diff3(A, B), AC = diff3(A, C), S = diff3(A, r->p0);
+            v3 N = cross3(AB, AC);
+            bary[O_facemap[i]] = -dot3(S, N);
+        }
+        for (int i = 0; i < 4; i++)
+            if (bary[i] < -1e-4f) include = 0;
+        if (include) {
+            r->eid = (is >= 0 ? m->srcelem[is] : r->eid);
+            float s = 0.f;
+            for (int i = 0; i < 4; i++) s += bary[i];
+            for (int i = 0; i < 4; i++) r->bary0[i] = bary[i] / s;
+            for (int i = 0; i < 4; i++)
+                if ((bary[i] / s) < 1e-4f) r->faceid = O_ifacemap[i] + 1;
+            return 0;
+        }
+    }
+    return 1;
+}
+
+typedef struct {
+    float* buf;       /* reclen floats per detected photon */
+    int64_t* pid;
+    int n, cap, reclen;
+} odet;
+
+/* onephoton, src/mmc_raytrace.c:1772-2126 (mmMCX, no iMMC, no debug
+ * output).  Returns the termination reason. */
+static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double* launchw,
+                         double* absorbw, orc_photon* rec) {
+    const orc_cfg* cfg = c->cfg;
+    const orc_mesh* m = c->m;
+    int prop = m->prop;
+    int fixcount = 0, exitdet = 0, oldeid = 0, reason = 0;
+    int reclen = (2 + (cfg->ismomentum > 0)) * prop + (cfg->issaveexit > 0) * 6 + 2;
+    float ppath[4096];
+    double raytet0 = c->raytet;
+    float rgain = 0.f, nscat = 0.f;
+
+    oray r;
+    memset(&r, 0, sizeof(r));
+    r.p0.x = cfg->srcpos[0]; r.p0.y = cfg->srcpos[1]; r.p0.z = cfg->srcpos[2];
+    r.vec.x = cfg->srcdir[0]; r.vec.y = cfg->srcdir[1]; r.vec.z = cfg->srcdir[2];
+    memcpy(r.bary0, cfg->bary0, sizeof(float) * 4);
+    r.eid = cfg->e0;
+    r.faceid = -1;
+    r.weight = 1.f;
+    r.oldidx = 0xFFFFFFFFu;
+    r.pout_undef = 1;
+    memset(ppath, 0, sizeof(float) * (size_t)(reclen - 1));
+    r.ppath = ppath;
+
+    if (orc_launch(&r, c, ran)) return -1;
+    float w0 = r.weight;
+    ppath[reclen - 2] = r.weight;
+    *launchw += r.weight;
+
+#define PPATH_ADD(cond)                                                            \
+    do {                                                                          \
+        if (cfg->issavedet && (cond)) ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove; \
+    } while (0)
+
+    while (1) {
+        if (c->raytet - raytet0 >= ORC_MAX_STEPS) {
+            reason = ORC_END_STEPLIMIT;
+            break;
+        }
+        r.slen = orc_trace(&r, c);
+        if (r.pout_undef) {
+            if (fixcount++ < MAX_TRIAL) {
+                orc_fixphoton(&r, m);
+                continue;
+            }
+            r.eid = ORC_ID_UNDEFINED;
+            r.faceid = -1;
+        }
+        /* the reference reads type[eid-1] before testing faceid>=0; with
+         * eid=ID_UNDEFINED faceid is -1, so no path is added */
+        if (cfg->issavedet && r.Lmove > 0.f && r.faceid >= 0 && m->type[r.eid - 1] > 0)
+            ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove;
+        if (r.faceid == -2) break;
+
+        while (r.faceid >= 0 && !r.isend) {
+            r.p0 = r.pout;
+            oldeid = r.eid;
+            r.eid = m->facenb[4 * (size_t)(r.eid - 1) + r.faceid];
+            float nold = med_of(m, m->type[oldeid - 1])[3];
+            if (cfg->isreflect && (r.eid <= 0 || med_of(m, m->type[r.eid - 1])[3] != nold)) {
+                if (!(r.eid <= 0 && ((nold == cfg->nout && cfg->isreflect != ORC_BC_MIRROR) ||
+                                     cfg->isreflect == ORC_BC_ABSORBEXT))) {
+                    float dir[3] = {r.vec.x, r.vec.y, r.vec.z};
+                    orc_reflect(m, cfg->method, cfg->nout, cfg->isreflect, cfg->isspecular, dir,
+                                oldeid, &r.eid, r.faceid, ran);
+                    r.vec.x = dir[0]; r.vec.y = dir[1]; r.vec.z = dir[2];
+                }
+            }
+            if (r.eid <= 0) break;
+            if (m->type[oldeid - 1] == 0 && m->type[r.eid - 1]) {
+                if (!cfg->voidtime) r.photontimer = 0.f;
+            }
+            if (m->type[oldeid - 1] && m->type[r.eid - 1] == 0) {
+                r.eid = 0;   /* no external detectors in this restatement */
+                break;
+            }
+            r.slen = orc_trace(&r, c);
+            PPATH_ADD(r.Lmove > 0.f && m->type[r.eid - 1] > 0);
+            fixcount = 0;
+            while (r.pout_undef && fixcount++ < MAX_TRIAL) {
+                orc_fixphoton(&r, m);
+                r.slen = orc_trace(&r, c);
+                PPATH_ADD(r.Lmove > 0.f && m->type[r.eid - 1] > 0);
+            }
+            if (r.pout_undef) {
+                r.eid = ORC_ID_UNDEFINED;
+                break;
+            }
+        }
+
+        if (r.eid <= 0 || r.pout_undef) {
+            if (r.eid != ORC_ID_UNDEFINED && cfg->issavedet && cfg->issaveexit) {
+                ppath[reclen - 2 - 6] = r.p0.x;
+                ppath[reclen - 2 - 5] = r.p0.y;
+                ppath[reclen - 2 - 4] = r.p0.z;
+                ppath[reclen - 2 - 3] = r.vec.x;
+                ppath[reclen - 2 - 2] = r.vec.y;
+                ppath[reclen - 2 - 1] = r.vec.z;
+            }
+            if (cfg->issavedet && r.eid <= 0) {
+                for (int i = 0; i < cfg->detnum; i++) {
+                    const float* d = cfg->detpos + 4 * i;
+                    if ((d[0] - r.p0.x) * (d[0] - r.p0.x) + (d[1] - r.p0.y) * (d[1] - r.p0.y) +
+                            (d[2] - r.p0.z) * (d[2] - r.p0.z) <
+                        d[3] * d[3]) {
+                        exitdet = i + 1;
+                        break;
+                    }
+                }
+            }
+            reason = (r.eid == ORC_ID_UNDEFINED) ? ORC_END_LOST : ORC_END_EXIT;
+            break;
+        }
+
+        if (cfg->minenergy > 0.f && r.weight < cfg->minenergy &&
+            (cfg->tend - cfg->tstart) * c->rtstep <= 1.f) {
+            if (orc_rand_next(ran) * cfg->roulettesize <= 1.f) {
+                float wb = r.weight;
+                r.weight *= cfg->roulettesize;
+                rgain += r.weight - wb;
+            } else {
+                reason = ORC_END_ROULETTE;
+                break;
+            }
+        }
+
+        {
+            float dir[3] = {r.vec.x, r.vec.y, r.vec.z};
+            r.slen0 = orc_scatter(med_of(m, m->type[r.eid - 1])[2], dir, ran);
+            r.vec.x = dir[0]; r.vec.y = dir[1]; r.vec.z = dir[2];
+        }
+        r.slen = r.slen0;
+        if (m->type[r.eid - 1] > 0) ppath[m->type[r.eid - 1] - 1]++;
+        nscat += 1.f;
+    }
+#undef PPATH_ADD
+
+    if (cfg->issavedet && exitdet > 0 && det && det->buf) {
+        int slot;
+#pragma omp atomic capture
+        slot = det->n++;
+        if (slot < det->cap) {
+            float* row = det->buf + (size_t)slot * reclen;
+            row[0] = (float)exitdet;
+            memcpy(row + 1, ppath, sizeof(float) * (size_t)(reclen - 1));
+            if (det->pid) det->pid[slot] = id;
+        }
+    }
+    *absorbw += r.Eabsorb;
+
+    if (rec) {
+        rec->p[0] = r.p0.x; rec->p[1] = r.p0.y; rec->p[2] = r.p0.z;
+        rec->v[0] = r.vec.x; rec->v[1] = r.vec.y; rec->v[2] = r.vec.z;
+        rec->w = (reason == ORC_END_ROULETTE) ? 0.f : r.weight;
+        rec->t = r.photontimer;
+        rec->eid = r.eid;
+        rec->reason = reason;
+        rec->nraytet = (int32_t)(c->raytet - raytet0);
+        rec->detid = exitdet;
+        rec->eabs = (float)r.Eabsorb;
+        rec->rgain = (reason == ORC_END_ROULETTE) ? rgain - r.weight : rgain;
+        rec->w0 = w0;
+        rec->nscat = nscat;
+    }
+    return reason;
+}
+
+static double now_ms(void) {
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec * 1000.0 + tv.tv_usec / 1000.0;
+}
+
+/* mmc_run_mp, src/mmc_host.c:178-468: OpenMP loop over photons.  With
+ * cfg->photonseed each photon runs from its own saved xorshift128+ state (the
+ * reference's seed-replay mode, src/mmc_host.c:280-281); otherwise each thread
+ * is seeded from srand(seed); rand() words (src/mmc_host.c:238-246). */
+int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
+    octx base;
+    base.cfg = cfg;
+    base.m = mesh;
+    base.weight = out->weight;
+    base.rtstep = 1.f / cfg->tstep;
+    base.framelen = cfg->basisorder ? mesh->nn : mesh->ne;
+    base.raytet = 0.0;
+    int reclen = (2 + (cfg->ismomentum > 0)) * mesh->prop + (cfg->issaveexit > 0) * 6 + 2;
+    odet det = {out->detdata, out->detphoton, 0, out->maxdet, reclen};
+    double launch = 0.0, absorb = 0.0, raytet = 0.0;
+    int err = 0;
+    double t0 = now_ms();
+    int nth = cfg->nthreads;
+#ifdef _OPENMP
+    if (nth <= 0) nth = omp_get_max_threads();
+#else
+    nth = 1;
+#endif
+    uint32_t* seeds = NULL;
+    if (!cfg->photonseed) {
+        seeds = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (size_t)nth);
+        srand(cfg->seed);
+        for (int i = 0; i < 4 * nth; i++) seeds[i] = (uint32_t)rand();
+    }
+
+#pragma omp parallel num_threads(nth) reduction(+ : launch, absorb, raytet) reduction(| : err)
+    {
+        octx c = base;
+        uint64_t tran[2] = {0, 0};
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        if (seeds) orc_rand_seed(seeds + 4 * tid, tran);
+        /* Kahan sums per thread as in src/mmc_raytrace.c:1812-1838 */
+        double lsum = 0.0, asum = 0.0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t id = 0; id < cfg->nphoton; id++) {
+            uint64_t pran[2];
+            uint64_t* ran = tran;
+            if (cfg->photonseed) {
+                pran[0] = cfg->photonseed[2 * id];
+                pran[1] = cfg->photonseed[2 * id + 1];
+                ran = pran;
+            }
+            orc_photon* rec = NULL;
+            if (out->rec && id >= cfg->recfirst && id < cfg->recfirst + cfg->reccount)
+                rec = out->rec + (id - cfg->recfirst);
+            double lw = 0.0, aw = 0.0;
+            if (orc_onephoton(id, &c, ran, &det, &lw, &aw, rec) < 0) err = 1;
+            lsum += lw;
+            asum += aw;
+        }
+        launch += lsum;
+        absorb += asum;
+        raytet += c.raytet;
+    }
+    free(seeds);
+    out->launchweight += launch;
+    out->absorbweight += absorb;
+    out->raytet += raytet;
+    out->detected = det.n;
+    out->reclen = reclen;
+    out->runtime_ms = now_ms() - t0;
+    out->error = err;
+    return err ? -1 : 0;
+}
+
+/* mesh_normalize, src/mmc_mesh.c:2154-2279 (single source, no RF/replay). */
+double orc_normalize(const orc_cfg* cfg, const orc_mesh* mesh, double* weight, float Eabsorb,
+                     float Etotal) {
+    int datalen = cfg->basisorder ? mesh->nn : mesh->ne;
+    double energydeposit = 0.f, normalizor;
+    if (cfg->outputtype == ORC_OT_ENERGY) {
+        normalizor = 1.f / Etotal;
+        for (size_t i = 0; i < (size_t)cfg->maxgate * datalen; i++) weight[i] *= normalizor;
+        return normalizor;
+    }
+    if (cfg->basisorder) {
+        for (int i = 0; i < cfg->maxgate; i++)
+            for (int j = 0; j < datalen; j++)
+                if (mesh->nvol[j] > 0.f) weight[(size_t)i * datalen + j] /= mesh->nvol[j];
+        for (int i = 0; i < mesh->ne; i++) {
+            const int* ee = mesh->elem + 4 * (size_t)i;
+            double energyelem = 0.f;
+            for (int j = 0; j < cfg->maxgate; j++)
+                for (int k = 0; k < 4; k++) {
+                    float re_val = (float)weight[(size_t)j * mesh->nn + ee[k] - 1];
+                    energyelem += re_val;
+                }
+            energydeposit += energyelem * mesh->evol[i] * med_of(mesh, mesh->type[i])[0];
+        }
+        normalizor = Eabsorb / (Etotal * energydeposit * 0.25f);
+    } else {
+        for (int i = 0; i < datalen; i++)
+            for (int j = 0; j < cfg->maxgate; j++) energydeposit += weight[(size_t)j * datalen + i];
+        for (int i = 0; i < datalen; i++) {
+            double energyelem = mesh->evol[i] * med_of(mesh, mesh->type[i])[0];
+            for (int j = 0; j < cfg->maxgate; j++) weight[(size_t)j * datalen + i] /= energyelem;
+        }
+        normalizor = Eabsorb / (Etotal * energydeposit);
+    }
+    if (cfg->outputtype == ORC_OT_FLUX) normalizor /= cfg->tstep;
+    for (size_t i = 0; i < (size_t)cfg->maxgate * datalen; i++) weight[i] *= normalizor;
+    return normalizor;
+}
