@@ -1,0 +1,138 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X photon tracer (one JSON line on stdout).
+
+Default workload (the plain drop-in path): the 100x100x50 mm layered slab with
+a spherical inclusion, 3.0M tetrahedra, 1e8 photons per step, 50 time gates
+of 0.1 ns, element fluence, 16 disk detectors.  A step is one full simulation
+on the device-resident mesh: the persistent transport kernel plus the FP64
+final reduction of the fluence field.  The mesh upload and the host copy of
+the field are outside the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--photons N] [--mode plain|batch|all]
+
+The JSON line carries `roofline` for the transport kernel (algorithmic bytes
+per ray-tetrahedron step x steps per launch / kernel time from HIP events on
+the launch stream) and `cpu_baseline` (the CPU oracle, a restatement of the
+reference's OpenMP path, on this host's cores over a bounded photon sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_STEP_ELEM = 92.0   # 64 B face planes + 16 B face neighbours + 4 B label + 8 B FP32 atomic RMW
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None):
+    """Time the CPU oracle (OpenMP restatement of mmc_run_mp) on a bounded
+    photon sample of the same workload."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle as O
+    node, elem, et, med = mesh_arrays
+    om = O.OracleMesh(node, elem, et, med, nodal=False)
+    threads = threads or min(os.cpu_count() or 1, 16)
+    import copy
+    s = copy.copy(sim)
+    done, ms = 0, 0.0
+    chunk = 2000 * threads
+    t0 = time.time()
+    while time.time() - t0 < budget_s:
+        s.nphoton = chunk
+        s.seed = sim.seed + done
+        r = O.run(om, s, nthreads=threads)
+        done += chunk
+        ms += r.runtime_ms
+    return {"value": done / ms, "unit": "photons/ms", "cores": threads, "kind": "port",
+            "sample": f"{done} photons of the same workload (glibc-libm oracle, {threads} OpenMP threads, "
+                      f"{ms / 1000:.1f} s)"}
+
+
+def bench_plain(args):
+    import mmc_amd as M
+    from mmc_amd import workloads as W
+    from mmc_amd import meshgen
+    t = time.time()
+    arrays = meshgen.slab_layers(1.0)
+    mesh = M.TetMesh(*arrays).prepare(nodal=False)
+    res = M.ResidentMesh(mesh)
+    log(f"slab mesh: {mesh.ne} tets, prep+upload {time.time() - t:.1f} s")
+    sim = W.slab_sim(args.photons)
+    for i in range(args.warmup):
+        r = res.run_batch([sim], want_field=False)[0]
+        log(f"warmup {i}: {r.kernel_ms:.1f} ms kernel, {r.total_ms:.1f} ms total")
+    steps_ms, kern_ms, raytet, photons = [], [], 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sim.photon_offset = (i + args.warmup) * args.photons
+        ts = time.perf_counter()
+        r = res.run_batch([sim], want_field=False)[0]
+        steps_ms.append((time.perf_counter() - ts) * 1e3)
+        kern_ms.append(r.kernel_ms)
+        raytet += r.raytet
+        photons += args.photons
+        log(f"step {i}: {steps_ms[-1]:.1f} ms, kernel {r.kernel_ms:.1f} ms, {r.raytet / args.photons:.0f} steps/photon, "
+            f"absorbed {r.absorbweight / r.launchweight:.4f}, detected {r.detected}")
+    total_ms = (time.perf_counter() - t0) * 1e3
+    value = photons / total_ms
+    k_avg = float(np.mean(kern_ms))
+    steps_per_launch = raytet / args.steps
+    achieved = steps_per_launch * BYTES_PER_STEP_ELEM / (k_avg * 1e-3) / 1e9
+    out = {
+        "metric": "photons/ms on 1 GPU (plain vs batched/served) + p50/p99 solve latency",
+        "value": value, "unit": "photons/ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "slab: 100x100x50 mm, 1 mm cells x6 = 3.0M tets, 4 layers + 10 mm sphere, "
+                               f"pencil at (50,50,0), {args.photons:.0e} photons/step, 50 gates x 0.1 ns, "
+                               "element fluence (branch-less Badouel), 16 disk detectors",
+                   "photons_per_step": args.photons, "tets": int(mesh.ne), "gates": 50,
+                   "steps_per_photon": raytet / photons, "ray_tet_steps_per_ms": raytet / sum(kern_ms)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "mmc::transport<false,false>", "kernel_ms": k_avg,
+                     "bytes_per_step": BYTES_PER_STEP_ELEM, "steps_per_launch": steps_per_launch},
+    }
+    res.release()
+    return out, arrays, sim
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=int, default=100_000_000)
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "all"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): photons shard across GPUs "
+              "with disjoint seed ranges and one reduction of the fluence, which is not run here",
+              file=sys.stderr)
+        sys.exit(2)
+    out, arrays, sim = bench_plain(args)
+    if args.mode in ("batch", "all"):
+        from mmc_amd.batchbench import bench_batch
+        out["batched"] = bench_batch(args)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds)
+        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

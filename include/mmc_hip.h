@@ -97,7 +97,9 @@ typedef struct mmc_hip_photon {
 } mmc_hip_photon;
 
 typedef struct mmc_hip_result {
-    double* field;           /* datalen*maxgate, datalen = nn (basisorder 1) or ne; accumulated (+=) */
+    double* field;           /* datalen*maxgate, datalen = nn (basisorder 1) or ne; accumulated (+=);
+                                NULL keeps the field on the device (fieldsum is still filled) */
+    double fieldsum;         /* sum of the raw (unnormalized) field of this call, FP64 */
     double launchweight;     /* total launched weight   (cfg->energytot) */
     double absorbweight;     /* total absorbed weight */
     double escapeweight;     /* final weight of photons that ended by exit or lost */

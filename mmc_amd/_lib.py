@@ -66,7 +66,7 @@ assert PHOTON_DTYPE.itemsize == C.sizeof(PhotonC) == 64
 
 class ResultC(C.Structure):
     _fields_ = [
-        ("field", f64p), ("launchweight", C.c_double), ("absorbweight", C.c_double),
+        ("field", f64p), ("fieldsum", C.c_double), ("launchweight", C.c_double), ("absorbweight", C.c_double),
         ("escapeweight", C.c_double), ("roulettegain", C.c_double), ("raytet", C.c_double),
         ("detdata", f32p), ("detphoton", i64p), ("detected", C.c_int64), ("reclen", C.c_int32),
         ("photons", C.c_void_p), ("recfirst", C.c_int64), ("reccount", C.c_int64),

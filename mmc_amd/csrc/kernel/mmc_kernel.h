@@ -95,7 +95,7 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
 
 template <bool NODAL, bool COMP>
 __global__ void transport(KArgs a);
-__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n);
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum);
 __global__ void fill_zero_u64(unsigned long long* p, int n);
 
 }  // namespace mmc

@@ -677,9 +677,10 @@ __global__ void __launch_bounds__(kBlock) transport(KArgs a) {
 
 /* FP32 -> FP64 accumulation of a finished field (the final reduction), with
  * the compensation buffer folded in when present; zeroes the FP32 buffers. */
-__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n) {
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    double part = 0.0;
     for (; i < n; i += stride) {
         double v = (double)f[i];
         f[i] = 0.f;
@@ -687,8 +688,12 @@ __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n) {
             v += (double)lo[i];
             lo[i] = 0.f;
         }
-        out[i] += v;
+        out[i] = v;
+        part += v;
     }
+    /* wave reduction, one FP64 atomic per wave */
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off);
+    if ((threadIdx.x & 63) == 0 && part != 0.0) atomicAdd(sum, part);
 }
 
 __global__ void fill_zero_u64(unsigned long long* p, int n) {

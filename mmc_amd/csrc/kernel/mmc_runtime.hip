@@ -452,7 +452,16 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
-    DevBuf d64;
+    /* FP64 final reduction of every simulation's field on the device; the
+     * host copy happens only when the caller passed a field array */
+    DevBuf d64, dsum;
+    uint64_t maxn = 0;
+    for (int i = 0; i < nsim; i++) maxn = std::max<uint64_t>(maxn, (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate);
+    if ((rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(maxn, 1), &dummy)) ||
+        (rc = dalloc((double**)&dsum.p, (size_t)nsim, &dummy)))
+        return rc;
+    HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
+    std::vector<double> sums(nsim, 0.0);
     for (int i = 0; i < nsim; i++) {
         mmc_hip_result& r = res[i];
         const double* A = acc.data() + (size_t)i * kAccCount;
@@ -463,30 +472,23 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
         r.raytet += A[kAccRaytet];
         r.detected += (int64_t)cnt[1 + i];
         r.kernel_ms += kms;
+        uint64_t n = (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate;
+        if (n == 0 || sims[i].nphoton == 0) continue;
+        int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
+        hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
+                           (float*)dfield.p + fieldoff[i], dlo.p ? (float*)dlo.p + fieldoff[i] : nullptr,
+                           (double*)d64.p, n, (double*)dsum.p + i);
+        HIPCHK(hipGetLastError());
         if (r.field) {
-            uint64_t n = (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate;
-            if (!d64.p && (rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(n, 1), &dummy))) {
-                /* fall back to a larger allocation per simulation below */
-                return rc;
-            }
-            HIPCHK(hipMemsetAsync(d64.p, 0, sizeof(double) * n, st));
-            int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
-            hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
-                               (float*)dfield.p + fieldoff[i], dlo.p ? (float*)dlo.p + fieldoff[i] : nullptr,
-                               (double*)d64.p, n);
             std::vector<double> hbuf(n);
             HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
-            if (i + 1 < nsim) {
-                uint64_t n2 = (uint64_t)datalen_of(m, sims + i + 1) * sims[i + 1].maxgate;
-                if (n2 > n) {
-                    (void)hipFree(d64.p);
-                    d64.p = nullptr;
-                }
-            }
         }
     }
+    HIPCHK(hipMemcpyAsync(sums.data(), dsum.p, sizeof(double) * nsim, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < nsim; i++) res[i].fieldsum += sums[i];
     /* detected photons: saved records are split per simulation by photon id range */
     if (savedet && maxdet > 0 && cnt[0] > 0) {
         int64_t nsaved = std::min<int64_t>((int64_t)cnt[0], maxdet);

@@ -36,8 +36,11 @@ def t6_lattice(nx: int, ny: int, nz: int, h: float = 1.0, origin=(0.0, 0.0, 0.0)
         v2 = v1 + step[perm[1]]
         v3 = v2 + step[perm[2]]
         tets.append(np.stack([base + v0, base + v1, base + v2, base + v3], axis=1))
-    elem = np.stack(tets, axis=1).reshape(-1, 4) + 1
-    return node, elem.astype(np.int32)
+    elem = (np.stack(tets, axis=1).reshape(-1, 4) + 1).astype(np.int32)
+    # orient every tetrahedron positively (odd permutations come out negative)
+    neg = signed_volumes(node, elem) < 0
+    elem[neg, 2], elem[neg, 3] = elem[neg, 3].copy(), elem[neg, 2].copy()
+    return node, elem
 
 
 def centroids(node: np.ndarray, elem: np.ndarray) -> np.ndarray:

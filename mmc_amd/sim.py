@@ -134,6 +134,7 @@ def _encloses(mesh: TetMesh, pos) -> bool:
 @dataclasses.dataclass
 class SimResult:
     field: Optional[np.ndarray]
+    fieldsum: float
     launchweight: float
     absorbweight: float
     escapeweight: float
@@ -176,7 +177,7 @@ class _ResultBuf:
         r = self.c
         n = int(min(r.detected, len(self.detdata))) if self.detdata is not None else 0
         return SimResult(
-            field=self.field, launchweight=r.launchweight, absorbweight=r.absorbweight,
+            field=self.field, fieldsum=r.fieldsum, launchweight=r.launchweight, absorbweight=r.absorbweight,
             escapeweight=r.escapeweight, roulettegain=r.roulettegain, raytet=r.raytet,
             detected=int(r.detected),
             detdata=self.detdata[:n] if self.detdata is not None else None,

@@ -599,6 +599,44 @@ static inline float orc_trace(oray* r, octx* c) {
     return (c->cfg->method == ORC_RT_PLUCKER) ? orc_plucker(r, c) : orc_badouel(r, c);
 }
 
+/* One tracer call from (p, v) in element eid with remaining scattering
+ * length slen and a far time limit; out = {faceid, isend, Lmove, pout x,y,z,
+ * defined, nexteid}.  For unit tests of the ray-tetrahedron tests. */
+void orc_probe_raytet(const orc_mesh* m, int method, int eid, const float p[3], const float v[3],
+                      float slen, float out[8]) {
+    orc_cfg cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.method = method;
+    cfg.basisorder = (method == ORC_RT_PLUCKER);
+    cfg.tstart = 0.f;
+    cfg.tend = 1.f;
+    cfg.tstep = 1.f;
+    cfg.maxgate = 1;
+    size_t n = (size_t)(cfg.basisorder ? m->nn : m->ne);
+    double* scratch = (double*)calloc(n, sizeof(double));
+    octx c = {&cfg, m, scratch, 1.f, (int)n, 0.0};
+    oray r;
+    memset(&r, 0, sizeof(r));
+    r.p0.x = p[0]; r.p0.y = p[1]; r.p0.z = p[2];
+    r.vec.x = v[0]; r.vec.y = v[1]; r.vec.z = v[2];
+    r.eid = eid;
+    r.weight = 1.f;
+    r.slen = slen;
+    r.oldidx = 0xFFFFFFFFu;
+    r.bary0[0] = r.bary0[1] = r.bary0[2] = r.bary0[3] = 0.25f;
+    float px = r.p0.x, py = r.p0.y, pz = r.p0.z;
+    orc_trace(&r, &c);
+    out[0] = (float)r.faceid;
+    out[1] = (float)r.isend;
+    out[2] = r.Lmove;
+    out[3] = r.pout_undef ? px : r.pout.x;
+    out[4] = r.pout_undef ? py : r.pout.y;
+    out[5] = r.pout_undef ? pz : r.pout.z;
+    out[6] = (float)!r.pout_undef;
+    out[7] = (float)r.nexteid;
+    free(scratch);
+}
+
 /* fixphoton, src/mmc_raytrace.c:200-213 */
 static void orc_fixphoton(oray* r, const orc_mesh* m) {
     const int* ee = m->elem + 4 * (size_t)(r->eid - 1);

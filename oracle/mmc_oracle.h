@@ -125,6 +125,8 @@ double orc_normalize(const orc_cfg* cfg, const orc_mesh* mesh, double* weight,
                      float Eabsorb, float Etotal);
 
 /* single-step probes used by unit tests */
+void  orc_probe_raytet(const orc_mesh* m, int method, int eid, const float p[3], const float v[3],
+                       float slen, float out[8]);
 float orc_scatter(float g, float dir[3], uint64_t t[2]);
 float orc_reflect(const orc_mesh* m, int method, float nout, int isreflect, int isspecular,
                   float dir[3], int oldeid, int* eid, int faceid, uint64_t t[2]);

@@ -93,6 +93,8 @@ def lib(math: str = "glibc"):
         L.orc_reflect.restype = C.c_float
         L.orc_reflect.argtypes = [C.POINTER(OMesh), C.c_int, C.c_float, C.c_int, C.c_int, f32p, C.c_int,
                                   i32p, C.c_int, u64p]
+        L.orc_probe_raytet.argtypes = [C.POINTER(OMesh), C.c_int, C.c_int, f32p, f32p, C.c_float, f32p]
+        L.orc_rand_seed.argtypes = [C.POINTER(C.c_uint32), u64p]
         _libs[math] = L
     return _libs[math]
 

@@ -1,0 +1,188 @@
+"""GPU parity: the HIP transport kernel against the CPU oracle.
+
+Per-photon checks use the oracle built with the same deterministic FP32 libm
+(include/mmc_fp32math.h): every photon must end at the same element, at the
+same position within 1e-4 mm, for the same reason.  Fluence checks against
+the glibc-libm oracle (the reference's libm) are statistical: the relative L2
+difference GPU-vs-CPU must be within 1.5x the spread of two independent-seed
+CPU runs.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+import mmc_amd as M
+from mmc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def seeds(cfg):
+    return M.photon_seeds(cfg.seed, cfg.photon_offset, cfg.nphoton)
+
+
+def compare_photons(g, o):
+    same_eid = g["eid"] == o["eid"]
+    dp = np.abs(g["p"].astype(np.float64) - o["p"]).max(axis=1)
+    ok = same_eid & (dp <= 1e-4) & (g["reason"] == o["reason"])
+    print(f"photons {len(g)}: same element {same_eid.mean():.5f}, |dp|<=1e-4 {(dp <= 1e-4).mean():.5f}, "
+          f"bitwise position {(dp == 0).mean():.5f}, mismatches {int((~ok).sum())}", flush=True)
+    return ok
+
+
+@pytest.fixture(scope="module")
+def cube():
+    arrays = M.meshgen.cube60(2.0)
+    return arrays, M.TetMesh(*arrays).prepare(nodal=True), O.OracleMesh(*arrays, nodal=True)
+
+
+@pytest.fixture(scope="module")
+def slab():
+    arrays = M.meshgen.slab_layers(2.0)
+    return arrays, M.TetMesh(*arrays).prepare(nodal=False), O.OracleMesh(*arrays, nodal=False)
+
+
+def test_cube_plucker_nodal_per_photon(cube):
+    _, tm, om = cube
+    cfg = W.cube60_sim(20000)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton), normalize=True)
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), normalize=True, math="portable")
+    ok = compare_photons(g.photons, o.photons)
+    assert ok.mean() >= 0.999
+    assert np.array_equal(g.photons["nraytet"], o.photons["nraytet"])
+    assert g.raytet == o.raytet
+    rel = np.linalg.norm(g.field - o.field) / np.linalg.norm(o.field)
+    assert rel < 1e-3          # FP32 atomics vs FP64 accumulation
+    assert abs(g.normalizer / o.normalizer - 1) < 1e-4
+
+
+def test_slab_badouel_element_per_photon_and_detectors(slab):
+    _, tm, om = slab
+    cfg = W.slab_sim(20000, seed=99)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton), normalize=True)
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), normalize=True, math="portable",
+              maxdet=cfg.nphoton)
+    ok = compare_photons(g.photons, o.photons)
+    assert ok.mean() >= 0.999
+    assert g.detected == o.detected > 0
+    gi, oi = np.argsort(g.detphoton), np.argsort(o.detphoton)
+    assert np.array_equal(g.detphoton[gi], o.detphoton[oi])
+    # detector id, scattering counts, partial paths and launch weight per record
+    assert np.allclose(g.detdata[gi], o.detdata[oi], rtol=1e-5, atol=1e-5)
+    rel = np.linalg.norm(g.field - o.field) / np.linalg.norm(o.field)
+    assert rel < 1e-4
+
+
+def test_energy_ledger(slab):
+    _, tm, _ = slab
+    cfg = W.slab_sim(50000, seed=5)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    ph = g.photons
+    led = ph["w0"].astype(np.float64) - ph["eabs"] - ph["w"] + ph["rgain"]
+    assert np.abs(led).max() < 1e-5
+    total = g.absorbweight + g.escapeweight - g.roulettegain
+    assert abs(total - g.launchweight) <= 1e-5 * g.launchweight
+    # the field holds the same energy: sum of (w0 - w1)/mua deposits times mua
+    assert g.fieldsum > 0
+
+
+def test_detected_weight_follows_beer_lambert(slab):
+    arrays, tm, _ = slab
+    cfg = W.slab_sim(50000, seed=11)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    med = arrays[3].astype(np.float64)
+    prop = len(med) - 1
+    ppath = g.detdata[:, 1 + prop:1 + 2 * prop].astype(np.float64)
+    w_mbll = g.detdata[:, -1] * np.exp(-(ppath * med[1:, 0]).sum(axis=1))
+    w_end = g.photons["w"][g.detphoton]
+    assert np.allclose(w_mbll, w_end, rtol=2e-4)
+    # time of flight = sum_i n_i L_i / c
+    t = (ppath * med[1:, 3]).sum(axis=1) * 3.335640951981520e-12
+    assert np.allclose(t, g.photons["t"][g.detphoton], rtol=2e-4)
+
+
+def test_fluence_within_monte_carlo_noise_of_reference_libm(cube):
+    arrays, _, _ = cube
+    a4 = M.meshgen.cube60(4.0)
+    tm = M.TetMesh(*a4).prepare(nodal=True)
+    om = O.OracleMesh(*a4, nodal=True)
+    cfg = W.cube60_sim(100000)
+    g = M.run(tm, cfg, normalize=True)
+    c1 = O.run(om, M.SimConfig(**{**cfg.__dict__, "seed": 11}), normalize=True)
+    c2 = O.run(om, M.SimConfig(**{**cfg.__dict__, "seed": 12}), normalize=True)
+    spread = np.linalg.norm(c1.field - c2.field) / np.linalg.norm(c1.field)
+    diff = np.linalg.norm(g.field - c1.field) / np.linalg.norm(c1.field)
+    print(f"relative L2: GPU vs CPU {diff:.4g}, CPU vs CPU {spread:.4g}")
+    assert diff <= 1.5 * spread
+    assert abs(g.absorbweight / g.launchweight - c1.absorbweight / c1.launchweight) < 0.01
+
+
+def test_roulette_per_photon():
+    node, elem, et, med = M.meshgen.cube60(4.0)
+    med = med.copy()
+    med[1] = [0.3, 2.0, 0.5, 1.37]      # strong absorption: weights fall below 1e-6
+    tm = M.TetMesh(node, elem, et, med).prepare(nodal=True)
+    om = O.OracleMesh(node, elem, et, med, nodal=True)
+    cfg = M.SimConfig(nphoton=20000, method="P", srcpos=(30.1, 30.2, 0), srcdir=(0, 0, 1), tend=5e-9, tstep=5e-9,
+                      seed=4)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), math="portable")
+    assert (o.photons["reason"] == 3).sum() > 100
+    assert compare_photons(g.photons, o.photons).mean() >= 0.999
+    assert abs(g.roulettegain - o.photons["rgain"].astype(np.float64).sum()) < 1e-3 * cfg.nphoton
+
+
+@pytest.mark.parametrize("srctype", ["planar", "pattern"])
+def test_area_sources_per_photon(srctype):
+    node, elem, et, med = M.meshgen.cube60(4.0)
+    base = M.TetMesh(node, elem, et, med).prepare(nodal=False)
+    c = M.meshgen.centroids(base.node, base.elem)
+    srcelem = (np.where(c[:, 2] < 4.0)[0] + 1).astype(np.int32)
+    tm = M.TetMesh(node, elem, et, med, srcelem=srcelem).prepare(nodal=False)
+    om = O.OracleMesh(node, elem, et, med, nodal=False, srcelem=srcelem)
+    pat = np.random.default_rng(3).integers(0, 2, size=(8, 8)).astype(np.float32)
+    kw = dict(srctype=srctype, srcpos=(20.0, 20.0, 0.0), srcparam1=(20.0, 0, 0, 8), srcparam2=(0, 20.0, 0, 8))
+    if srctype == "pattern":
+        kw["srcpattern"] = pat
+    cfg = M.SimConfig(nphoton=10000, method="S", srcdir=(0, 0, 1), tend=5e-9, tstep=5e-10, seed=21,
+                      e0=int(srcelem[0]), **kw)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), math="portable")
+    assert compare_photons(g.photons, o.photons).mean() >= 0.999
+    assert abs(g.launchweight - o.launchweight) < 1e-3
+
+
+def test_batch_result_equals_result_alone(slab):
+    _, tm, _ = slab
+    sims = W.slab_batch_sims(3, seed=1, lo=2000, hi=6000)
+    rm = M.ResidentMesh(tm)
+    together = rm.run_batch(sims, records=(0, sims[0].nphoton))
+    alone = [rm.run_batch([s], records=(0, s.nphoton) if i == 0 else None)[0] for i, s in enumerate(sims)]
+    rm.release()
+    assert np.array_equal(together[0].photons, alone[0].photons)
+    for a, b in zip(together, alone):
+        assert a.launchweight == b.launchweight
+        assert abs(a.absorbweight - b.absorbweight) < 1e-6 * a.launchweight
+        assert np.allclose(a.field, b.field, rtol=1e-5, atol=1e-9 * np.abs(b.field).max())
+
+
+def test_zero_and_single_photon(slab):
+    _, tm, _ = slab
+    r0 = M.run(tm, W.slab_sim(0))
+    assert r0.launchweight == 0 and r0.fieldsum == 0
+    r1 = M.run(tm, W.slab_sim(1), records=(0, 1))
+    assert r1.launchweight == 1.0 and r1.photons["reason"][0] in (1, 2)
+
+
+def test_compensated_atomics_track_fp64_accumulation(cube):
+    _, tm, om = cube
+    cfg = W.cube60_sim(50000)
+    o = O.run(om, cfg, photonseed=seeds(cfg), math="portable")
+    plain = M.run(tm, cfg)
+    comp = M.run(tm, cfg, tune=M.sim.tune_struct(compensated=1))
+    e_plain = np.linalg.norm(plain.field - o.field) / np.linalg.norm(o.field)
+    e_comp = np.linalg.norm(comp.field - o.field) / np.linalg.norm(o.field)
+    print(f"FP32 atomics vs FP64: {e_plain:.3g}; compensated: {e_comp:.3g}")
+    assert e_comp <= e_plain
+    assert e_comp < 1e-6
