@@ -90,6 +90,11 @@ def bench_plain(args):
     k_avg = float(np.mean(kern_ms))
     steps_per_launch = raytet / args.steps
     achieved = steps_per_launch * BYTES_PER_STEP_ELEM / (k_avg * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", "transport_slab_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f)["bytes_per_step"] * steps_per_launch   # measured HBM bytes per step
     out = {
         "metric": "photons/ms on 1 GPU (plain vs batched/served) + p50/p99 solve latency",
         "value": value, "unit": "photons/ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -101,12 +106,29 @@ def bench_plain(args):
                    "photons_per_step": args.photons, "tets": int(mesh.ne), "gates": 50,
                    "steps_per_photon": raytet / photons, "ray_tet_steps_per_ms": raytet / sum(kern_ms)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "mmc::transport<false,false>", "kernel_ms": k_avg,
                      "bytes_per_step": BYTES_PER_STEP_ELEM, "steps_per_launch": steps_per_launch},
     }
     res.release()
     return out, arrays, sim
+
+
+def copy_probe(nbytes: int):
+    """Device-to-device copy of a known size through the HIP runtime: the
+    calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
 def main():
@@ -118,7 +140,11 @@ def main():
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "all"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes device-to-device")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): photons shard across GPUs "
               "with disjoint seed ranges and one reduction of the fluence, which is not run here",
