@@ -114,6 +114,72 @@ def bench_plain(args):
     return out, arrays, sim
 
 
+def bench_batch(args):
+    """The batched workload on the slab mesh: one persistent launch for a set
+    of simulations with their own pencil source, media and photon count,
+    against the plain drop-in (upload, run, free) once per simulation."""
+    import mmc_amd as M
+    from mmc_amd import autotune as AT
+    from mmc_amd import meshgen
+    from mmc_amd import workloads as W
+    mesh = M.TetMesh(*meshgen.slab_layers(1.0)).prepare(nodal=False)
+    sims = W.slab_batch_sims(256)[: args.batch_sims]
+    photons = sum(s.nphoton for s in sims)
+    rm = M.ResidentMesh(mesh)
+    t = time.perf_counter()
+    best = AT.autotune(rm, sims)
+    tune_s = time.perf_counter() - t
+    log(f"autotune ({tune_s:.1f} s): block {best['block']}, blocks/CU {best['blocks_per_cu']}, chunk {best['chunk']}")
+    t = time.perf_counter()
+    res = rm.run_batch(sims, want_field=False, tune=AT.tune_of(best))
+    batch_ms = (time.perf_counter() - t) * 1e3
+    steps = sum(r.raytet for r in res)
+    log(f"batched: {len(sims)} simulations, {photons} photons, {batch_ms:.0f} ms, {steps / photons:.0f} steps/photon")
+    rm.release()
+    t = time.perf_counter()
+    plain_steps = 0.0
+    for s in sims:
+        r = M.run(mesh, s, want_field=False)
+        plain_steps += r.raytet
+    plain_ms = (time.perf_counter() - t) * 1e3
+    log(f"plain drop-in, one call per simulation: {plain_ms:.0f} ms")
+    return {"workload": f"slab mesh (3.0M tets), first {len(sims)} of the 256-simulation batch: pencils on a "
+                        "16x16 grid of 4 mm pitch, per-tissue media from seed 7, photon counts log-uniform "
+                        "in [1e5, 1e7], 10 gates x 0.5 ns, element fluence",
+            "simulations": len(sims), "photons": photons,
+            "batched_photons_per_ms": photons / batch_ms, "plain_photons_per_ms": photons / plain_ms,
+            "speedup": plain_ms / batch_ms, "same_steps": plain_steps == steps,
+            "autotune": {k: best[k] for k in ("block", "blocks_per_cu", "chunk")}, "autotune_s": tune_s}
+
+
+def bench_serve(args):
+    """The serving workload: Poisson arrivals (200 requests/s) over three
+    resident meshes (60 mm cube 162k tets, slab 3.0M tets, jittered box
+    5.2M tets), served by the continuous-batching loop and, for comparison,
+    by the plain drop-in one request at a time."""
+    import mmc_amd as M
+    from mmc_amd import meshgen
+    from mmc_amd import serve as S
+    t = time.time()
+    meshes = {"cube": M.TetMesh(*meshgen.cube60(2.0)).prepare(nodal=False),
+              "slab": M.TetMesh(*meshgen.slab_layers(1.0)).prepare(nodal=False),
+              "jitter": M.TetMesh(*meshgen.jittered_box(120, 120, 60, 1.0, amp=0.2)).prepare(nodal=False)}
+    log(f"serving meshes ready in {time.time() - t:.1f} s: " + ", ".join(f"{k} {m.ne}" for k, m in meshes.items()))
+    hi = args.serve_max_photons
+    reqs = S.poisson_requests(meshes, args.serve_requests, rate=200.0, photons=(1e5, hi))
+    srv = S.Server(meshes, arena_bytes=64 << 30)
+    served = srv.serve(reqs)
+    srv.close()
+    log(f"served: {served}")
+    plain = S.serve_plain(meshes, S.poisson_requests(meshes, args.serve_requests, rate=200.0, photons=(1e5, hi)))
+    log(f"plain: {plain}")
+    return {"workload": f"{args.serve_requests} requests, Poisson 200/s, meshes cube 162k / slab 3.0M / "
+                        f"jittered 5.2M tets, photons log-uniform [1e5, {hi:.0e}], 1-50 gates, per-tissue media "
+                        "as the batch workload, element fluence",
+            "served": served, "plain": plain,
+            "speedup": served["photons_per_ms"] / plain["photons_per_ms"]}
+
+
 def copy_probe(nbytes: int):
     """Device-to-device copy of a known size through the HIP runtime: the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -137,8 +203,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=100_000_000)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "all"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "all"])
+    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-max-photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-sims", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes device-to-device")
     args = ap.parse_args()
@@ -152,8 +221,9 @@ def main():
         sys.exit(2)
     out, arrays, sim = bench_plain(args)
     if args.mode in ("batch", "all"):
-        from mmc_amd.batchbench import bench_batch
         out["batched"] = bench_batch(args)
+    if args.mode in ("serve", "all"):
+        out["serving"] = bench_serve(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -152,6 +152,22 @@ int32_t mmc_hip_run_batch(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_
 /* Device bytes the outputs of a simulation occupy (admission accounting). */
 int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
 
+/* ---- continuous batching (no reference counterpart): simulations are run in
+ *      photon chunks whose FP32 fields stay in a caller-managed device arena
+ *      across launches and are reduced to FP64 when a request completes ---- */
+typedef struct mmc_hip_arenares* mmc_hip_arena_handle;
+int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, mmc_hip_arena_handle* out);
+int32_t mmc_hip_arena_release(mmc_hip_arena_handle a);
+/* One persistent launch over nsim chunks; chunk i accumulates into
+ * arena[fieldoff[i] .. fieldoff[i] + datalen*maxgate).  Each sim's
+ * photon_offset selects which photons of its request the chunk runs. */
+int32_t mmc_hip_run_chunks(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_t nsim,
+                           mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                           const mmc_hip_tune* tune, mmc_hip_result* res);
+/* FP64 reduction of arena[off, off+n): adds it into host (if not NULL),
+ * returns its sum, and zeroes the region for reuse. */
+int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, uint64_t n, double* host, double* sum);
+
 #ifdef __cplusplus
 }
 #endif

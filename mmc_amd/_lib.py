@@ -94,6 +94,11 @@ SIGNATURES = {
     "mmc_hip_run_batch": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.POINTER(TuneC),
                                       C.POINTER(ResultC)]),
     "mmc_hip_sim_output_bytes": (C.c_int64, [C.c_void_p, C.POINTER(SimC)]),
+    "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "mmc_hip_arena_release": (C.c_int32, [C.c_void_p]),
+    "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
+                                       C.POINTER(TuneC), C.POINTER(ResultC)]),
+    "mmc_hip_arena_collect": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, f64p, f64p]),
     "mmc_host_volume": (None, [C.c_int32, f32p, C.c_int32, i32p, i32p, f32p, f32p]),
     "mmc_host_facenb": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p]),
     "mmc_host_planes": (None, [C.c_int32, f32p, i32p, f32p]),

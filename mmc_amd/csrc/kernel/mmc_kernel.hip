@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(kBlock) transport(KArgs a) {
     /* ---- lane state ---- */
     bool alive = false;
     bool exhausted = false;
+    int64_t next_g = 0, end_g = 0;      /* this lane's claimed photon ids */
     int sim = -1;
     int64_t gid = 0;
     float px = 0.f, py = 0.f, pz = 0.f, vx = 0.f, vy = 0.f, vz = 1.f;
@@ -167,19 +168,30 @@ __global__ void __launch_bounds__(kBlock) transport(KArgs a) {
 
     while (true) {
         /* ------------------------------------------------ refill dead lanes */
-        uint64_t deadmask = __ballot(!alive);
-        if (deadmask && !exhausted) {
-            int ndead = __popcll(deadmask);
-            int leader = __ffsll((unsigned long long)deadmask) - 1;
+        /* Lanes whose photon ended take the next id of their private range;
+         * lanes with an empty range are refilled together: one atomic per
+         * wave claims chunk ids for each of them. */
+        const bool need = !alive && next_g >= end_g;
+        const uint64_t needmask = __ballot(need);
+        if (needmask && !exhausted) {
+            const int nneed = __popcll(needmask);
+            const int leader = __ffsll((unsigned long long)needmask) - 1;
+            const unsigned long long want = (unsigned long long)nneed * (unsigned long long)a.chunk;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)ndead);
+            if (lane == leader) base = atomicAdd(a.queue, want);
             uint32_t blo = __shfl((uint32_t)base, leader);
             uint32_t bhi = __shfl((uint32_t)(base >> 32), leader);
             base = ((unsigned long long)bhi << 32) | blo;
-            if ((int64_t)(base + ndead) >= a.total) exhausted = true;
-            if (!alive) {
-                int64_t g = (int64_t)base + __popcll(deadmask & lanemask_lt);
-                if (g < a.total) {
+            if ((int64_t)(base + want) >= a.total) exhausted = true;
+            if (need) {
+                next_g = (int64_t)base + (int64_t)__popcll(needmask & lanemask_lt) * a.chunk;
+                end_g = min(next_g + (int64_t)a.chunk, a.total);
+            }
+        }
+        {
+            if (!alive && next_g < end_g) {
+                {
+                    const int64_t g = next_g++;
                     /* locate the simulation of photon g */
                     int lo = 0, hi = nsim - 1;
                     while (lo < hi) {

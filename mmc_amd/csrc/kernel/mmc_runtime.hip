@@ -233,8 +233,18 @@ struct DevBuf {
     }
 };
 
-extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
-                                     const mmc_hip_tune* tune, mmc_hip_result* res) {
+struct mmc_hip_arenares {
+    float* field = nullptr;
+    float* lo = nullptr;
+    uint64_t n = 0;
+};
+
+/* One persistent launch over nsim simulations.  Without an arena the fields
+ * are allocated here, reduced to FP64 and freed; with an arena they are
+ * accumulated in place at the caller's offsets and left there. */
+static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                            const mmc_hip_tune* tune, mmc_hip_result* res, mmc_hip_arenares* arena,
+                            const uint64_t* arena_off) {
     auto t_start = std::chrono::steady_clock::now();
     if (!m || !sims || nsim < 1 || !res) return set_err(-1, "null argument");
     for (int i = 0; i < nsim; i++) {
@@ -245,7 +255,7 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     const int nmed = m->prop + 1;
     const int block = (tune && tune->block > 0) ? tune->block : kBlock;
     if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
-    const bool comp = tune && tune->compensated;
+    const bool comp = arena ? (arena->lo != nullptr) : (tune && tune->compensated);
 
     /* ---- simulation tables ---- */
     std::vector<SimHot> hot(nsim);
@@ -275,9 +285,17 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
         h.flags = ((s.minenergy > 0.f && span <= 1.f) ? kFlagRoulette : 0) |
                   (s.outputtype == MMC_OUT_ENERGY ? kFlagEnergy : 0) |
                   ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0);
-        fieldoff[i] = fieldlen;
-        h.fieldoff = fieldlen;
-        fieldlen += (uint64_t)datalen_of(m, &s) * s.maxgate;
+        const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate;
+        if (arena) {
+            if (arena_off[i] + flen > arena->n)
+                return set_err(-1, "simulation %d: field [%llu, +%llu) outside the arena", i,
+                               (unsigned long long)arena_off[i], (unsigned long long)flen);
+            fieldoff[i] = arena_off[i];
+        } else {
+            fieldoff[i] = fieldlen;
+            fieldlen += flen;
+        }
+        h.fieldoff = fieldoff[i];
         c.srcpos = make_float4(s.srcpos[0], s.srcpos[1], s.srcpos[2], s.srcpos[3]);
         c.srcdir = make_float4(s.srcdir[0], s.srcdir[1], s.srcdir[2], s.srcdir[3]);
         c.srcparam1 = make_float4(s.srcparam1[0], s.srcparam1[1], s.srcparam1[2], s.srcparam1[3]);
@@ -309,7 +327,7 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
         c.nphoton = s.nphoton;
         c.photon_first = s.photon_offset;
         c.seed = s.seed;
-        c.fieldoff = fieldlen;
+        c.fieldoff = fieldoff[i];
         const float* md = s.med ? s.med : m->med.data();
         for (int k = 0; k < nmed; k++)
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
@@ -331,12 +349,12 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     if ((rc = dalloc((SimHot**)&dhot.p, nsim, &dummy)) || (rc = dalloc((SimCold**)&dcold.p, nsim, &dummy)) ||
         (rc = dalloc((float4**)&dmed.p, media.size(), &dummy)) || (rc = dalloc((int64_t**)&dcum.p, cum.size(), &dummy)) ||
         (rc = dalloc((float4**)&ddet.p, dets.size(), &dummy)) || (rc = dalloc((float**)&dpat.p, pats.size(), &dummy)) ||
-        (rc = dalloc((float**)&dfield.p, fieldlen, &dummy)) ||
+        (!arena && (rc = dalloc((float**)&dfield.p, fieldlen, &dummy))) ||
         (rc = dalloc((double**)&dacc.p, (size_t)nsim * kAccCount, &dummy)) ||
         (rc = dalloc((unsigned long long**)&ddetcnt.p, (size_t)nsim + 1, &dummy)) ||
         (rc = dalloc((unsigned long long**)&dq.p, 1, &dummy)))
         return rc;
-    if (comp && (rc = dalloc((float**)&dlo.p, fieldlen, &dummy))) return rc;
+    if (!arena && comp && (rc = dalloc((float**)&dlo.p, fieldlen, &dummy))) return rc;
     if (savedet && maxdet > 0) {
         if ((rc = dalloc((float**)&ddetdata.p, (size_t)maxdet * reclen, &dummy)) ||
             (rc = dalloc((int64_t**)&ddetph.p, (size_t)maxdet, &dummy)))
@@ -366,8 +384,10 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
         HIPCHK(hipMemcpyAsync(ddet.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
     if (!pats.empty())
         HIPCHK(hipMemcpyAsync(dpat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
-    if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+    if (!arena) {
+        HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+        if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+    }
     HIPCHK(hipMemsetAsync(dacc.p, 0, sizeof(double) * nsim * kAccCount, st));
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)dq.p, 1);
     hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 64) / 64), dim3(64), 0, st,
@@ -393,8 +413,8 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     a.cum = (const int64_t*)dcum.p;
     a.detpos = (const float4*)ddet.p;
     a.pattern = (const float*)dpat.p;
-    a.field = (float*)dfield.p;
-    a.field_lo = (float*)dlo.p;
+    a.field = arena ? arena->field : (float*)dfield.p;
+    a.field_lo = arena ? arena->lo : (float*)dlo.p;
     a.simacc = (double*)dacc.p;
     a.detdata = (float*)ddetdata.p;
     a.detphoton = (int64_t*)ddetph.p;
@@ -407,7 +427,7 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     a.reccount = reccount;
     a.queue = (unsigned long long*)dq.p;
     a.total = total;
-    a.chunk = 1;
+    a.chunk = (tune && tune->chunk > 0) ? tune->chunk : 1;
     a.maxsteps = MMC_MAX_STEPS;
 
     const int nslots = kLaneSlots + 2 * m->prop + 3;
@@ -455,14 +475,27 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     /* FP64 final reduction of every simulation's field on the device; the
      * host copy happens only when the caller passed a field array */
     DevBuf d64, dsum;
+    if (arena) {
+        for (int i = 0; i < nsim; i++) {
+            mmc_hip_result& r = res[i];
+            const double* A = acc.data() + (size_t)i * kAccCount;
+            r.launchweight += A[kAccLaunch];
+            r.absorbweight += A[kAccAbsorb];
+            r.escapeweight += A[kAccEscape];
+            r.roulettegain += A[kAccRGain];
+            r.raytet += A[kAccRaytet];
+            r.detected += (int64_t)cnt[1 + i];
+            r.kernel_ms += kms;
+        }
+    }
     uint64_t maxn = 0;
     for (int i = 0; i < nsim; i++) maxn = std::max<uint64_t>(maxn, (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate);
-    if ((rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(maxn, 1), &dummy)) ||
-        (rc = dalloc((double**)&dsum.p, (size_t)nsim, &dummy)))
+    if (!arena && ((rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(maxn, 1), &dummy)) ||
+                   (rc = dalloc((double**)&dsum.p, (size_t)nsim, &dummy))))
         return rc;
-    HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
+    if (!arena) HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
     std::vector<double> sums(nsim, 0.0);
-    for (int i = 0; i < nsim; i++) {
+    for (int i = 0; i < nsim && !arena; i++) {
         mmc_hip_result& r = res[i];
         const double* A = acc.data() + (size_t)i * kAccCount;
         r.launchweight += A[kAccLaunch];
@@ -486,9 +519,11 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
             for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
         }
     }
-    HIPCHK(hipMemcpyAsync(sums.data(), dsum.p, sizeof(double) * nsim, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (int i = 0; i < nsim; i++) res[i].fieldsum += sums[i];
+    if (!arena) {
+        HIPCHK(hipMemcpyAsync(sums.data(), dsum.p, sizeof(double) * nsim, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < nsim; i++) res[i].fieldsum += sums[i];
+    }
     /* detected photons: saved records are split per simulation by photon id range */
     if (savedet && maxdet > 0 && cnt[0] > 0) {
         int64_t nsaved = std::min<int64_t>((int64_t)cnt[0], maxdet);
@@ -517,6 +552,89 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     }
     double tms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     for (int i = 0; i < nsim; i++) res[i].total_ms += tms;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                                     const mmc_hip_tune* tune, mmc_hip_result* res) {
+    return launch_batch(m, sims, nsim, tune, res, nullptr, nullptr);
+}
+
+extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, mmc_hip_arena_handle* out) {
+    if (!out || nfloat == 0) return set_err(-1, "bad arena request");
+    mmc_hip_arenares* a = new mmc_hip_arenares();
+    a->n = nfloat;
+    int64_t dummy = 0;
+    int rc = dalloc(&a->field, (size_t)nfloat, &dummy);
+    if (!rc && compensated) rc = dalloc(&a->lo, (size_t)nfloat, &dummy);
+    if (!rc) {
+        hipError_t e = hipMemset(a->field, 0, sizeof(float) * nfloat);
+        if (!e && a->lo) e = hipMemset(a->lo, 0, sizeof(float) * nfloat);
+        if (e) rc = set_err(-3, "arena clear failed: %s", hipGetErrorString(e));
+    }
+    if (rc) {
+        if (a->field) (void)hipFree(a->field);
+        if (a->lo) (void)hipFree(a->lo);
+        delete a;
+        return rc;
+    }
+    *out = a;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_arena_release(mmc_hip_arena_handle a) {
+    if (!a) return 0;
+    if (a->field) (void)hipFree(a->field);
+    if (a->lo) (void)hipFree(a->lo);
+    delete a;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_run_chunks(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                                      mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                                      const mmc_hip_tune* tune, mmc_hip_result* res) {
+    if (!arena || !fieldoff) return set_err(-1, "null arena");
+    return launch_batch(m, sims, nsim, tune, res, arena, fieldoff);
+}
+
+extern "C" int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, uint64_t n, double* host,
+                                         double* sum) {
+    if (!a || off + n > a->n) return set_err(-1, "collect outside the arena");
+    if (n == 0) {
+        if (sum) *sum = 0.0;
+        return 0;
+    }
+    double *d64 = nullptr, *dsum = nullptr;
+    int64_t dummy = 0;
+    int rc = dalloc(&d64, (size_t)n, &dummy);
+    if (!rc) rc = dalloc(&dsum, 1, &dummy);
+    if (rc) {
+        if (d64) (void)hipFree(d64);
+        return rc;
+    }
+    hipError_t e = hipMemset(dsum, 0, sizeof(double));
+    int dev = 0, ncu = 0;
+    if (!e) e = hipGetDevice(&dev);
+    if (!e) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (!e) {
+        int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)std::max(ncu, 1) * 16);
+        hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, 0, a->field + off,
+                           a->lo ? a->lo + off : nullptr, d64, n, dsum);
+        e = hipGetLastError();
+    }
+    std::vector<double> hbuf;
+    double s = 0.0;
+    if (!e && host) {
+        hbuf.resize(n);
+        e = hipMemcpy(hbuf.data(), d64, sizeof(double) * n, hipMemcpyDeviceToHost);
+    }
+    if (!e) e = hipMemcpy(&s, dsum, sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d64);
+    (void)hipFree(dsum);
+    if (e) return set_err(-3, "collect failed: %s", hipGetErrorString(e));
+    if (host)
+        for (uint64_t k = 0; k < n; k++) host[k] += hbuf[k];
+    if (sum) *sum = s;
     return 0;
 }
 

@@ -186,3 +186,28 @@ def test_compensated_atomics_track_fp64_accumulation(cube):
     print(f"FP32 atomics vs FP64: {e_plain:.3g}; compensated: {e_comp:.3g}")
     assert e_comp <= e_plain
     assert e_comp < 1e-6
+
+
+def test_served_results_equal_results_alone():
+    from mmc_amd.serve import Request, Server
+    arrays = M.meshgen.slab_layers(4.0)
+    mesh = M.TetMesh(*arrays).prepare(nodal=False)
+    cube = M.TetMesh(*M.meshgen.cube60(4.0)).prepare(nodal=False)
+    meshes = {"slab": mesh, "cube": cube}
+    cfgs = [("slab", W.slab_batch_sims(3, seed=2, lo=30000, hi=120000)[i]) for i in range(3)]
+    cfgs.append(("cube", M.SimConfig(nphoton=70000, method="S", srcpos=(30.1, 30.2, 0), srcdir=(0, 0, 1),
+                                     tend=5e-9, tstep=1e-9, seed=77)))
+    reqs = [Request(rid=i, mesh=k, cfg=c, arrival=0.001 * i, want_field=True) for i, (k, c) in enumerate(cfgs)]
+    srv = Server(meshes, arena_bytes=1 << 28, slice_photons=60000)
+    srv.sched.min_chunk = 20000          # several chunks per request
+    stats = srv.serve(reqs)
+    srv.close()
+    assert stats["requests"] == len(reqs) and stats["launches"] > len(reqs)
+    for r in reqs:
+        alone = M.run(meshes[r.mesh], r.cfg)
+        assert r.launches > 1 or r.cfg.nphoton <= 20000
+        assert r.launchweight == alone.launchweight
+        assert abs(r.absorbweight - alone.absorbweight) <= 1e-6 * alone.launchweight
+        assert abs(r.raytet - alone.raytet) == 0
+        rel = np.linalg.norm(r.field - alone.field) / np.linalg.norm(alone.field)
+        assert rel < 1e-5, rel

@@ -1,0 +1,63 @@
+"""Host-side logic of the serving loop (no GPU): the field-arena allocator,
+HBM admission with a small fake budget, and chunk planning."""
+import numpy as np
+
+import mmc_amd as M
+from mmc_amd.serve import FieldArena, Request, Scheduler
+
+
+def test_arena_first_fit_and_coalescing():
+    a = FieldArena(1024)
+    x = a.alloc(100)
+    y = a.alloc(200)
+    z = a.alloc(64)
+    assert (x, y, z) == (0, 128, 384)
+    assert a.alloc(1024) == -1
+    a.release(y, 200)
+    assert a.alloc(150) == 128          # reuses the hole
+    a.release(x, 100)
+    a.release(128, 150)
+    a.release(z, 64)
+    assert a.free == [(0, 1024)] and a.available == 1024
+
+
+def mkreq(i, mesh, gates, n, t):
+    cfg = M.SimConfig(nphoton=n, method="S", tend=5e-9, tstep=5e-9 / gates)
+    return Request(rid=i, mesh=mesh, cfg=cfg, arrival=t)
+
+
+def test_admission_by_field_bytes_and_fifo_order():
+    s = Scheduler(arena_floats=10_000, datalen={"a": 1000, "b": 3000}, slice_photons=1000, min_chunk=100)
+    reqs = [mkreq(0, "a", 5, 5000, 0.0), mkreq(1, "b", 2, 500, 0.1), mkreq(2, "a", 2, 300, 0.2)]
+    for r in reqs:
+        s.arrive(r)
+    admitted = s.admit(0.3)
+    # request 0 needs 5000 floats, request 1 needs 6000: only request 0 fits
+    assert [r.rid for r in admitted] == [0]
+    assert [r.rid for r in s.waiting] == [1, 2]
+    mesh, chunks = s.plan()
+    assert mesh == "a" and [(r.rid, n) for r, n in chunks] == [(0, 1000)]
+    reqs[0].done = 5000
+    s.complete(reqs[0], 1.0)
+    admitted = s.admit(1.0)
+    assert [r.rid for r in admitted] == [1, 2]   # FIFO: 1 first, then 2 fits too
+    mesh, chunks = s.plan()
+    assert mesh == "b" and chunks[0][1] == 500
+
+
+def test_request_larger_than_arena_is_rejected():
+    s = Scheduler(arena_floats=100, datalen={"a": 1000})
+    try:
+        s.arrive(mkreq(0, "a", 1, 10, 0.0))
+    except ValueError:
+        return
+    raise AssertionError("oversized request admitted")
+
+
+def test_fair_share_of_the_slice():
+    s = Scheduler(arena_floats=10**6, datalen={"a": 10}, slice_photons=1000, min_chunk=100)
+    for i, n in enumerate([10_000, 150, 10_000, 10_000]):
+        s.arrive(mkreq(i, "a", 1, n, 0.0))
+    s.admit(0.0)
+    _, chunks = s.plan()
+    assert [n for _, n in chunks] == [250, 150, 250, 250]
