@@ -17,8 +17,8 @@ from .sim import ResidentMesh, SimConfig, tune_struct
 
 _cache: Dict[Tuple, dict] = {}
 
-BLOCKS = (128, 256)
-BLOCKS_PER_CU = (0, 2)          # 0 = as many as occupancy allows
+BLOCKS = (256,)
+BLOCKS_PER_CU = (0, 5, 6, 8)    # 0 = the default register budget; 5/6/8 select kernels compiled for that residency
 CHUNKS = (1, 8)
 
 
@@ -35,7 +35,7 @@ def sample_batch(sims: List[SimConfig], total: int) -> List[SimConfig]:
     return out
 
 
-def autotune(rm: ResidentMesh, sims: List[SimConfig], sample_photons: int = 2_000_000, verbose=False) -> dict:
+def autotune(rm: ResidentMesh, sims: List[SimConfig], sample_photons: int = 6_000_000, verbose=False) -> dict:
     key = (rm.mesh.ne, sims[0].method, min(len(sims), 512) // 16)
     if key in _cache:
         return _cache[key]

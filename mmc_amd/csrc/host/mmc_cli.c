@@ -1,0 +1,472 @@
+/*
+ * mmc_cli.c -- command-line front end of the MI355X photon tracer.
+ *
+ * Reads the reference's JSON input (Session / Forward / Domain / Mesh /
+ * Optode, src/mmc_utils.c:1566-2392) or its mesh .dat files, honours the
+ * photon-transport subset of its flags (src/mmc_utils.c:103-128,3814-4307),
+ * prepares the mesh (include/mmc_host.h), runs the HIP transport through the
+ * C ABI (include/mmc_hip.h) and writes the reference's outputs:
+ * <session>.dat (ascii, "%d\t%e" per node/element per gate, src/mmc_mesh.c:
+ * 1726-1800), <session>.bin (float64, src/mmc_utils.c:919-993) and
+ * <session>.mch (history header + detected-photon records,
+ * src/mmc_utils.c:4309-4342).
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "mmc_hip.h"
+#include "mmc_host.h"
+#include "mmc_json.h"
+
+#define MAXDET 256
+
+typedef struct {
+    mmc_hip_sim sim;
+    char session[256];
+    char meshid[512];
+    char rootdir[1024];
+    char method;           /* 'p' or 's' */
+    int isnormalized, issave2pt, outputformat; /* 0 ascii, 1 bin */
+    long seed;
+    float unitinmm;
+    /* mesh given inline in the JSON */
+    float* node;
+    int nn;
+    int* elem5;
+    int ne;
+    /* media, row 0 = background */
+    float* med;
+    int nmed;
+    float detpos[4 * MAXDET];
+    float* pattern;
+    int gpuid;
+} cli_cfg;
+
+static void die(const char* fmt, const char* arg) {
+    fprintf(stderr, "MMC ERROR: ");
+    fprintf(stderr, fmt, arg ? arg : "");
+    fprintf(stderr, "\n");
+    exit(1);
+}
+
+static void defaults(cli_cfg* c) {
+    memset(c, 0, sizeof(*c));
+    /* mcx_initcfg, src/mmc_utils.c:207-363 */
+    c->sim.nphoton = 0;
+    c->seed = 0x623F9A9E;
+    c->sim.isreflect = 1;
+    c->isnormalized = 1;
+    c->issave2pt = 1;
+    c->sim.basisorder = 1;
+    c->method = 'p';
+    c->sim.minenergy = 1e-6f;
+    c->sim.roulettesize = 10.f;
+    c->sim.nout = 1.f;
+    c->unitinmm = 1.f;
+    c->sim.outputtype = MMC_OUT_FLUX;
+    c->sim.voidtime = 1;
+    c->sim.maxdetphoton = 1000000;
+    c->sim.srcnum = 1;
+    snprintf(c->session, sizeof(c->session), "default");
+}
+
+static char* slurp(const char* path) {
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return NULL;
+    fseek(fp, 0, SEEK_END);
+    long n = ftell(fp);
+    fseek(fp, 0, SEEK_SET);
+    char* buf = (char*)malloc((size_t)n + 1);
+    if (fread(buf, 1, (size_t)n, fp) != (size_t)n) {
+        fclose(fp);
+        free(buf);
+        return NULL;
+    }
+    buf[n] = 0;
+    fclose(fp);
+    return buf;
+}
+
+static void vec_of(const mj_node* a, float* out, int n) {
+    for (int i = 0; i < n && i < mj_size(a); i++) out[i] = (float)mj_num(mj_at(a, i), 0);
+}
+
+static int keyidx(const char* s, const char* const* table) {
+    for (int i = 0; table[i][0]; i++)
+        if (!strcmp(s, table[i])) return i;
+    return -1;
+}
+
+static const char* const kSrcTypes[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern", ""};
+
+/* mcx_loadjson subset (src/mmc_utils.c:1566-2392) */
+static void load_json(cli_cfg* c, const char* path) {
+    char* text = slurp(path);
+    if (!text) die("can not read input file %s", path);
+    char err[256];
+    mj_node* root = mj_parse(text, err, sizeof(err));
+    free(text);
+    if (!root) die("%s", err);
+    const mj_node *v, *Session = mj_get(root, "Session"), *Forward = mj_get(root, "Forward"),
+                      *Domain = mj_get(root, "Domain"), *Mesh = mj_get(root, "Mesh"), *Optode = mj_get(root, "Optode");
+    if (!Mesh) Mesh = mj_get(root, "Shapes");
+    if (Mesh) {
+        if ((v = mj_get(Mesh, "MeshID"))) snprintf(c->meshid, sizeof(c->meshid), "%s", mj_str(v, ""));
+        if ((v = mj_get(Mesh, "MeshNode")) && v->type == MJ_ARRAY) {
+            c->nn = mj_size(v);
+            c->node = (float*)malloc(sizeof(float) * 3 * (size_t)c->nn);
+            int i = 0;
+            for (const mj_node* r = v->child; r; r = r->next, i++) {
+                if (mj_size(r) != 3) die("%s", "Each element in MeshNode must have 3 numbers");
+                vec_of(r, c->node + 3 * i, 3);
+            }
+        }
+        if ((v = mj_get(Mesh, "MeshElem")) && v->type == MJ_ARRAY) {
+            c->ne = mj_size(v);
+            c->elem5 = (int*)malloc(sizeof(int) * 5 * (size_t)c->ne);
+            int i = 0;
+            for (const mj_node* r = v->child; r; r = r->next, i++) {
+                if (mj_size(r) != 5) die("%s", "Each element in MeshElem must have 5 integers");
+                for (int k = 0; k < 5; k++) c->elem5[5 * i + k] = (int)mj_num(mj_at(r, k), 0);
+            }
+        }
+        v = mj_get(Mesh, "InitElem");
+        if (!v) die("%s", "InitElem must be given");
+        c->sim.e0 = (int)mj_num(v, 0);
+        if ((v = mj_get(Mesh, "LengthUnit"))) c->unitinmm = (float)mj_num(v, 1);
+    }
+    if (Domain) {
+        const mj_node* meds = mj_get(Domain, "Media");
+        if (meds && meds->type == MJ_ARRAY) {
+            c->nmed = mj_size(meds);
+            c->med = (float*)calloc(4 * (size_t)c->nmed, sizeof(float));
+            int i = 0;
+            for (const mj_node* m = meds->child; m; m = m->next, i++) {
+                if (m->type == MJ_OBJECT) {
+                    const char* names[4] = {"mua", "mus", "g", "n"};
+                    for (int k = 0; k < 4; k++) {
+                        const mj_node* x = mj_get(m, names[k]);
+                        if (!x) die("medium is missing field %s", names[k]);
+                        c->med[4 * i + k] = (float)mj_num(x, 0);
+                    }
+                } else if (m->type == MJ_ARRAY && mj_size(m) >= 4) {
+                    vec_of(m, c->med + 4 * i, 4);
+                } else {
+                    die("%s", "Domain.Media must be an array of objects or of 4-element arrays");
+                }
+            }
+        }
+        if ((v = mj_get(Domain, "LengthUnit"))) c->unitinmm = (float)mj_num(v, 1);
+        if ((v = mj_get(Domain, "MeshID"))) snprintf(c->meshid, sizeof(c->meshid), "%s", mj_str(v, ""));
+    }
+    if (Optode) {
+        const mj_node* src = mj_get(Optode, "Source");
+        if (src) {
+            if ((v = mj_get(src, "Pos"))) vec_of(v, c->sim.srcpos, 4);
+            if ((v = mj_get(src, "Dir"))) vec_of(v, c->sim.srcdir, 4);
+            if ((v = mj_get(src, "Type"))) {
+                int t = keyidx(mj_str(v, ""), kSrcTypes);
+                if (t < 0) die("unsupported source type %s", mj_str(v, ""));
+                c->sim.srctype = t;
+            }
+            if ((v = mj_get(src, "Param1"))) vec_of(v, c->sim.srcparam1, 4);
+            if ((v = mj_get(src, "Param2"))) vec_of(v, c->sim.srcparam2, 4);
+            if ((v = mj_get(src, "Pattern"))) {
+                const mj_node* data = v->type == MJ_OBJECT ? mj_get(v, "Data") : v;
+                int n = mj_size(data);
+                c->pattern = (float*)calloc((size_t)(n ? n : 1), sizeof(float));
+                vec_of(data, c->pattern, n);
+                c->sim.srcpattern = c->pattern;
+            }
+        }
+        const mj_node* dets = mj_get(Optode, "Detector");
+        if (dets && dets->type == MJ_ARRAY) {
+            int i = 0;
+            for (const mj_node* d = dets->child; d && i < MAXDET; d = d->next, i++) {
+                vec_of(mj_get(d, "Pos"), c->detpos + 4 * i, 3);
+                c->detpos[4 * i + 3] = (float)mj_num(mj_get(d, "R"), 1.0);
+            }
+            c->sim.detnum = i;
+        }
+    }
+    if (Session) {
+        if ((v = mj_get(Session, "Photons"))) c->sim.nphoton = (int64_t)mj_num(v, 0);
+        c->seed = (long)mj_num(mj_get(Session, "RNGSeed"), -1);
+        if ((v = mj_get(Session, "ID"))) snprintf(c->session, sizeof(c->session), "%s", mj_str(v, "default"));
+        c->sim.isreflect = (int)mj_num(mj_get(Session, "DoMismatch"), c->sim.isreflect);
+        c->issave2pt = (int)mj_num(mj_get(Session, "DoSaveVolume"), c->issave2pt);
+        c->isnormalized = (int)mj_num(mj_get(Session, "DoNormalize"), c->isnormalized);
+        c->sim.issavedet = (int)mj_num(mj_get(Session, "DoPartialPath"), c->sim.issavedet);
+        c->sim.isspecular = (int)mj_num(mj_get(Session, "DoSpecular"), c->sim.isspecular);
+        c->sim.issaveexit = (int)mj_num(mj_get(Session, "DoSaveExit"), c->sim.issaveexit);
+        c->sim.basisorder = (int)mj_num(mj_get(Session, "BasisOrder"), c->sim.basisorder);
+        if ((v = mj_get(Session, "OutputFormat"))) c->outputformat = !strcmp(mj_str(v, "ascii"), "bin");
+        if ((v = mj_get(Session, "RayTracer"))) c->method = mj_str(v, "p")[0] | 0x20;
+        if ((v = mj_get(Session, "OutputType"))) {
+            char t = mj_str(v, "x")[0] | 0x20;
+            c->sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
+        }
+    }
+    if (Forward) {
+        c->sim.tstart = (float)mj_num(mj_get(Forward, "T0"), 0);
+        c->sim.tend = (float)mj_num(mj_get(Forward, "T1"), 0);
+        c->sim.tstep = (float)mj_num(mj_get(Forward, "Dt"), 0);
+        c->sim.nout = (float)mj_num(mj_get(Forward, "N0"), c->sim.nout);
+    }
+    mj_free(root);
+}
+
+/* ------------------------------------------------------------------ mesh files */
+static FILE* open_mesh_file(const cli_cfg* c, const char* pattern) {
+    char name[2048], path[4096];
+    snprintf(name, sizeof(name), pattern, c->meshid);
+    snprintf(path, sizeof(path), "%s%s", c->rootdir, name);
+    FILE* fp = fopen(path, "rt");
+    if (!fp) fp = fopen(name, "rt");
+    return fp;
+}
+
+static void load_mesh_files(cli_cfg* c) {
+    int a, b;
+    FILE* fp = open_mesh_file(c, "node_%s.dat");
+    if (!fp) die("can not open node file for mesh %s", c->meshid);
+    if (fscanf(fp, "%d %d", &a, &c->nn) != 2 || c->nn <= 0) die("%s", "node file has wrong format");
+    c->node = (float*)malloc(sizeof(float) * 3 * (size_t)c->nn);
+    for (int i = 0; i < c->nn; i++)
+        if (fscanf(fp, "%d %f %f %f", &a, c->node + 3 * i, c->node + 3 * i + 1, c->node + 3 * i + 2) != 4)
+            die("%s", "node file has wrong format");
+    fclose(fp);
+    fp = open_mesh_file(c, "elem_%s.dat");
+    if (!fp) die("can not open element file for mesh %s", c->meshid);
+    if (fscanf(fp, "%d %d", &a, &c->ne) != 2 || c->ne <= 0) die("%s", "element file has wrong format");
+    c->elem5 = (int*)malloc(sizeof(int) * 5 * (size_t)c->ne);
+    for (int i = 0; i < c->ne; i++) {
+        int* e = c->elem5 + 5 * i;
+        if (fscanf(fp, "%d %d %d %d %d %d", &a, e, e + 1, e + 2, e + 3, e + 4) != 6)
+            die("%s", "element file has wrong format");
+    }
+    fclose(fp);
+    if (!c->med) {
+        fp = open_mesh_file(c, "prop_%s.dat");
+        if (!fp) die("can not open property file for mesh %s", c->meshid);
+        if (fscanf(fp, "%d %d", &a, &b) != 2 || b <= 0) die("%s", "property file has wrong format");
+        c->nmed = b + 1;
+        c->med = (float*)calloc(4 * (size_t)c->nmed, sizeof(float));
+        for (int i = 1; i <= b; i++)
+            if (fscanf(fp, "%d %f %f %f %f", &a, c->med + 4 * i, c->med + 4 * i + 1, c->med + 4 * i + 2,
+                       c->med + 4 * i + 3) != 5)
+                die("%s", "property file has wrong format");
+        fclose(fp);
+    }
+}
+
+static void usage(void) {
+    printf(
+        "mmc (MI355X HIP backend) -- mesh Monte Carlo photon transport\n"
+        "usage: mmc -f input.json [options]\n"
+        "  -f/--input FILE       JSON input (Session/Forward/Domain/Mesh/Optode)\n"
+        "  -n/--photon N         photons to simulate\n"
+        "  -E/--seed N           RNG seed (negative: time-based)\n"
+        "  -s/--session NAME     output file prefix\n"
+        "  -M/--method [P|S]     Plucker (nodal, -C 1) or branch-less Badouel (element, -C 0)\n"
+        "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
+        "  -O/--outputtype [X|F|E] flux, fluence or energy deposit\n"
+        "  -F/--outputformat [ascii|bin]\n"
+        "  -b/--reflect [0|1|2|3] boundary: no reflect, Fresnel, absorb exterior, mirror\n"
+        "  -U/--normalize [0|1]  normalize the output\n"
+        "  -S/--save2pt [0|1]    save the fluence field\n"
+        "  -d/--savedet [0|1]    save detected photons (.mch)\n"
+        "  -x/--saveexit [0|1]   save exit position and direction of detected photons\n"
+        "  -H/--maxdetphoton N   detected-photon buffer size\n"
+        "  -e/--minenergy W      roulette threshold\n"
+        "  -V/--specular [0|1]   specular reflection at launch\n"
+        "  -k/--voidtime [0|1]   count time in void elements\n"
+        "  -u/--unitinmm U       mesh length unit in mm\n"
+        "  -G/--gpu 1            the GPU (one GPU only)\n"
+        "  -c/--compute hip      backend (hip only)\n");
+}
+
+static const char* opt_value(int argc, char** argv, int* i) {
+    if (*i + 1 >= argc) die("missing value for %s", argv[*i]);
+    return argv[++*i];
+}
+
+int main(int argc, char** argv) {
+    cli_cfg c;
+    defaults(&c);
+    const char* input = NULL;
+    for (int i = 1; i < argc; i++)
+        if (!strcmp(argv[i], "-f") || !strcmp(argv[i], "--input")) input = (i + 1 < argc) ? argv[i + 1] : NULL;
+    if (argc == 1 || !input) {
+        for (int i = 1; i < argc; i++)
+            if (!strcmp(argv[i], "-h") || !strcmp(argv[i], "--help")) {
+                usage();
+                return 0;
+            }
+        usage();
+        return input ? 0 : 1;
+    }
+    load_json(&c, input);
+    {
+        const char* slash = strrchr(input, '/');
+        if (slash) snprintf(c.rootdir, sizeof(c.rootdir), "%.*s/", (int)(slash - input), input);
+    }
+    /* flags given on the command line win over the file */
+    for (int i = 1; i < argc; i++) {
+        const char* a = argv[i];
+        if (!strcmp(a, "-f") || !strcmp(a, "--input")) i++;
+        else if (!strcmp(a, "-n") || !strcmp(a, "--photon")) c.sim.nphoton = (int64_t)atof(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-E") || !strcmp(a, "--seed")) c.seed = atol(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-s") || !strcmp(a, "--session")) snprintf(c.session, sizeof(c.session), "%s", opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-M") || !strcmp(a, "--method")) c.method = opt_value(argc, argv, &i)[0] | 0x20;
+        else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
+            char t = opt_value(argc, argv, &i)[0] | 0x20;
+            c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
+        } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) c.outputformat = !strcmp(opt_value(argc, argv, &i), "bin");
+        else if (!strcmp(a, "-b") || !strcmp(a, "--reflect")) c.sim.isreflect = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-U") || !strcmp(a, "--normalize")) c.isnormalized = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-S") || !strcmp(a, "--save2pt")) c.issave2pt = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-d") || !strcmp(a, "--savedet")) c.sim.issavedet = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-x") || !strcmp(a, "--saveexit")) c.sim.issaveexit = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-H") || !strcmp(a, "--maxdetphoton")) c.sim.maxdetphoton = atol(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-e") || !strcmp(a, "--minenergy")) c.sim.minenergy = (float)atof(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-V") || !strcmp(a, "--specular")) c.sim.isspecular = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-k") || !strcmp(a, "--voidtime")) c.sim.voidtime = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-u") || !strcmp(a, "--unitinmm")) c.unitinmm = (float)atof(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-G") || !strcmp(a, "--gpu")) c.gpuid = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-c") || !strcmp(a, "--compute")) {
+            const char* b = opt_value(argc, argv, &i);
+            if (strcmp(b, "hip")) die("this build has only the hip backend (asked for %s)", b);
+        } else if (!strcmp(a, "-h") || !strcmp(a, "--help")) {
+            usage();
+            return 0;
+        } else die("unknown or unsupported option %s", a);
+    }
+    if (c.gpuid < 0) die("%s", "the CPU path (-G -1) is not part of this build");
+    if (c.sim.nphoton <= 0) die("%s", "cfg.nphoton must be a positive number");
+    if (c.sim.tstart > c.sim.tend || c.sim.tstep == 0.f) die("%s", "incorrect time gate settings or missing tstart/tend/tstep fields");
+    /* mcx_validatecfg, src/mmc_utils.c:3500-3530 */
+    if (c.sim.tstep > c.sim.tend - c.sim.tstart) c.sim.tstep = c.sim.tend - c.sim.tstart;
+    c.sim.maxgate = (int)((c.sim.tend - c.sim.tstart) / c.sim.tstep + 0.5);
+    c.sim.tend = c.sim.tstart + c.sim.tstep * c.sim.maxgate;
+    if (c.seed < 0) c.seed = (long)time(NULL);
+    c.sim.seed = (uint64_t)(uint32_t)c.seed;
+    c.sim.method = (c.method == 'p') ? MMC_RT_PLUCKER : (c.method == 's') ? MMC_RT_BLBADOUEL : -1;
+    if (c.sim.method < 0) die("%s", "only -M P and -M S are available in this build");
+    if (c.sim.issavedet && c.sim.detnum == 0) c.sim.issavedet = 0;
+    if (!c.sim.issavedet) c.sim.issaveexit = 0;
+    c.sim.detpos = c.detpos;
+
+    if (!c.node) load_mesh_files(&c);
+    if (!c.med) die("%s", "no media given (Domain.Media or prop_<id>.dat)");
+    c.med[0] = 0.f;
+    c.med[1] = 0.f;
+    c.med[2] = 1.f;
+    c.med[3] = c.sim.nout;
+    if (c.unitinmm != 1.f)
+        for (int i = 1; i < c.nmed; i++) {
+            c.med[4 * i] *= c.unitinmm;
+            c.med[4 * i + 1] *= c.unitinmm;
+        }
+    int prop = c.nmed - 1;
+
+    /* mmc_prep: volumes, neighbours, surface correction, exterior faces, planes */
+    int32_t* elem = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)c.ne);
+    int32_t* type = (int32_t*)malloc(sizeof(int32_t) * (size_t)c.ne);
+    for (int i = 0; i < c.ne; i++) {
+        for (int k = 0; k < 4; k++) elem[4 * i + k] = c.elem5[5 * i + k];
+        type[i] = c.elem5[5 * i + 4];
+        if (type[i] < 0 || type[i] > prop) die("%s", "element labels outside the media list");
+    }
+    float* evol = (float*)malloc(sizeof(float) * (size_t)c.ne);
+    float* nvol = (float*)malloc(sizeof(float) * (size_t)c.nn);
+    int32_t* facenb = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)c.ne);
+    float* planes = (float*)malloc(sizeof(float) * 16 * (size_t)c.ne);
+    mmc_host_volume(c.nn, c.node, c.ne, elem, type, evol, nvol);
+    if (mmc_host_facenb(c.nn, c.ne, elem, facenb)) die("%s", "mesh has element node ids outside 1..nn");
+    if (c.sim.basisorder && c.isnormalized == 1)
+        mmc_host_surface_nvol(c.nn, c.ne, elem, type, facenb, c.med, prop, c.sim.isreflect, nvol);
+    int nf = mmc_host_number_exterior(c.ne, facenb);
+    mmc_host_planes(c.ne, c.node, elem, planes);
+    if (c.sim.srctype == MMC_SRC_PENCIL) {
+        if (c.sim.e0 <= 0 || c.sim.e0 > c.ne ||
+            mmc_host_barycentric(c.node, elem, c.sim.e0, c.sim.srcpos, c.sim.bary0)) {
+            c.sim.e0 = mmc_host_initelem(c.ne, c.node, elem, c.sim.srcpos, c.sim.bary0);
+            if (c.sim.e0 <= 0) die("%s", "initial element does not enclose the source!");
+        }
+    }
+    mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med, NULL, 0};
+    int datalen = c.sim.basisorder ? c.nn : c.ne;
+    double* field = (double*)calloc((size_t)datalen * c.sim.maxgate, sizeof(double));
+    mmc_hip_result res;
+    memset(&res, 0, sizeof(res));
+    res.field = field;
+    int reclen = 2 * prop + 6 * (c.sim.issaveexit > 0) + 2;
+    if (c.sim.issavedet) {
+        res.detdata = (float*)calloc((size_t)c.sim.maxdetphoton * reclen, sizeof(float));
+        res.reclen = reclen;
+    }
+    char dev[256] = "";
+    mmc_hip_device_info(dev, sizeof(dev), NULL);
+    printf("simulating %lld photons on %s (seed %u, method %c, %d gates)\n", (long long)c.sim.nphoton, dev,
+           (unsigned)c.sim.seed, c.method == 'p' ? 'P' : 'S', c.sim.maxgate);
+    if (mmc_hip_run(&mesh, &c.sim, NULL, &res) != 0) die("%s", mmc_hip_last_error());
+    printf("speed ...\t%.2f photon/ms, %.0f ray-tetrahedron tests (%.2f test/ms)\n",
+           (double)c.sim.nphoton / res.total_ms, res.raytet, res.raytet / res.kernel_ms);
+    double normalizer = 1.0;
+    if (c.isnormalized) {
+        mmc_hip_sim s = c.sim;
+        normalizer = mmc_host_normalize(&s, c.nn, c.ne, elem, type, c.med, evol, nvol, field, res.absorbweight,
+                                        res.launchweight);
+    }
+    printf("total simulated energy: %.0f\tabsorbed: %5.5f%%\tnormalizor=%g\n", res.launchweight,
+           100.0 * res.absorbweight / res.launchweight, normalizer);
+    char fname[1024];
+    if (c.issave2pt) {
+        if (c.outputformat) {
+            snprintf(fname, sizeof(fname), "%s.bin", c.session);
+            if (mmc_host_save_bin(fname, field, (int64_t)datalen * c.sim.maxgate)) die("can not write %s", fname);
+        } else {
+            snprintf(fname, sizeof(fname), "%s.dat", c.session);
+            if (mmc_host_save_dat(fname, field, datalen, c.sim.maxgate)) die("can not write %s", fname);
+        }
+    }
+    if (c.sim.issavedet) {
+        int64_t saved = res.detected < c.sim.maxdetphoton ? res.detected : c.sim.maxdetphoton;
+        mmc_mch_header his;
+        memset(&his, 0, sizeof(his));
+        memcpy(his.magic, "MCXH", 4);
+        his.version = 1;
+        his.maxmedia = (uint32_t)prop;
+        his.detnum = (uint32_t)c.sim.detnum;
+        his.colcount = (uint32_t)reclen;
+        his.totalphoton = (uint32_t)c.sim.nphoton;
+        his.detected = (uint32_t)res.detected;
+        his.savedphoton = (uint32_t)saved;
+        his.unitinmm = c.unitinmm;
+        his.normalizer = (float)normalizer;
+        his.srcnum = 1;
+        his.respin = 1;
+        his.savedetflag = c.sim.issaveexit ? 0x77 : 0x47;
+        snprintf(fname, sizeof(fname), "%s.mch", c.session);
+        if (mmc_host_save_mch(fname, &his, res.detdata, saved)) die("can not write %s", fname);
+        printf("detected %lld photons\n", (long long)res.detected);
+    }
+    free(field);
+    free(res.detdata);
+    free(elem);
+    free(type);
+    free(evol);
+    free(nvol);
+    free(facenb);
+    free(planes);
+    free(c.node);
+    free(c.elem5);
+    free(c.med);
+    free(c.pattern);
+    return 0;
+}

@@ -93,7 +93,10 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
     if ((s0 | s1) == 0) s1 = 1;
 }
 
-template <bool NODAL, bool COMP>
+/* OCC: minimum resident workgroups of kBlock threads per CU the register
+ * allocation must allow (1 = no constraint; 5, 6, 8 trade registers, and at
+ * 6 and 8 some scratch, for more resident waves) */
+template <bool NODAL, bool COMP, int OCC>
 __global__ void transport(KArgs a);
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum);
 __global__ void fill_zero_u64(unsigned long long* p, int n);

@@ -108,8 +108,8 @@ __device__ __forceinline__ void deposit(const KArgs& a, uint64_t idx, float v) {
  * deposits (src/mmc_raytrace.c:227-508); otherwise the branch-less Badouel
  * tracer with element deposits (src/mmc_raytrace.c:1414-1709).
  */
-template <bool NODAL, bool COMP>
-__global__ void __launch_bounds__(kBlock) transport(KArgs a) {
+template <bool NODAL, bool COMP, int OCC>
+__global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nsim = a.nsim;
     const int nmed = a.prop + 1;
@@ -713,9 +713,14 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-template __global__ void transport<false, false>(KArgs);
-template __global__ void transport<false, true>(KArgs);
-template __global__ void transport<true, false>(KArgs);
-template __global__ void transport<true, true>(KArgs);
+#define MMC_INSTANTIATE(OCC)                                              \
+    template __global__ void transport<false, false, OCC>(KArgs);         \
+    template __global__ void transport<false, true, OCC>(KArgs);          \
+    template __global__ void transport<true, false, OCC>(KArgs);          \
+    template __global__ void transport<true, true, OCC>(KArgs);
+MMC_INSTANTIATE(1)
+MMC_INSTANTIATE(5)
+MMC_INSTANTIATE(6)
+MMC_INSTANTIATE(8)
 
 }  // namespace mmc

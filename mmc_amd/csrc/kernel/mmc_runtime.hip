@@ -436,13 +436,26 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     shmem = (shmem + 15) & ~(size_t)15;
     if (shmem > 160 * 1024) return set_err(-1, "batch of %d simulations needs %zu B of LDS", nsim, shmem);
 
-    const void* kfn = nodal ? (comp ? (const void*)transport<true, true> : (const void*)transport<true, false>)
-                            : (comp ? (const void*)transport<false, true> : (const void*)transport<false, false>);
-    int dev = 0, ncu = 0, occ = 0;
+    /* the register budget of the kernel variant follows the requested
+     * residency: 5, 6 or 8 workgroups per CU select variants compiled for it */
+    const int want = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : 0;
+    const int occ = want >= 8 ? 8 : want >= 6 ? 6 : want == 5 ? 5 : 1;
+    const void* kfn = nullptr;
+#define MMC_PICK(O)                                                                              \
+    kfn = nodal ? (comp ? (const void*)transport<true, true, O> : (const void*)transport<true, false, O>) \
+                : (comp ? (const void*)transport<false, true, O> : (const void*)transport<false, false, O>);
+    switch (occ) {
+        case 5: MMC_PICK(5) break;
+        case 6: MMC_PICK(6) break;
+        case 8: MMC_PICK(8) break;
+        default: MMC_PICK(1) break;
+    }
+#undef MMC_PICK
+    int dev = 0, ncu = 0, maxbpc = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, shmem));
-    int bpc = (tune && tune->blocks_per_cu > 0) ? std::min(tune->blocks_per_cu, std::max(occ, 1)) : std::max(occ, 1);
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&maxbpc, kfn, block, shmem));
+    int bpc = want > 0 ? std::min(want, std::max(maxbpc, 1)) : std::max(maxbpc, 1);
     int grid = ncu * bpc;
 
     hipEvent_t e0, e1;
@@ -450,13 +463,8 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
     if (total > 0) {
-        if (nodal) {
-            if (comp) hipLaunchKernelGGL((transport<true, true>), dim3(grid), dim3(block), shmem, st, a);
-            else hipLaunchKernelGGL((transport<true, false>), dim3(grid), dim3(block), shmem, st, a);
-        } else {
-            if (comp) hipLaunchKernelGGL((transport<false, true>), dim3(grid), dim3(block), shmem, st, a);
-            else hipLaunchKernelGGL((transport<false, false>), dim3(grid), dim3(block), shmem, st, a);
-        }
+        void* kargs[] = {&a};
+        HIPCHK(hipLaunchKernel(kfn, dim3(grid), dim3(block), kargs, shmem, st));
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(e1, st));

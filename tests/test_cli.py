@@ -1,0 +1,66 @@
+"""The command-line front end (mmc_amd/bin/mmc): the reference's JSON input
+and flags, and its .dat / .bin / .mch outputs."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+MMC = os.path.join(ROOT, "mmc_amd", "bin", "mmc")
+
+
+def run(args, cwd):
+    return subprocess.run([MMC] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+def test_help_and_errors(tmp_path):
+    assert run(["-h"], tmp_path).returncode == 0
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"Session": {"Photons": 10,, }}')
+    r = run(["-f", str(bad)], tmp_path)
+    assert r.returncode != 0 and "JSON error" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--zip", "lz4"], tmp_path)
+    assert r.returncode != 0 and "unsupported option" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-G", "-1"], tmp_path)
+    assert r.returncode != 0 and "CPU path" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-M", "H"], tmp_path)
+    assert r.returncode != 0
+
+
+def read_mch(path):
+    raw = open(path, "rb").read()
+    head = struct.unpack("4s7If I f I i I 2i", raw[:64])
+    assert head[0] == b"MCXH"
+    colcount, saved = head[4], head[7]
+    data = np.frombuffer(raw[64:64 + 4 * colcount * saved], np.float32).reshape(saved, colcount)
+    return head, data
+
+
+@pytest.mark.gpu
+def test_cube2_json_element_output_and_detectors(tmp_path):
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-n", "20000", "-M", "S", "-C", "0", "-d", "1",
+             "-s", "c2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "photon/ms" in r.stdout
+    dat = np.loadtxt(tmp_path / "c2.dat")
+    assert dat.shape == (6 * 50, 2)                   # 6 elements x 50 gates
+    assert np.array_equal(dat[:6, 0], np.arange(1, 7))
+    assert np.all(dat[:, 1] >= 0) and dat[:, 1].sum() > 0
+    head, rec = read_mch(tmp_path / "c2.mch")
+    assert head[2] == 1 and head[3] == 4 and head[4] == 2 * 1 + 2   # maxmedia, detnum, colcount
+    assert rec.shape[0] == head[7] > 0
+    assert set(np.unique(rec[:, 0]).astype(int)) <= {1, 2, 3, 4}
+    assert np.all(rec[:, -1] == 1.0)                  # launch weight
+
+
+@pytest.mark.gpu
+def test_onecube_mesh_files_nodal_binary(tmp_path):
+    r = run(["-f", os.path.join(GOLDEN, "onecube", "onecube.json"), "-n", "5000", "-M", "P", "-C", "1",
+             "-F", "bin", "-s", "oc"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    f = np.fromfile(tmp_path / "oc.bin", np.float64)
+    assert f.shape == (8 * 10,)                       # 8 nodes x 10 gates
+    assert np.all(f >= 0) and f.sum() > 0
