@@ -180,6 +180,36 @@ def bench_serve(args):
             "speedup": served["photons_per_ms"] / plain["photons_per_ms"]}
 
 
+def bench_large(args):
+    """The large-scale workload: the 24.0M-tet jittered shells mesh, a 60x60 mm
+    planar source with 64x64 Bernoulli patterns, 100 gates of 0.05 ns,
+    Plucker nodal fluence with FP64 final reduction, 32 detectors, run as 8
+    pattern requests (args.large_photons in total) through the serving loop."""
+    import mmc_amd as M
+    from mmc_amd import serve as S
+    from mmc_amd import workloads as W
+    t = time.time()
+    mesh = W.shells_mesh(1.0)
+    log(f"shells mesh: {mesh.ne} tets, {mesh.nn} nodes, {len(mesh.srcelem)} launch candidates, "
+        f"built in {time.time() - t:.1f} s")
+    sims = W.shells_sims(8, int(args.large_photons))
+    reqs = [S.Request(rid=i, mesh="shells", cfg=c, arrival=0.0) for i, c in enumerate(sims)]
+    srv = S.Server({"shells": mesh}, arena_bytes=32 << 30, slice_photons=int(args.large_slice))
+    log(f"mesh resident: {srv.res['shells'].device_bytes / 2**30:.2f} GiB")
+    stats = srv.serve(reqs)
+    srv.close()
+    steps = sum(r.raytet for r in reqs)
+    stats["steps_per_photon"] = steps / stats["photons"]
+    stats["detected"] = int(sum(r.detected for r in reqs))
+    stats["absorbed_fraction"] = sum(r.absorbweight for r in reqs) / sum(r.launchweight for r in reqs)
+    log(f"large: {stats}")
+    return {"workload": f"jittered 200x200x100 mm box, 1 mm cells x6 = {mesh.ne / 1e6:.1f}M tets, 5 tissues in "
+                        "nested ellipsoidal shells, 60x60 mm planar source with 64x64 Bernoulli(0.5) patterns, "
+                        f"{args.large_photons:.0e} photons as 8 pattern requests, 100 gates x 0.05 ns, Plucker "
+                        "nodal fluence (FP32 atomics, FP64 reduction), 32 detectors",
+            "tets": int(mesh.ne), "nodes": int(mesh.nn), **stats}
+
+
 def copy_probe(nbytes: int):
     """Device-to-device copy of a known size through the HIP runtime: the
     calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -203,7 +233,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=100_000_000)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "all"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "all", "large"])
+    ap.add_argument("--large-photons", type=float, default=1e9)
+    ap.add_argument("--large-slice", type=float, default=2.4e7)
     ap.add_argument("--serve-requests", type=int, default=8)
     ap.add_argument("--serve-max-photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -224,6 +256,8 @@ def main():
         out["batched"] = bench_batch(args)
     if args.mode in ("serve", "all"):
         out["serving"] = bench_serve(args)
+    if args.mode == "large":
+        out["large"] = bench_large(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds)
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

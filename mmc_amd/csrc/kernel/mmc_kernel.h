@@ -52,6 +52,12 @@ struct KArgs {
     const float4* enode;             /* ne*3: the 12 node coordinates of each element (Plucker) */
     const int32_t* srcelem;
     int32_t srcelemlen;
+    /* launch locator over srcelem: uniform grid, CSR lists of positions in
+     * srcelem (ascending), or null for a linear search */
+    const int32_t* loc_start;
+    const int32_t* loc_elem;
+    float4 loc_min, loc_inv;
+    int4 loc_dim;
     int32_t nn, ne, prop, nsim;
     /* simulations */
     const SimHot* hot;               /* nsim */

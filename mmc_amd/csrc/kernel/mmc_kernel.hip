@@ -233,11 +233,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         px = EPSF * vx + 1.f * px;
                         py = EPSF * vy + 1.f * py;
                         pz = EPSF * vz + 1.f * pz;
-                        /* enclosing element: e0 first, then the candidate list */
+                        /* enclosing element: e0 first, then the candidate list in
+                         * its order; with a locator grid only the candidates whose
+                         * (widened) bounding box covers the launch point's cell are
+                         * tested, in the same order, so the first match is the same */
                         int found = 0;
-                        for (int is = -1; is < a.srcelemlen && !found; is++) {
-                            int ce = (is < 0) ? eid : a.srcelem[is];
-                            if (ce <= 0) continue;
+                        auto try_elem = [&](int ce) -> bool {
                             int4 ee = a.elem[ce - 1];
                             float bc[4];
                             for (int f = 0; f < 4; f++) {
@@ -253,12 +254,27 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                 bc[c_facemap[f]] = -(sx * nx + sy * ny + sz * nz);
                             }
                             if (bc[0] < -1e-4f || bc[1] < -1e-4f || bc[2] < -1e-4f || bc[3] < -1e-4f)
-                                continue;
+                                return false;
                             float s = 0.f;
                             s += bc[0]; s += bc[1]; s += bc[2]; s += bc[3];
                             eid = ce;
                             b0 = bc[0] / s; b1 = bc[1] / s; b2 = bc[2] / s; b3 = bc[3] / s;
-                            found = 1;
+                            return true;
+                        };
+                        if (eid > 0 && eid <= a.ne) found = try_elem(eid);
+                        if (!found && a.loc_start) {
+                            int cx = (int)floorf((px - a.loc_min.x) * a.loc_inv.x);
+                            int cy = (int)floorf((py - a.loc_min.y) * a.loc_inv.y);
+                            int cz = (int)floorf((pz - a.loc_min.z) * a.loc_inv.z);
+                            /* clamped like the host's cell ranges (a monotone map) */
+                            cx = min(max(cx, 0), a.loc_dim.x - 1);
+                            cy = min(max(cy, 0), a.loc_dim.y - 1);
+                            cz = min(max(cz, 0), a.loc_dim.z - 1);
+                            int cell = (cz * a.loc_dim.y + cy) * a.loc_dim.x + cx;
+                            for (int k = a.loc_start[cell]; k < a.loc_start[cell + 1] && !found; k++)
+                                found = try_elem(a.srcelem[a.loc_elem[k]]);
+                        } else {
+                            for (int is = 0; is < a.srcelemlen && !found; is++) found = try_elem(a.srcelem[is]);
                         }
                         if (!found) eid = ID_UNDEF;   /* reported as a lost photon */
                     }

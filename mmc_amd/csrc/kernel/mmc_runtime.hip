@@ -53,6 +53,10 @@ struct mmc_hip_meshres {
     float4* enode = nullptr;
     int32_t* srcelem = nullptr;
     int32_t srcelemlen = 0;
+    int32_t* loc_start = nullptr;
+    int32_t* loc_elem = nullptr;
+    float4 loc_min, loc_inv;
+    int4 loc_dim;
     std::vector<float> med;       /* (prop+1)*4 default media */
     int64_t bytes = 0;
 };
@@ -96,7 +100,91 @@ static void mesh_free(mmc_hip_meshres* m) {
     (void)hipFree(m->node);
     (void)hipFree(m->enode);
     (void)hipFree(m->srcelem);
+    (void)hipFree(m->loc_start);
+    (void)hipFree(m->loc_elem);
     delete m;
+}
+
+/* Uniform grid over the candidate launch elements: each cell lists, in
+ * ascending candidate order, the candidates whose bounding box (widened by a
+ * margin that covers the -1e-4 barycentric tolerance of the launch test)
+ * overlaps it.  Cell coordinates are computed with the same float operations
+ * as the kernel's, so a point inside a widened box always falls in a cell of
+ * that box's range. */
+static hipError_t build_locator(const mmc_hip_mesh* mesh, mmc_hip_meshres* m) {
+    const int n = mesh->srcelemlen;
+    std::vector<float> lo(3 * (size_t)n), hi(3 * (size_t)n);
+    float gmin[3] = {1e30f, 1e30f, 1e30f}, gmax[3] = {-1e30f, -1e30f, -1e30f};
+    double mean_ext[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const int32_t* e = mesh->elem + 4 * (size_t)(mesh->srcelem[i] - 1);
+        for (int k = 0; k < 3; k++) {
+            float a = 1e30f, b = -1e30f;
+            for (int j = 0; j < 4; j++) {
+                float v = mesh->node[3 * (size_t)(e[j] - 1) + k];
+                a = std::min(a, v);
+                b = std::max(b, v);
+            }
+            float pad = 1e-3f * (b - a) + 1e-3f;
+            lo[3 * i + k] = a - pad;
+            hi[3 * i + k] = b + pad;
+            gmin[k] = std::min(gmin[k], a - pad);
+            gmax[k] = std::max(gmax[k], b + pad);
+            mean_ext[k] += (double)(b - a) / n;
+        }
+    }
+    int dim[3];
+    float inv[3];
+    for (int k = 0; k < 3; k++) {
+        double ext = std::max((double)gmax[k] - gmin[k], 1e-6);
+        double cell = std::max(mean_ext[k], ext / 256.0);
+        dim[k] = std::max(1, std::min(256, (int)std::ceil(ext / cell)));
+        inv[k] = (float)(dim[k] / ext);
+    }
+    auto cell_of = [&](float v, int k) {
+        int c = (int)floorf((v - gmin[k]) * inv[k]);
+        return std::max(0, std::min(dim[k] - 1, c));
+    };
+    const size_t ncell = (size_t)dim[0] * dim[1] * dim[2];
+    std::vector<int32_t> start(ncell + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<int32_t> fill;
+        if (pass) fill.assign(start.begin(), start.end() - 1);
+        std::vector<int32_t>* out = nullptr;
+        static thread_local std::vector<int32_t> items;
+        if (pass) {
+            items.assign((size_t)start[ncell], 0);
+            out = &items;
+        }
+        for (int i = 0; i < n; i++) {
+            int c0[3], c1[3];
+            for (int k = 0; k < 3; k++) {
+                c0[k] = cell_of(lo[3 * i + k], k);
+                c1[k] = cell_of(hi[3 * i + k], k);
+            }
+            for (int z = c0[2]; z <= c1[2]; z++)
+                for (int y = c0[1]; y <= c1[1]; y++)
+                    for (int x = c0[0]; x <= c1[0]; x++) {
+                        size_t c = ((size_t)z * dim[1] + y) * dim[0] + x;
+                        if (!pass) start[c + 1]++;
+                        else (*out)[fill[c]++] = i;
+                    }
+        }
+        if (!pass) {
+            for (size_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+        } else {
+            int64_t dummy = 0;
+            if (dalloc(&m->loc_start, ncell + 1, &m->bytes) || dalloc(&m->loc_elem, items.size(), &dummy))
+                return hipErrorOutOfMemory;
+            hipError_t e = hipMemcpy(m->loc_start, start.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice);
+            if (!e) e = hipMemcpy(m->loc_elem, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice);
+            if (e) return e;
+        }
+    }
+    m->loc_min = make_float4(gmin[0], gmin[1], gmin[2], 0.f);
+    m->loc_inv = make_float4(inv[0], inv[1], inv[2], 0.f);
+    m->loc_dim = make_int4(dim[0], dim[1], dim[2], 0);
+    return hipSuccess;
 }
 
 extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_handle* out) {
@@ -176,6 +264,7 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
         e = hipMemcpy(m->srcelem, mesh->srcelem, 4 * (size_t)mesh->srcelemlen, hipMemcpyHostToDevice);
     }
     m->srcelemlen = mesh->srcelemlen;
+    if (!e && mesh->srcelemlen > 64) e = build_locator(mesh, m);
     if (e != hipSuccess) {
         mesh_free(m);
         return set_err(-3, "mesh upload failed: %s", hipGetErrorString(e));
@@ -403,6 +492,11 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     a.enode = m->enode;
     a.srcelem = m->srcelem;
     a.srcelemlen = m->srcelemlen;
+    a.loc_start = m->loc_start;
+    a.loc_elem = m->loc_elem;
+    a.loc_min = m->loc_min;
+    a.loc_inv = m->loc_inv;
+    a.loc_dim = m->loc_dim;
     a.nn = m->nn;
     a.ne = m->ne;
     a.prop = m->prop;

@@ -126,3 +126,21 @@ def jittered_box(nx=120, ny=120, nz=60, h=1.0, amp=0.2, seed=20241, axes=None, n
     for lab in range(1, prop + 1):
         med[lab] = [0.01 * lab, 1.0 + 0.5 * lab, 0.9, 1.37]
     return node, elem, etype, med
+
+
+def shells_box(nx=200, ny=200, nz=100, h=1.0, amp=0.2, seed=31,
+               outer=(90.0, 80.0, 40.0), inner=(60.0, 50.0, 25.0), nshell=4):
+    """Jittered box with nested ellipsoidal shells: label 1 outside the
+    largest ellipsoid, labels 2..nshell+1 inside successively smaller ones
+    (semi-axes linear from `outer` to `inner`), centred in the box."""
+    axes = [np.asarray(outer) + (np.asarray(inner) - np.asarray(outer)) * t
+            for t in np.linspace(0.0, 1.0, nshell)]
+    node, elem, etype, _ = jittered_box(nx, ny, nz, h, amp=amp, seed=seed, axes=axes, nlabel=nshell + 1)
+    # properties of the layered slab, and mua = 0.1/mm for the innermost shell
+    med = np.array([[0, 0, 1, 1],
+                    [0.019, 7.8, 0.89, 1.37],
+                    [0.019, 7.8, 0.89, 1.37],
+                    [0.004, 0.009, 0.89, 1.37],
+                    [0.02, 9.0, 0.89, 1.37],
+                    [0.1, 9.0, 0.89, 1.37]], np.float32)
+    return node, elem, etype, med

@@ -100,16 +100,20 @@ class Scheduler:
     """Admission and chunk planning (host logic, testable without a GPU)."""
 
     def __init__(self, arena_floats: int, datalen: Dict[str, int], slice_photons: int = 2_000_000,
-                 min_chunk: int = 100_000):
+                 min_chunk: int = 100_000, nodelen: Optional[Dict[str, int]] = None):
         self.arena = FieldArena(arena_floats)
-        self.datalen = datalen
+        self.datalen = datalen          # elements per mesh (element fields)
+        self.nodelen = nodelen or {}    # nodes per mesh (nodal fields)
         self.slice_photons = slice_photons
         self.min_chunk = min_chunk
         self.waiting: deque = deque()
         self.active: List[Request] = []
 
+    def rows(self, r: Request) -> int:
+        return self.nodelen[r.mesh] if r.cfg.basisorder else self.datalen[r.mesh]
+
     def field_len(self, r: Request) -> int:
-        return self.datalen[r.mesh] * r.cfg.gates()[0]
+        return self.rows(r) * r.cfg.gates()[0]
 
     def arrive(self, r: Request, mesh: Optional[TetMesh] = None):
         if mesh is not None and r.cfg.e0 <= 0 and r.cfg.srctype == "pencil":
@@ -152,16 +156,17 @@ class Scheduler:
 
 class Server:
     def __init__(self, meshes: Dict[str, TetMesh], arena_bytes: int = 32 << 30, slice_photons: int = 2_000_000,
-                 tune=None):
+                 tune=None, compensated: bool = False):
         self.res = {k: ResidentMesh(m) for k, m in meshes.items()}
         self.meshes = meshes
         L = _lib.lib()
         h = C.c_void_p()
         nfloat = arena_bytes // 4
-        _lib.check(L.mmc_hip_arena_create(C.c_uint64(nfloat), 0, C.byref(h)), "mmc_hip_arena_create")
+        _lib.check(L.mmc_hip_arena_create(C.c_uint64(nfloat), int(compensated), C.byref(h)),
+                   "mmc_hip_arena_create")
         self.arena = h
-        dl = {k: m.ne for k, m in meshes.items()}     # element fields
-        self.sched = Scheduler(nfloat, dl, slice_photons)
+        self.sched = Scheduler(nfloat, {k: m.ne for k, m in meshes.items()}, slice_photons,
+                               nodelen={k: m.nn for k, m in meshes.items()})
         self.tune = tune if tune is not None else tune_struct()
         self.launches = 0
         self.kernel_ms = 0.0
@@ -207,7 +212,7 @@ class Server:
         L = _lib.lib()
         s = C.c_double()
         if r.want_field:
-            r.field = np.zeros((r.cfg.gates()[0], self.sched.datalen[r.mesh]), np.float64)
+            r.field = np.zeros((r.cfg.gates()[0], self.sched.rows(r)), np.float64)
             host = ptr(r.field, C.c_double)
         else:
             host = C.cast(None, f64p)

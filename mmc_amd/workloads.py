@@ -8,6 +8,11 @@ slab        100x100x50 mm box, 1 mm cells x6 tets (3.0M tets), four layers and a
 slab_batch  256 simulations on the slab mesh: pencils on a 16x16 grid of 4 mm
             pitch, per-tissue media drawn from a seed, photon counts
             log-uniform in [1e5, 1e7], 10 gates of 0.5 ns
+shells      jittered 200x200x100 mm box, 1 mm cells x6 tets (24.0M tets; the
+            stated ~22M tets implies ~1 mm cells, not 0.6 mm which would give
+            111M), five tissues in nested ellipsoidal shells; a 60x60 mm planar
+            source with a 64x64 Bernoulli(0.5) pattern, 100 gates of 0.05 ns,
+            Plucker nodal fluence, 32 detectors; run as 8 pattern requests
 """
 from __future__ import annotations
 
@@ -63,4 +68,37 @@ def slab_batch_sims(nsim: int = 256, seed: int = 7, prop: int = 5, lo=1e5, hi=1e
         n = int(round(10 ** rng.uniform(np.log10(lo), np.log10(hi))))
         sims.append(SimConfig(nphoton=n, seed=1000 + k, method="S", srcpos=(x, y, 0.0), srcdir=(0.0, 0.0, 1.0),
                               tstart=0.0, tend=5e-9, tstep=5e-10, outputtype="x", med=med))
+    return sims
+
+
+def shells_mesh(h: float = 1.0, window=(70.0, 130.0)):
+    """The large jittered mesh with the launch candidates of the planar
+    source: the elements of the top cell layer under the source window."""
+    n = int(round(200 / h))
+    node, elem, et, med = meshgen.shells_box(n, n, n // 2, h)
+    c = meshgen.centroids(node, elem)
+    lo, hi = window[0] - 2 * h, window[1] + 2 * h
+    top = (c[:, 2] < h) & (c[:, 0] > lo) & (c[:, 0] < hi) & (c[:, 1] > lo) & (c[:, 1] < hi)
+    srcelem = (np.nonzero(top)[0] + 1).astype(np.int32)
+    return TetMesh(node, elem, et, med, srcelem=srcelem).prepare(nodal=True)
+
+
+def shells_detectors() -> np.ndarray:
+    ang = np.arange(32) * (2 * np.pi / 32)
+    return np.stack([100 + 45 * np.cos(ang), 100 + 45 * np.sin(ang), np.zeros(32), np.full(32, 1.5)],
+                    1).astype(np.float32)
+
+
+def shells_sims(nreq: int = 8, photons: int = 1_000_000_000, seed: int = 5):
+    """8 pattern requests sharing the 1e9 photons, each with its own 64x64
+    Bernoulli(0.5) pattern over the 60x60 mm window."""
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(nreq):
+        pat = (rng.random((64, 64)) < 0.5).astype(np.float32)
+        sims.append(SimConfig(nphoton=photons // nreq, seed=900 + k, method="P", srctype="pattern",
+                              srcpos=(70.0, 70.0, 0.0), srcdir=(0.0, 0.0, 1.0),
+                              srcparam1=(60.0, 0.0, 0.0, 64.0), srcparam2=(0.0, 60.0, 0.0, 64.0),
+                              srcpattern=pat, tstart=0.0, tend=5e-9, tstep=5e-11, outputtype="x",
+                              issavedet=1, detpos=shells_detectors(), maxdetphoton=1_000_000))
     return sims
