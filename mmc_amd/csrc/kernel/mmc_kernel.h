@@ -16,12 +16,16 @@ constexpr int kBlock = 256;          /* default threads per workgroup (4 waves) 
 constexpr int kMaxProp = 15;         /* media labels 1..kMaxProp per mesh */
 constexpr int kLaneSlots = 8;        /* per-lane LDS accumulators (energy, Kahan terms) */
 
-/* Per-simulation parameters read on every step; staged in LDS (32 B). */
-enum { kFlagRoulette = 1, kFlagEnergy = 2, kFlagSaveDet = 4 };
+/* Per-simulation parameters read on every step or crossing; staged in LDS
+ * (48 B).  flags: bit 0 roulette, 1 energy output, 2 save detectors,
+ * 3 voidtime, bits 4-5 boundary condition (MMC_BC_*), bits 6-7 isspecular. */
+enum { kFlagRoulette = 1, kFlagEnergy = 2, kFlagSaveDet = 4, kFlagVoidTime = 8 };
+constexpr int kReflShift = 4, kSpecShift = 6;
 struct SimHot {
     float tstart, tend, rtstep, minenergy;
     float roulettesize;
-    int32_t maxgate, flags, pad0;
+    int32_t maxgate, flags;
+    float nout;                      /* refractive index outside the mesh */
     uint64_t fieldoff;               /* float offset of this simulation's field */
     uint64_t pad1;
 };

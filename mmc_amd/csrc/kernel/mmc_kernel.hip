@@ -38,12 +38,13 @@ namespace mmc {
 
 enum { CTX_OUTER = 0, CTX_INNER = 1, CTX_INNERFIX = 2 };
 
-/* face tables (src/mmc_raytrace.c:75-152, src/mmc_mesh.c:59-103) */
-__constant__ int c_fc[4][3] = {{0, 4, 2}, {3, 5, 4}, {2, 5, 1}, {1, 3, 0}};
-__constant__ int c_nc[4][3] = {{3, 0, 1}, {3, 1, 2}, {2, 0, 3}, {1, 0, 2}};
-__constant__ int c_out[4][3] = {{0, 3, 1}, {3, 2, 1}, {0, 2, 3}, {0, 1, 2}};
-__constant__ int c_facemap[4] = {2, 0, 1, 3};
-__constant__ int c_ifacemap[4] = {1, 2, 0, 3};
+/* The face tables of src/mmc_raytrace.c:75-152 and src/mmc_mesh.c:59-103
+ * appear below as selects and packed nibbles so that every index resolves in
+ * registers:
+ *   Plucker face -> edge products  fc = {0,4,2} {3,5,4} {2,5,1} {1,3,0}
+ *   Plucker face -> nodes          nc = {3,0,1} {3,1,2} {2,0,3} {1,0,2}
+ *   launch sub-volume faces        out = {0,3,1} {3,2,1} {0,2,3} {0,1,2},
+ *                                  written to barycentric slot {2,0,1,3} */
 
 /* plane index -> facenb column, and its inverse */
 __device__ __forceinline__ int faceorder(int i) { return (0x1320 >> (4 * (3 - i))) & 0xF; }
@@ -55,6 +56,11 @@ __device__ __forceinline__ int pick4(int4 v, int i) {
 __device__ __forceinline__ float pick4f(float4 v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
+__device__ __forceinline__ float pick4a(const float* v, int i) {
+    return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+/* entry i (0..3) of a table packed as four nibbles, most significant first */
+__device__ __forceinline__ int nib4(int packed, int i) { return (packed >> (4 * (3 - i))) & 0xF; }
 /* packed face neighbour: (element << 4) | media label of that element */
 __device__ __forceinline__ int nb_id(int packed) { return packed >> 4; }
 __device__ __forceinline__ int nb_type(int packed) { return packed & 15; }
@@ -237,32 +243,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                          * its order; with a locator grid only the candidates whose
                          * (widened) bounding box covers the launch point's cell are
                          * tested, in the same order, so the first match is the same */
-                        int found = 0;
-                        auto try_elem = [&](int ce) -> bool {
-                            int4 ee = a.elem[ce - 1];
-                            float bc[4];
-                            for (int f = 0; f < 4; f++) {
-                                float4 A = a.node[pick4(ee, c_out[f][0]) - 1];
-                                float4 B = a.node[pick4(ee, c_out[f][1]) - 1];
-                                float4 Cc = a.node[pick4(ee, c_out[f][2]) - 1];
-                                float abx = B.x - A.x, aby = B.y - A.y, abz = B.z - A.z;
-                                float acx = Cc.x - A.x, acy = Cc.y - A.y, acz = Cc.z - A.z;
-                                float sx = px - A.x, sy = py - A.y, sz = pz - A.z;
-                                float nx = aby * acz - abz * acy;
-                                float ny = abz * acx - abx * acz;
-                                float nz = abx * acy - aby * acx;
-                                bc[c_facemap[f]] = -(sx * nx + sy * ny + sz * nz);
-                            }
-                            if (bc[0] < -1e-4f || bc[1] < -1e-4f || bc[2] < -1e-4f || bc[3] < -1e-4f)
-                                return false;
-                            float s = 0.f;
-                            s += bc[0]; s += bc[1]; s += bc[2]; s += bc[3];
-                            eid = ce;
-                            b0 = bc[0] / s; b1 = bc[1] / s; b2 = bc[2] / s; b3 = bc[3] / s;
-                            return true;
-                        };
-                        if (eid > 0 && eid <= a.ne) found = try_elem(eid);
-                        if (!found && a.loc_start) {
+                        /* candidates in order: k = -1 is e0, then either the
+                         * locator cell's list or the whole srcelem list; one
+                         * loop keeps a single copy of the containment test */
+                        int kbeg = -1, kend = a.srcelemlen;
+                        const bool useloc = a.loc_start != nullptr;
+                        if (useloc) {
                             int cx = (int)floorf((px - a.loc_min.x) * a.loc_inv.x);
                             int cy = (int)floorf((py - a.loc_min.y) * a.loc_inv.y);
                             int cz = (int)floorf((pz - a.loc_min.z) * a.loc_inv.z);
@@ -271,10 +257,43 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             cy = min(max(cy, 0), a.loc_dim.y - 1);
                             cz = min(max(cz, 0), a.loc_dim.z - 1);
                             int cell = (cz * a.loc_dim.y + cy) * a.loc_dim.x + cx;
-                            for (int k = a.loc_start[cell]; k < a.loc_start[cell + 1] && !found; k++)
-                                found = try_elem(a.srcelem[a.loc_elem[k]]);
-                        } else {
-                            for (int is = 0; is < a.srcelemlen && !found; is++) found = try_elem(a.srcelem[is]);
+                            kbeg = a.loc_start[cell] - 1;
+                            kend = a.loc_start[cell + 1];
+                        }
+                        const int e0 = eid;
+                        bool found = false;
+                        for (int k = kbeg; k < kend && !found; k++) {
+                            int ce;
+                            if (k == kbeg) {
+                                if (e0 <= 0 || e0 > a.ne) continue;
+                                ce = e0;
+                            } else {
+                                ce = a.srcelem[useloc ? a.loc_elem[k] : k];
+                            }
+                            const int4 ee = a.elem[ce - 1];
+                            const float4 N0 = a.node[ee.x - 1], N1 = a.node[ee.y - 1], N2 = a.node[ee.z - 1],
+                                         N3 = a.node[ee.w - 1];
+                            /* signed sub-volume opposite each node (src/mmc_raytrace.c:2566-2590) */
+                            float bc[4];
+                            #pragma unroll
+                            for (int f = 0; f < 4; f++) {
+                                const float4 A = f == 1 ? N3 : N0;
+                                const float4 B = f == 0 ? N3 : (f == 1 ? N2 : (f == 2 ? N2 : N1));
+                                const float4 Cc = f == 0 ? N1 : (f == 1 ? N1 : (f == 2 ? N3 : N2));
+                                float abx = B.x - A.x, aby = B.y - A.y, abz = B.z - A.z;
+                                float acx = Cc.x - A.x, acy = Cc.y - A.y, acz = Cc.z - A.z;
+                                float sx = px - A.x, sy = py - A.y, sz = pz - A.z;
+                                float nx = aby * acz - abz * acy;
+                                float ny = abz * acx - abx * acz;
+                                float nz = abx * acy - aby * acx;
+                                bc[(0x2013 >> (4 * (3 - f))) & 0xF] = -(sx * nx + sy * ny + sz * nz);
+                            }
+                            if (bc[0] < -1e-4f || bc[1] < -1e-4f || bc[2] < -1e-4f || bc[3] < -1e-4f) continue;
+                            float sum = 0.f;
+                            sum += bc[0]; sum += bc[1]; sum += bc[2]; sum += bc[3];
+                            eid = ce;
+                            b0 = bc[0] / sum; b1 = bc[1] / sum; b2 = bc[2] / sum; b3 = bc[3] / sum;
+                            found = true;
                         }
                         if (!found) eid = ID_UNDEF;   /* reported as a lost photon */
                     }
@@ -405,29 +424,45 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 float bo[4] = {0.f, 0.f, 0.f, 0.f};
                 float Lp0 = 0.f;
                 Lmove = 0.f;
-                for (int i = 0; i < 4; i++) {
-                    if (i >= 2) wv[c_fc[i][1]] = -wv[c_fc[i][1]];
-                    float wa = wv[c_fc[i][0]], wb = wv[c_fc[i][1]], wc = wv[c_fc[i][2]];
-                    if ((__float_as_uint(wa) & 0x80000000u) & (__float_as_uint(wb) & 0x80000000u) &
-                        (__float_as_uint(wc) ^ 0x80000000u)) {
-                        float Rv = 1.f / (-wa - wb + wc);
-                        int na = c_nc[i][0], nb = c_nc[i][1], nc = c_nc[i][2];
-                        float ba = -wa * Rv, bb = -wb * Rv, bcc = wc * Rv;
-                        bo[na] = ba; bo[nb] = bb; bo[nc] = bcc;
-                        poutx = ba * nxs[na] + bb * nxs[nb] + bcc * nxs[nc];
-                        pouty = ba * nys[na] + bb * nys[nb] + bcc * nys[nc];
-                        poutz = ba * nzs[na] + bb * nzs[nb] + bcc * nzs[nc];
-                        undef = 0;
-                        float dx = poutx - px, dy = pouty - py, dz = poutz - pz;
-                        Lp0 = sqrtf(dx * dx + dy * dy + dz * dz);
-                        float dlen = (mus <= EPSF) ? R_MIN_MUSF : slen / mus;
-                        faceidx = i;
-                        faceid = faceorder(i);
-                        nexteid = pick4(fnb, faceid);
-                        isend = (Lp0 > dlen);
-                        Lmove = isend ? dlen : Lp0;
-                        break;
-                    }
+                /* the four faces in order; face i sees Plucker products
+                 * (a, b, c) = wv[fc[i]] with b negated for faces 2 and 3, and
+                 * the ray leaves through the first face with a < 0, b < 0,
+                 * c >= 0 by sign bit.  All four are tested and the exit
+                 * computed once for the selected face. */
+                const float fa[4] = {wv[0], wv[3], wv[2], wv[1]};
+                const float fb[4] = {wv[4], wv[5], -wv[5], -wv[3]};
+                const float fcv[4] = {wv[2], wv[4], wv[1], wv[0]};
+                int sel = 4;
+                #pragma unroll
+                for (int i = 3; i >= 0; i--) {
+                    uint32_t ua = __float_as_uint(fa[i]), ub = __float_as_uint(fb[i]), uc = __float_as_uint(fcv[i]);
+                    if ((ua & ub & ~uc) >> 31) sel = i;
+                }
+                if (sel < 4) {
+                    const float wa = sel == 0 ? fa[0] : (sel == 1 ? fa[1] : (sel == 2 ? fa[2] : fa[3]));
+                    const float wb = sel == 0 ? fb[0] : (sel == 1 ? fb[1] : (sel == 2 ? fb[2] : fb[3]));
+                    const float wc = sel == 0 ? fcv[0] : (sel == 1 ? fcv[1] : (sel == 2 ? fcv[2] : fcv[3]));
+                    /* face nodes nc[i] = {3,0,1} {3,1,2} {2,0,3} {1,0,2} */
+                    const int na = nib4(0x3321, sel), nb = nib4(0x0100, sel), nc = nib4(0x1232, sel);
+                    float Rv = 1.f / (-wa - wb + wc);
+                    float ba = -wa * Rv, bb = -wb * Rv, bcc = wc * Rv;
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) bo[k] = (k == na) ? ba : ((k == nb) ? bb : ((k == nc) ? bcc : 0.f));
+                    const float xa = pick4a(nxs, na), xb = pick4a(nxs, nb), xc = pick4a(nxs, nc);
+                    const float ya = pick4a(nys, na), yb = pick4a(nys, nb), yc = pick4a(nys, nc);
+                    const float za = pick4a(nzs, na), zb = pick4a(nzs, nb), zc = pick4a(nzs, nc);
+                    poutx = ba * xa + bb * xb + bcc * xc;
+                    pouty = ba * ya + bb * yb + bcc * yc;
+                    poutz = ba * za + bb * zb + bcc * zc;
+                    undef = 0;
+                    float dx = poutx - px, dy = pouty - py, dz = poutz - pz;
+                    Lp0 = sqrtf(dx * dx + dy * dy + dz * dz);
+                    float dlen = (mus <= EPSF) ? R_MIN_MUSF : slen / mus;
+                    faceidx = sel;
+                    faceid = faceorder(sel);
+                    nexteid = pick4(fnb, faceid);
+                    isend = (Lp0 > dlen);
+                    Lmove = isend ? dlen : Lp0;
                 }
                 if (!undef) {
                     if ((int)((tof + Lmove * rc - hot.tstart) * hot.rtstep) >=
@@ -473,9 +508,11 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             /* carry the exit-face barycentrics into the next element */
                             int4 ne4 = a.elem[nb_id(nexteid) - 1];
                             float nbv[4] = {0.f, 0.f, 0.f, 0.f};
+                            #pragma unroll
                             for (int j = 0; j < 3; j++) {
-                                int node = pick4(ee, c_nc[faceidx][j]);
-                                float bv = bo[c_nc[faceidx][j]];
+                                const int nj = nib4(j == 0 ? 0x3321 : (j == 1 ? 0x0100 : 0x1232), faceidx);
+                                int node = pick4(ee, nj);
+                                float bv = pick4a(bo, nj);
                                 if (ne4.x == node) nbv[0] = bv;
                                 else if (ne4.y == node) nbv[1] = bv;
                                 else if (ne4.z == node) nbv[2] = bv;
@@ -532,7 +569,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
 
             /* ---- crossing into the neighbour (inner while of onephoton) ---- */
             if (eid != ID_UNDEF && faceid >= 0 && !isend) {
-                const SimCold& C = a.cold[sim];
+                const int isreflect = (hot.flags >> kReflShift) & 3;
+                const int isspecular = (hot.flags >> kSpecShift) & 3;
                 px = poutx; py = pouty; pz = poutz;
                 int oldeid = eid;
                 int oldtype = ctype;
@@ -540,19 +578,19 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 int neid = nb_id(nbp);
                 int ntype = neid > 0 ? nb_type(nbp) : 0;
                 float nold = med[oldtype].w;
-                if (C.isreflect && (neid <= 0 || med[ntype].w != nold)) {
-                    if (!(neid <= 0 && ((nold == C.nout && C.isreflect != MMC_BC_MIRROR) ||
-                                        C.isreflect == MMC_BC_ABSORBEXT))) {
+                if (isreflect && (neid <= 0 || med[ntype].w != nold)) {
+                    if (!(neid <= 0 && ((nold == hot.nout && isreflect != MMC_BC_MIRROR) ||
+                                        isreflect == MMC_BC_ABSORBEXT))) {
                         /* reflectray (src/mmc_raytrace.c:2257-2332) */
                         int f = ifaceorder(faceid);
                         const float4* P = a.planes + 4 * (size_t)(oldeid - 1);
                         float pnx = pick4f(P[0], f), pny = pick4f(P[1], f), pnz = pick4f(P[2], f);
                         float Icos = fabsf(vx * pnx + vy * pny + vz * pnz);
                         float n1 = nold;
-                        float n2 = (neid > 0) ? med[ntype].w : C.nout;
+                        float n2 = (neid > 0) ? med[ntype].w : hot.nout;
                         float t0 = n1 * n1, t1 = n2 * n2;
                         float t2 = 1.f - t0 / t1 * (1.f - Icos * Icos);
-                        if (t2 > 0.f && !(neid <= 0 && C.isreflect == MMC_BC_MIRROR)) {
+                        if (t2 > 0.f && !(neid <= 0 && isreflect == MMC_BC_MIRROR)) {
                             float Re = t0 * Icos * Icos + t1 * t2;
                             t2 = sqrtf(t2);
                             float Im = 2.f * n1 * n2 * Icos * t2;
@@ -564,7 +602,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                 vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
                                 neid = oldeid;
                                 ntype = oldtype;
-                            } else if (C.isspecular == 2 && neid == 0) {
+                            } else if (isspecular == 2 && neid == 0) {
                             } else {
                                 float k = -Icos;
                                 vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
@@ -583,7 +621,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 }
                 eid = neid;
                 if (eid > 0) {
-                    if (oldtype == 0 && ntype != 0 && !C.voidtime) tof = 0.f;
+                    if (oldtype == 0 && ntype != 0 && !(hot.flags & kFlagVoidTime)) tof = 0.f;
                     if (oldtype != 0 && ntype == 0) eid = 0;
                 }
                 if (eid > 0) {
@@ -612,7 +650,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     const float g = med[ctype].z;
                     float sphi, cphi, stheta, ctheta, tmp0, tmp1;
                     float nextslen = rng_scatlen(r0, r1);
-                    tmp0 = mmc_twopi_u((uint32_t)(rng_next(r0, r1) * 8388608.0f));
+                    tmp0 = mmc_twopi(rng_next(r0, r1));
                     mmc_sincosf(tmp0, &sphi, &cphi);
                     if (g > EPSF) {
                         tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * rng_next(r0, r1));

@@ -312,6 +312,9 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
     if (s->issavedet && s->detnum > 0 && !s->detpos) return set_err(-1, "detectors without positions");
     if (s->outputtype != MMC_OUT_FLUX && s->outputtype != MMC_OUT_FLUENCE && s->outputtype != MMC_OUT_ENERGY)
         return set_err(-1, "unsupported output type %d", s->outputtype);
+    if (s->isreflect < MMC_BC_NOREFLECT || s->isreflect > MMC_BC_MIRROR)
+        return set_err(-1, "unsupported boundary condition %d", s->isreflect);
+    if (s->isspecular < 0 || s->isspecular > 2) return set_err(-1, "isspecular must be 0, 1 or 2");
     return 0;
 }
 
@@ -373,7 +376,10 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
         float span = (s.tend - s.tstart) * h.rtstep;
         h.flags = ((s.minenergy > 0.f && span <= 1.f) ? kFlagRoulette : 0) |
                   (s.outputtype == MMC_OUT_ENERGY ? kFlagEnergy : 0) |
-                  ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0);
+                  ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0) |
+                  (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
+                  ((s.isspecular & 3) << kSpecShift);
+        h.nout = s.nout;
         const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate;
         if (arena) {
             if (arena_off[i] + flen > arena->n)

@@ -1,6 +1,7 @@
 """Accuracy of the deterministic FP32 libm (include/mmc_fp32math.h) against
-double precision, and exactness of the integer emulation of the reference's
-double-precision azimuth (float)(2*pi*(double)u) (src/mmc_mesh.c:1666)."""
+double precision, and the reference's double-precision azimuth
+(float)(2*pi*(double)u) (src/mmc_mesh.c:1666) for every random number the
+generator can produce."""
 import ctypes as C
 import os
 import subprocess
@@ -16,7 +17,7 @@ void v_exp(const float* x, float* y, long n) { for (long i = 0; i < n; i++) y[i]
 void v_log(const float* x, float* y, long n) { for (long i = 0; i < n; i++) y[i] = mmc_logf(x[i]); }
 void v_sincos(const float* x, float* s, float* c, long n) { for (long i = 0; i < n; i++) mmc_sincosf(x[i], s + i, c + i); }
 void v_acos(const float* x, float* y, long n) { for (long i = 0; i < n; i++) y[i] = mmc_acosf(x[i]); }
-void v_twopi(const unsigned* m, float* y, long n) { for (long i = 0; i < n; i++) y[i] = mmc_twopi_u(m[i]); }
+void v_twopi(const float* u, float* y, long n) { for (long i = 0; i < n; i++) y[i] = mmc_twopi(u[i]); }
 """
 
 
@@ -27,7 +28,7 @@ def mlib():
     so = os.path.join(BUILD, "libmathprobe.so")
     with open(src, "w") as f:
         f.write(SRC)
-    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-mfma", "-I", os.path.join(ROOT, "include"),
                     src, "-o", so], check=True)
     return C.CDLL(so)
 
@@ -89,8 +90,9 @@ def test_acos(mlib):
 
 def test_twopi_matches_double_product_exhaustively(mlib):
     m = np.arange(1 << 23, dtype=np.uint32)
+    u = (m.astype(np.float64) * 2.0 ** -23).astype(np.float32)     # every generator output
     y = np.zeros(len(m), np.float32)
-    call(mlib, "v_twopi", m, y)
-    u = (m.astype(np.float64) * 2.0 ** -23).astype(np.float32)
+    call(mlib, "v_twopi", u, y)
     ref = ((2.0 * np.pi) * u.astype(np.float64)).astype(np.float32)
     assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+    assert y[0] == 0.0 and y[-1] < np.float32(2 * np.pi)
