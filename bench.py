@@ -167,7 +167,7 @@ def bench_serve(args):
     log(f"serving meshes ready in {time.time() - t:.1f} s: " + ", ".join(f"{k} {m.ne}" for k, m in meshes.items()))
     hi = args.serve_max_photons
     reqs = S.poisson_requests(meshes, args.serve_requests, rate=200.0, photons=(1e5, hi))
-    srv = S.Server(meshes, arena_bytes=64 << 30)
+    srv = S.Server(meshes, arena_bytes=16 << 30)
     served = srv.serve(reqs)
     srv.close()
     log(f"served: {served}")
@@ -194,7 +194,7 @@ def bench_large(args):
         f"built in {time.time() - t:.1f} s")
     sims = W.shells_sims(8, int(args.large_photons))
     reqs = [S.Request(rid=i, mesh="shells", cfg=c, arrival=0.0) for i, c in enumerate(sims)]
-    srv = S.Server({"shells": mesh}, arena_bytes=32 << 30, slice_photons=int(args.large_slice))
+    srv = S.Server({"shells": mesh}, arena_bytes=16 << 30, slice_photons=int(args.large_slice))
     log(f"mesh resident: {srv.res['shells'].device_bytes / 2**30:.2f} GiB")
     stats = srv.serve(reqs)
     srv.close()

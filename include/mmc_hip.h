@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 1
+#define MMC_HIP_ABI_VERSION 2
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -121,6 +121,8 @@ typedef struct mmc_hip_tune {
     int32_t blocks_per_cu;   /* resident workgroups per CU */
     int32_t chunk;           /* photons claimed per lane refill round (1 = per photon) */
     int32_t compensated;     /* 1: TwoSum-compensated FP32 atomics (hi+lo buffers) */
+    int32_t replicas;        /* FP32 field copies, one per XCD group (0 = automatic:
+                                8 when they fit the device-memory budget, else 4, 2, 1) */
 } mmc_hip_tune;
 
 typedef struct mmc_hip_meshres* mmc_hip_mesh_handle;
@@ -156,7 +158,11 @@ int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
  *      photon chunks whose FP32 fields stay in a caller-managed device arena
  *      across launches and are reduced to FP64 when a request completes ---- */
 typedef struct mmc_hip_arenares* mmc_hip_arena_handle;
-int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, mmc_hip_arena_handle* out);
+/* An arena of nfloat field values held as `replicas` copies (1..8; the
+ * kernels deposit into the copy of the XCD they run on and collect sums
+ * the copies), plus a compensation buffer per copy when compensated. */
+int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, int32_t replicas,
+                             mmc_hip_arena_handle* out);
 int32_t mmc_hip_arena_release(mmc_hip_arena_handle a);
 /* One persistent launch over nsim chunks; chunk i accumulates into
  * arena[fieldoff[i] .. fieldoff[i] + datalen*maxgate).  Each sim's

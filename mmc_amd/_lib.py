@@ -76,7 +76,7 @@ class ResultC(C.Structure):
 
 class TuneC(C.Structure):
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
-                ("compensated", C.c_int32)]
+                ("compensated", C.c_int32), ("replicas", C.c_int32)]
 
 
 _lib = None
@@ -94,7 +94,7 @@ SIGNATURES = {
     "mmc_hip_run_batch": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.POINTER(TuneC),
                                       C.POINTER(ResultC)]),
     "mmc_hip_sim_output_bytes": (C.c_int64, [C.c_void_p, C.POINTER(SimC)]),
-    "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "mmc_hip_arena_release": (C.c_int32, [C.c_void_p]),
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
                                        C.POINTER(TuneC), C.POINTER(ResultC)]),
@@ -116,6 +116,9 @@ SIGNATURES = {
 }
 
 
+ABI_VERSION = 2        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+
+
 def lib():
     """Load libmmc_hip.so (raises if it has not been built)."""
     global _lib
@@ -128,6 +131,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.mmc_hip_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIBPATH} has ABI {L.mmc_hip_abi_version()}, this binding expects "
+                               f"{ABI_VERSION}: rebuild with __graft_entry__.build()")
         _lib = L
     return _lib
 

@@ -70,9 +70,12 @@ struct KArgs {
     const int64_t* cum;              /* nsim+1 cumulative photon counts */
     const float4* detpos;
     const float* pattern;
-    /* outputs */
+    /* outputs: nrep copies of the FP32 field, repstride floats apart; a
+     * workgroup deposits into copy (XCD id mod nrep) */
     float* field;
     float* field_lo;                 /* compensation buffer (TwoSum atomics) or null */
+    uint64_t repstride;
+    int32_t nrep, pad_rep;
     double* simacc;                  /* nsim*kAccCount */
     float* detdata;
     int64_t* detphoton;
@@ -108,7 +111,8 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
  * 6 and 8 some scratch, for more resident waves) */
 template <bool NODAL, bool COMP, int OCC>
 __global__ void transport(KArgs a);
-__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum);
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
+                             uint64_t repstride);
 __global__ void fill_zero_u64(unsigned long long* p, int n);
 
 }  // namespace mmc

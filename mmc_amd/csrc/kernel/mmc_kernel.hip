@@ -94,19 +94,32 @@ __device__ __forceinline__ void kahan_lds(float* s, float* c, float v) {
     *s = t;
 }
 
+/* FP32 field deposit into this workgroup's copy of the field (f, lo).  A
+ * zero deposit is skipped: x + 0 == x for the non-negative values the field
+ * holds, and the first step after every face crossing flushes an empty
+ * coalescing register (src/mmc_raytrace.c:1562-1612 adds it), which would
+ * otherwise double the atomics. */
 template <bool COMP>
-__device__ __forceinline__ void deposit(const KArgs& a, uint64_t idx, float v) {
+__device__ __forceinline__ void deposit(float* f, float* lo, uint64_t idx, float v) {
+    if (v == 0.f) return;
     if (!COMP) {
-        atomicAdd(a.field + idx, v);
+        atomicAdd(f + idx, v);
     } else {
         /* TwoSum against the value the atomic returned: hi + lo holds the
          * exact running sum up to the rounding of lo itself */
-        float old = atomicAdd(a.field + idx, v);
+        float old = atomicAdd(f + idx, v);
         float s = old + v;
         float bp = s - old;
         float err = (old - (s - bp)) + (v - bp);
-        if (err != 0.f) atomicAdd(a.field_lo + idx, err);
+        if (err != 0.f) atomicAdd(lo + idx, err);
     }
+}
+
+/* Physical XCD (accelerator complex die) executing this wave, 0..7 on MI355X. */
+__device__ __forceinline__ uint32_t xcd_id() {
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x & 0xF;
 }
 
 /*
@@ -140,6 +153,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     __syncthreads();
 
     const int lane = __lane_id();
+    /* Each XCD has its own L2: with one field copy per XCD no cache line is
+     * shared between dies, so the atomics stay local to one L2 instead of
+     * contending across the fabric (all copies are summed in FP64 at the end). */
+    const uint64_t rep = a.nrep > 1 ? (uint64_t)(xcd_id() % (uint32_t)a.nrep) : 0;
+    float* const fcopy = a.field + rep * a.repstride;
+    float* const lcopy = COMP ? a.field_lo + rep * a.repstride : nullptr;
     const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     /* ---- lane state ---- */
@@ -386,14 +405,14 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     const uint64_t foff = hot.fieldoff;
                     if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
                     if (newidx != oldidx) {
-                        deposit<COMP>(a, foff + oldidx, oldw);
+                        deposit<COMP>(fcopy, lcopy, foff + oldidx, oldw);
                         oldidx = newidx;
                         oldw = ww;
                     } else {
                         oldw += ww;
                     }
                     if (faceid == -2 || !isend) {
-                        deposit<COMP>(a, foff + newidx, oldw);
+                        deposit<COMP>(fcopy, lcopy, foff + newidx, oldw);
                         oldw = 0.f;
                     }
                 }
@@ -497,10 +516,10 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                 bo[2] = (1.f - ratio) * b2 + ratio * bo[2];
                                 bo[3] = (1.f - ratio) * b3 + ratio * bo[3];
                             }
-                            deposit<COMP>(a, base + (uint32_t)(ee.x - 1), ww * (b0 + bo[0]));
-                            deposit<COMP>(a, base + (uint32_t)(ee.y - 1), ww * (b1 + bo[1]));
-                            deposit<COMP>(a, base + (uint32_t)(ee.z - 1), ww * (b2 + bo[2]));
-                            deposit<COMP>(a, base + (uint32_t)(ee.w - 1), ww * (b3 + bo[3]));
+                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.x - 1), ww * (b0 + bo[0]));
+                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.y - 1), ww * (b1 + bo[1]));
+                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.z - 1), ww * (b2 + bo[2]));
+                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.w - 1), ww * (b3 + bo[3]));
                         }
                         if (isend) {
                             b0 = bo[0]; b1 = bo[1]; b2 = bo[2]; b3 = bo[3];
@@ -741,18 +760,24 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     flush_sim(sim);
 }
 
-/* FP32 -> FP64 accumulation of a finished field (the final reduction), with
- * the compensation buffer folded in when present; zeroes the FP32 buffers. */
-__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum) {
+/* FP32 -> FP64 accumulation of a finished field (the final reduction): the
+ * nrep copies (and their compensation buffers) summed in copy order; zeroes
+ * the FP32 buffers for reuse. */
+__global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
+                             uint64_t repstride) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     double part = 0.0;
     for (; i < n; i += stride) {
-        double v = (double)f[i];
-        f[i] = 0.f;
-        if (lo) {
-            v += (double)lo[i];
-            lo[i] = 0.f;
+        double v = 0.0;
+        for (int r = 0; r < nrep; r++) {
+            const uint64_t k = i + (uint64_t)r * repstride;
+            v += (double)f[k];
+            f[k] = 0.f;
+            if (lo) {
+                v += (double)lo[k];
+                lo[k] = 0.f;
+            }
         }
         out[i] = v;
         part += v;

@@ -326,10 +326,22 @@ struct DevBuf {
 };
 
 struct mmc_hip_arenares {
-    float* field = nullptr;
+    float* field = nullptr;          /* nrep copies of n floats */
     float* lo = nullptr;
     uint64_t n = 0;
+    int nrep = 1;
 };
+
+/* Field copies for a launch: one per XCD (8 on MI355X) while the copies fit
+ * half of the free device memory and 64 GiB, else 4, 2 or 1. */
+static int auto_replicas(uint64_t bytes_one_copy) {
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return 1;
+    const uint64_t budget = std::min<uint64_t>((uint64_t)freeb / 2, 64ull << 30);
+    for (int r = 8; r > 1; r >>= 1)
+        if ((uint64_t)r * bytes_one_copy <= budget) return r;
+    return 1;
+}
 
 /* One persistent launch over nsim simulations.  Without an arena the fields
  * are allocated here, reduced to FP64 and freed; with an arena they are
@@ -438,18 +450,26 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     const int reclen = maxrec;
 
     /* ---- device buffers ---- */
+    int nrep = 1;
+    if (arena) {
+        nrep = arena->nrep;
+    } else if (tune && tune->replicas > 0) {
+        nrep = std::min(tune->replicas, 8);
+    } else {
+        nrep = auto_replicas((uint64_t)fieldlen * sizeof(float) * (comp ? 2 : 1));
+    }
     DevBuf dhot, dcold, dmed, dcum, ddet, dpat, dfield, dlo, dacc, ddetdata, ddetph, ddetcnt, dq, drec;
     int64_t dummy = 0;
     int rc;
     if ((rc = dalloc((SimHot**)&dhot.p, nsim, &dummy)) || (rc = dalloc((SimCold**)&dcold.p, nsim, &dummy)) ||
         (rc = dalloc((float4**)&dmed.p, media.size(), &dummy)) || (rc = dalloc((int64_t**)&dcum.p, cum.size(), &dummy)) ||
         (rc = dalloc((float4**)&ddet.p, dets.size(), &dummy)) || (rc = dalloc((float**)&dpat.p, pats.size(), &dummy)) ||
-        (!arena && (rc = dalloc((float**)&dfield.p, fieldlen, &dummy))) ||
+        (!arena && (rc = dalloc((float**)&dfield.p, (size_t)fieldlen * nrep, &dummy))) ||
         (rc = dalloc((double**)&dacc.p, (size_t)nsim * kAccCount, &dummy)) ||
         (rc = dalloc((unsigned long long**)&ddetcnt.p, (size_t)nsim + 1, &dummy)) ||
         (rc = dalloc((unsigned long long**)&dq.p, 1, &dummy)))
         return rc;
-    if (!arena && comp && (rc = dalloc((float**)&dlo.p, fieldlen, &dummy))) return rc;
+    if (!arena && comp && (rc = dalloc((float**)&dlo.p, (size_t)fieldlen * nrep, &dummy))) return rc;
     if (savedet && maxdet > 0) {
         if ((rc = dalloc((float**)&ddetdata.p, (size_t)maxdet * reclen, &dummy)) ||
             (rc = dalloc((int64_t**)&ddetph.p, (size_t)maxdet, &dummy)))
@@ -480,8 +500,8 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     if (!pats.empty())
         HIPCHK(hipMemcpyAsync(dpat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
     if (!arena) {
-        HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
-        if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen : 1), st));
+        HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
+        if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
     }
     HIPCHK(hipMemsetAsync(dacc.p, 0, sizeof(double) * nsim * kAccCount, st));
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)dq.p, 1);
@@ -515,6 +535,8 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     a.pattern = (const float*)dpat.p;
     a.field = arena ? arena->field : (float*)dfield.p;
     a.field_lo = arena ? arena->lo : (float*)dlo.p;
+    a.nrep = nrep;
+    a.repstride = arena ? arena->n : fieldlen;
     a.simacc = (double*)dacc.p;
     a.detdata = (float*)ddetdata.p;
     a.detphoton = (int64_t*)ddetph.p;
@@ -618,7 +640,7 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
         int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
         hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
                            (float*)dfield.p + fieldoff[i], dlo.p ? (float*)dlo.p + fieldoff[i] : nullptr,
-                           (double*)d64.p, n, (double*)dsum.p + i);
+                           (double*)d64.p, n, (double*)dsum.p + i, nrep, (uint64_t)fieldlen);
         HIPCHK(hipGetLastError());
         if (r.field) {
             std::vector<double> hbuf(n);
@@ -668,16 +690,20 @@ extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* s
     return launch_batch(m, sims, nsim, tune, res, nullptr, nullptr);
 }
 
-extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, mmc_hip_arena_handle* out) {
+extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, int32_t replicas,
+                                        mmc_hip_arena_handle* out) {
     if (!out || nfloat == 0) return set_err(-1, "bad arena request");
+    if (replicas < 1 || replicas > 8) return set_err(-1, "arena replicas must be 1..8");
     mmc_hip_arenares* a = new mmc_hip_arenares();
     a->n = nfloat;
+    a->nrep = replicas;
+    const size_t total = (size_t)nfloat * replicas;
     int64_t dummy = 0;
-    int rc = dalloc(&a->field, (size_t)nfloat, &dummy);
-    if (!rc && compensated) rc = dalloc(&a->lo, (size_t)nfloat, &dummy);
+    int rc = dalloc(&a->field, total, &dummy);
+    if (!rc && compensated) rc = dalloc(&a->lo, total, &dummy);
     if (!rc) {
-        hipError_t e = hipMemset(a->field, 0, sizeof(float) * nfloat);
-        if (!e && a->lo) e = hipMemset(a->lo, 0, sizeof(float) * nfloat);
+        hipError_t e = hipMemset(a->field, 0, sizeof(float) * total);
+        if (!e && a->lo) e = hipMemset(a->lo, 0, sizeof(float) * total);
         if (e) rc = set_err(-3, "arena clear failed: %s", hipGetErrorString(e));
     }
     if (rc) {
@@ -727,7 +753,7 @@ extern "C" int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, u
     if (!e) {
         int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)std::max(ncu, 1) * 16);
         hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, 0, a->field + off,
-                           a->lo ? a->lo + off : nullptr, d64, n, dsum);
+                           a->lo ? a->lo + off : nullptr, d64, n, dsum, a->nrep, a->n);
         e = hipGetLastError();
     }
     std::vector<double> hbuf;

@@ -155,14 +155,17 @@ class Scheduler:
 
 
 class Server:
-    def __init__(self, meshes: Dict[str, TetMesh], arena_bytes: int = 32 << 30, slice_photons: int = 2_000_000,
-                 tune=None, compensated: bool = False):
+    def __init__(self, meshes: Dict[str, TetMesh], arena_bytes: int = 8 << 30, slice_photons: int = 2_000_000,
+                 tune=None, compensated: bool = False, replicas: int = 8):
+        """arena_bytes of FP32 field space for admitted requests, held as
+        `replicas` per-XCD copies (device memory = arena_bytes * replicas,
+        twice that when compensated)."""
         self.res = {k: ResidentMesh(m) for k, m in meshes.items()}
         self.meshes = meshes
         L = _lib.lib()
         h = C.c_void_p()
         nfloat = arena_bytes // 4
-        _lib.check(L.mmc_hip_arena_create(C.c_uint64(nfloat), int(compensated), C.byref(h)),
+        _lib.check(L.mmc_hip_arena_create(C.c_uint64(nfloat), int(compensated), int(replicas), C.byref(h)),
                    "mmc_hip_arena_create")
         self.arena = h
         self.sched = Scheduler(nfloat, {k: m.ne for k, m in meshes.items()}, slice_photons,

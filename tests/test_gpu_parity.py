@@ -197,6 +197,19 @@ def test_compensated_atomics_track_fp64_accumulation(cube):
     assert e_comp < 1e-6
 
 
+@pytest.mark.parametrize("replicas", [1, 3, 8])
+def test_field_copies_sum_to_the_fp64_accumulation(cube, replicas):
+    # the per-XCD field copies only split the atomics; their FP64 sum is the
+    # field (compensated copies: every photon's deposit is in exactly one copy)
+    _, tm, om = cube
+    cfg = W.cube60_sim(30000)
+    o = O.run(om, cfg, photonseed=seeds(cfg), math="portable")
+    g = M.run(tm, cfg, tune=M.sim.tune_struct(compensated=1, replicas=replicas))
+    assert g.launchweight == o.launchweight
+    assert np.linalg.norm(g.field - o.field) / np.linalg.norm(o.field) < 1e-6
+    assert abs(g.fieldsum - o.field.sum()) <= 1e-6 * abs(o.field.sum())
+
+
 def test_served_results_equal_results_alone():
     from mmc_amd.serve import Request, Server
     arrays = M.meshgen.slab_layers(4.0)
