@@ -94,14 +94,9 @@ __device__ __forceinline__ void kahan_lds(float* s, float* c, float v) {
     *s = t;
 }
 
-/* FP32 field deposit into this workgroup's copy of the field (f, lo).  A
- * zero deposit is skipped: x + 0 == x for the non-negative values the field
- * holds, and the first step after every face crossing flushes an empty
- * coalescing register (src/mmc_raytrace.c:1562-1612 adds it), which would
- * otherwise double the atomics. */
+/* FP32 field deposit into this workgroup's copy of the field (f, lo). */
 template <bool COMP>
 __device__ __forceinline__ void deposit(float* f, float* lo, uint64_t idx, float v) {
-    if (v == 0.f) return;
     if (!COMP) {
         atomicAdd(f + idx, v);
     } else {
@@ -177,6 +172,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     int ctx = CTX_OUTER, fixcount = 0;
     float eabs = 0.f, eabs_c = 0.f;
     uint32_t raytet = 0, raytet_start = 0;
+    int cache_e = -1;                   /* element whose planes are in cNX..cfnb */
+    int4 cfnb = make_int4(0, 0, 0, 0);
+    float4 cNX = make_float4(0.f, 0.f, 0.f, 0.f), cNY = cNX, cNZ = cNX, cDD = cNX;
 
     auto flush_sim = [&](int s) {
         if (s < 0) return;
@@ -343,15 +341,27 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         if (ctx >= 0) {
             /* ============================ ray-tetrahedron step ============ */
             const int e = eid - 1;
-            const int4 fnb = a.facenb[e];
+            int4 fnb;
+            if (!NODAL) {
+                /* the element's planes and neighbours stay in registers while
+                 * the photon scatters inside it; reloaded after a crossing */
+                if (e != cache_e) {
+                    const float4* P = a.planes + 4 * (size_t)e;
+                    cNX = P[0]; cNY = P[1]; cNZ = P[2]; cDD = P[3];
+                    cfnb = a.facenb[e];
+                    cache_e = e;
+                }
+                fnb = cfnb;
+            } else {
+                fnb = a.facenb[e];
+            }
             const float4 m = med[ctype];
             const float mua = m.x, mus = m.y, nidx = m.w;
             const float rc = nidx * R_C0F;
             const float currweight = w;
             raytet++;
             if (!NODAL) {
-                const float4* P = a.planes + 4 * (size_t)e;
-                const float4 NX = P[0], NY = P[1], NZ = P[2], DD = P[3];
+                const float4 NX = cNX, NY = cNY, NZ = cNZ, DD = cDD;
                 float T[4];
                 #pragma unroll
                 for (int i = 0; i < 4; i++) {
@@ -405,7 +415,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     const uint64_t foff = hot.fieldoff;
                     if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
                     if (newidx != oldidx) {
-                        deposit<COMP>(fcopy, lcopy, foff + oldidx, oldw);
+                        /* the first step after a face crossing finds the
+                         * register already flushed (oldw == 0): the reference
+                         * adds that zero (src/mmc_raytrace.c:1562-1612); x + 0
+                         * == x for the non-negative field, so the atomic is
+                         * skipped, which halves the atomics */
+                        if (oldw != 0.f) deposit<COMP>(fcopy, lcopy, foff + oldidx, oldw);
                         oldidx = newidx;
                         oldw = ww;
                     } else {
