@@ -18,7 +18,7 @@ from .sim import ResidentMesh, SimConfig, tune_struct
 _cache: Dict[Tuple, dict] = {}
 
 BLOCKS = (256,)
-BLOCKS_PER_CU = (0, 5, 6, 8)    # 0 = the default register budget; 5/6/8 select kernels compiled for that residency
+BLOCKS_PER_CU = (0, 5)          # 0 = the default register budget (4 per CU); 5 selects the kernel compiled for 5
 CHUNKS = (1, 8)
 
 

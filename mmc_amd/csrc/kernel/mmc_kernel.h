@@ -107,9 +107,10 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
 }
 
 /* OCC: minimum resident workgroups of kBlock threads per CU the register
- * allocation must allow (1 = no constraint; 5, 6, 8 trade registers, and at
- * 6 and 8 some scratch, for more resident waves) */
-template <bool NODAL, bool COMP, int OCC>
+ * allocation must allow (1 = no constraint; 5 trades registers for a fifth
+ * workgroup).  STAGE: per-simulation tables staged in LDS (small batches) or
+ * read through the caches (large batches, whose tables would crowd LDS). */
+template <bool NODAL, bool COMP, int OCC, bool STAGE>
 __global__ void transport(KArgs a);
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
                              uint64_t repstride);

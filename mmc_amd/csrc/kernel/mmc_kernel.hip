@@ -122,14 +122,18 @@ __device__ __forceinline__ uint32_t xcd_id() {
  * deposits (src/mmc_raytrace.c:227-508); otherwise the branch-less Badouel
  * tracer with element deposits (src/mmc_raytrace.c:1414-1709).
  */
-template <bool NODAL, bool COMP, int OCC>
+template <bool NODAL, bool COMP, int OCC, bool STAGE>
 __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nsim = a.nsim;
     const int nmed = a.prop + 1;
+    /* per-simulation tables: staged in LDS for batches whose tables are
+     * small, read through L1/L2 otherwise (lanes of a wave mostly share a
+     * simulation, so those reads broadcast) */
+    const int nstage = STAGE ? nsim : 0;
     SimHot* s_hot = (SimHot*)smem;
-    float4* s_med = (float4*)(s_hot + nsim);
-    int64_t* s_cum = (int64_t*)(s_med + (size_t)nsim * nmed);
+    float4* s_med = (float4*)(s_hot + nstage);
+    int64_t* s_cum = (int64_t*)(s_med + (size_t)nstage * nmed);
     float* s_lane = (float*)(s_cum + nsim + 1);
     const int nthr = blockDim.x;
     const int tid = threadIdx.x;
@@ -141,8 +145,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     const int slot_ns = slot_w0 + 2;
     const int nslots = slot_w0 + 3;
 
-    for (int i = tid; i < nsim; i += nthr) s_hot[i] = a.hot[i];
-    for (int i = tid; i < nsim * nmed; i += nthr) s_med[i] = a.media[i];
+    for (int i = tid; i < nstage; i += nthr) s_hot[i] = a.hot[i];
+    for (int i = tid; i < nstage * nmed; i += nthr) s_med[i] = a.media[i];
     for (int i = tid; i <= nsim; i += nthr) s_cum[i] = a.cum[i];
     for (int k = 0; k < nslots; k++) *LS(k) = 0.f;
     __syncthreads();
@@ -328,8 +332,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         if (__ballot(alive) == 0) break;
         if (!alive) continue;
 
-        const SimHot hot = s_hot[sim];
-        const float4* med = s_med + (size_t)sim * nmed;
+        const SimHot hot = STAGE ? s_hot[sim] : a.hot[sim];
+        const size_t mbase = (size_t)sim * nmed;
+        auto med = [&](int k) -> float4 { return STAGE ? s_med[mbase + k] : a.media[mbase + k]; };
         int reason = 0;
         int faceid = -1, isend = 0, undef = 1, nexteid = 0, faceidx = 4;
         float poutx = 0.f, pouty = 0.f, poutz = 0.f;
@@ -355,7 +360,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
             } else {
                 fnb = a.facenb[e];
             }
-            const float4 m = med[ctype];
+            const float4 m = med(ctype);
             const float mua = m.x, mus = m.y, nidx = m.w;
             const float rc = nidx * R_C0F;
             const float currweight = w;
@@ -611,8 +616,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 int nbp = nexteid;
                 int neid = nb_id(nbp);
                 int ntype = neid > 0 ? nb_type(nbp) : 0;
-                float nold = med[oldtype].w;
-                if (isreflect && (neid <= 0 || med[ntype].w != nold)) {
+                float nold = med(oldtype).w;
+                if (isreflect && (neid <= 0 || med(ntype).w != nold)) {
                     if (!(neid <= 0 && ((nold == hot.nout && isreflect != MMC_BC_MIRROR) ||
                                         isreflect == MMC_BC_ABSORBEXT))) {
                         /* reflectray (src/mmc_raytrace.c:2257-2332) */
@@ -621,7 +626,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         float pnx = pick4f(P[0], f), pny = pick4f(P[1], f), pnz = pick4f(P[2], f);
                         float Icos = fabsf(vx * pnx + vy * pny + vz * pnz);
                         float n1 = nold;
-                        float n2 = (neid > 0) ? med[ntype].w : hot.nout;
+                        float n2 = (neid > 0) ? med(ntype).w : hot.nout;
                         float t0 = n1 * n1, t1 = n2 * n2;
                         float t2 = 1.f - t0 / t1 * (1.f - Icos * Icos);
                         if (t2 > 0.f && !(neid <= 0 && isreflect == MMC_BC_MIRROR)) {
@@ -681,7 +686,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 }
                 if (!reason) {
                     /* mc_next_scatter (src/mmc_mesh.c:1644-1715) */
-                    const float g = med[ctype].z;
+                    const float g = med(ctype).z;
                     float sphi, cphi, stheta, ctheta, tmp0, tmp1;
                     float nextslen = rng_scatlen(r0, r1);
                     tmp0 = mmc_twopi(rng_next(r0, r1));
@@ -807,14 +812,13 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-#define MMC_INSTANTIATE(OCC)                                              \
-    template __global__ void transport<false, false, OCC>(KArgs);         \
-    template __global__ void transport<false, true, OCC>(KArgs);          \
-    template __global__ void transport<true, false, OCC>(KArgs);          \
-    template __global__ void transport<true, true, OCC>(KArgs);
-MMC_INSTANTIATE(1)
-MMC_INSTANTIATE(5)
-MMC_INSTANTIATE(6)
-MMC_INSTANTIATE(8)
+#define MMC_INSTANTIATE(OCC, STAGE)                                          \
+    template __global__ void transport<false, false, OCC, STAGE>(KArgs);     \
+    template __global__ void transport<false, true, OCC, STAGE>(KArgs);      \
+    template __global__ void transport<true, false, OCC, STAGE>(KArgs);      \
+    template __global__ void transport<true, true, OCC, STAGE>(KArgs);
+MMC_INSTANTIATE(1, true)
+MMC_INSTANTIATE(5, true)
+MMC_INSTANTIATE(1, false)
 
 }  // namespace mmc

@@ -553,24 +553,28 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     a.maxsteps = MMC_MAX_STEPS;
 
     const int nslots = kLaneSlots + 2 * m->prop + 3;
-    size_t shmem = sizeof(SimHot) * nsim + sizeof(float4) * (size_t)nsim * nmed +
-                   sizeof(int64_t) * (nsim + 1) + sizeof(float) * (size_t)nslots * block;
+    /* per-simulation tables go to LDS while they take at most 16 KiB, so that
+     * large batches keep 4+ workgroups per CU */
+    const size_t simtab = (sizeof(SimHot) + sizeof(float4) * (size_t)nmed) * nsim;
+    const bool stage = simtab <= 16384;
+    size_t shmem = (stage ? simtab : 0) + sizeof(int64_t) * (nsim + 1) + sizeof(float) * (size_t)nslots * block;
     shmem = (shmem + 15) & ~(size_t)15;
     if (shmem > 160 * 1024) return set_err(-1, "batch of %d simulations needs %zu B of LDS", nsim, shmem);
 
     /* the register budget of the kernel variant follows the requested
-     * residency: 5, 6 or 8 workgroups per CU select variants compiled for it */
+     * residency: 5 or more workgroups per CU select the variant compiled for
+     * 5; large batches read their tables through the caches (STAGE=false) */
     const int want = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : 0;
-    const int occ = want >= 8 ? 8 : want >= 6 ? 6 : want == 5 ? 5 : 1;
     const void* kfn = nullptr;
-#define MMC_PICK(O)                                                                              \
-    kfn = nodal ? (comp ? (const void*)transport<true, true, O> : (const void*)transport<true, false, O>) \
-                : (comp ? (const void*)transport<false, true, O> : (const void*)transport<false, false, O>);
-    switch (occ) {
-        case 5: MMC_PICK(5) break;
-        case 6: MMC_PICK(6) break;
-        case 8: MMC_PICK(8) break;
-        default: MMC_PICK(1) break;
+#define MMC_PICK(O, S)                                                                               \
+    kfn = nodal ? (comp ? (const void*)transport<true, true, O, S> : (const void*)transport<true, false, O, S>) \
+                : (comp ? (const void*)transport<false, true, O, S> : (const void*)transport<false, false, O, S>);
+    if (!stage) {
+        MMC_PICK(1, false)
+    } else if (want >= 5) {
+        MMC_PICK(5, true)
+    } else {
+        MMC_PICK(1, true)
     }
 #undef MMC_PICK
     int dev = 0, ncu = 0, maxbpc = 0;
