@@ -70,6 +70,25 @@ int32_t mmc_host_save_bin(const char* fname, const double* field, int64_t count)
 int32_t mmc_host_save_mch(const char* fname, const mmc_mch_header* his, const float* data,
                           int64_t count);
 
+/* Output formats of -F (enum TOutputFormat order, src/mmc_utils.h:112). */
+#define MMC_FMT_ASCII 0   /* <session>.dat, "%d\t%e" per value */
+#define MMC_FMT_BIN   1   /* <session>.bin, raw doubles */
+#define MMC_FMT_NII   2   /* <session>.nii, NIfTI-1, dim = (srcnum, maxgate, datalen) */
+#define MMC_FMT_HDR   3   /* <session>.hdr + .img, Analyze 7.5 */
+#define MMC_FMT_MC2   4   /* <session>.mc2, raw doubles */
+#define MMC_FMT_TX3   5   /* <session>.tx3, GL_RGBA32F + int dim[3] + doubles */
+#define MMC_FMT_JNII  6   /* <session>.jnii, JNIfTI JSON (zlib + base64, or raw base64) */
+#define MMC_FMT_BNII  7   /* <session>.bnii, JNIfTI binary JSON (BJData draft 2) */
+/* Saves a [srcnum][maxgate][datalen] double field as <session>.<ext>
+ * (mesh_saveweight + mcx_savedata, src/mmc_mesh.c:1726-1800,
+ * src/mmc_utils.c:919-993); compress selects zlib for jnii/bnii. */
+int32_t mmc_host_save_field(const char* session, int32_t format, const double* field, int32_t datalen,
+                            int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype,
+                            int32_t compress);
+/* Index of a -F name ("ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii",
+ * "bnii"), or -1. */
+int32_t mmc_host_format_id(const char* name);
+
 #ifdef __cplusplus
 }
 #endif

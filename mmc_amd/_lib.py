@@ -113,6 +113,9 @@ SIGNATURES = {
     "mmc_host_save_dat": (C.c_int32, [C.c_char_p, f64p, C.c_int32, C.c_int32]),
     "mmc_host_save_bin": (C.c_int32, [C.c_char_p, f64p, C.c_int64]),
     "mmc_host_save_mch": (C.c_int32, [C.c_char_p, C.c_void_p, f32p, C.c_int64]),
+    "mmc_host_save_field": (C.c_int32, [C.c_char_p, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_float, C.c_int32, C.c_int32]),
+    "mmc_host_format_id": (C.c_int32, [C.c_char_p]),
 }
 
 

@@ -7,7 +7,8 @@
  * prepares the mesh (include/mmc_host.h), runs the HIP transport through the
  * C ABI (include/mmc_hip.h) and writes the reference's outputs:
  * <session>.dat (ascii, "%d\t%e" per node/element per gate, src/mmc_mesh.c:
- * 1726-1800), <session>.bin (float64, src/mmc_utils.c:919-993) and
+ * 1726-1800), or .bin/.nii/.hdr+.img/.mc2/.tx3/.jnii/.bnii (src/mmc_utils.c:
+ * 919-993, through mmc_host_save_field) and
  * <session>.mch (history header + detected-photon records,
  * src/mmc_utils.c:4309-4342).
  */
@@ -29,7 +30,8 @@ typedef struct {
     char meshid[512];
     char rootdir[1024];
     char method;           /* 'p' or 's' */
-    int isnormalized, issave2pt, outputformat; /* 0 ascii, 1 bin */
+    int isnormalized, issave2pt, outputformat; /* MMC_FMT_* (enum TOutputFormat order) */
+    int zip;                        /* jnii/bnii: 1 zlib (default), 0 raw */
     long seed;
     float unitinmm;
     /* mesh given inline in the JSON */
@@ -70,6 +72,7 @@ static void defaults(cli_cfg* c) {
     c->sim.voidtime = 1;
     c->sim.maxdetphoton = 1000000;
     c->sim.srcnum = 1;
+    c->zip = 1;                      /* zipid = zmZlib, src/mmc_utils.c:236 */
     snprintf(c->session, sizeof(c->session), "default");
 }
 
@@ -203,7 +206,10 @@ static void load_json(cli_cfg* c, const char* path) {
         c->sim.isspecular = (int)mj_num(mj_get(Session, "DoSpecular"), c->sim.isspecular);
         c->sim.issaveexit = (int)mj_num(mj_get(Session, "DoSaveExit"), c->sim.issaveexit);
         c->sim.basisorder = (int)mj_num(mj_get(Session, "BasisOrder"), c->sim.basisorder);
-        if ((v = mj_get(Session, "OutputFormat"))) c->outputformat = !strcmp(mj_str(v, "ascii"), "bin");
+        if ((v = mj_get(Session, "OutputFormat"))) {
+            c->outputformat = mmc_host_format_id(mj_str(v, "ascii"));
+            if (c->outputformat < 0) die("unsupported Session.OutputFormat '%s'", mj_str(v, ""));
+        }
         if ((v = mj_get(Session, "RayTracer"))) c->method = mj_str(v, "p")[0] | 0x20;
         if ((v = mj_get(Session, "OutputType"))) {
             char t = mj_str(v, "x")[0] | 0x20;
@@ -274,7 +280,8 @@ static void usage(void) {
         "  -M/--method [P|S]     Plucker (nodal, -C 1) or branch-less Badouel (element, -C 0)\n"
         "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
         "  -O/--outputtype [X|F|E] flux, fluence or energy deposit\n"
-        "  -F/--outputformat [ascii|bin]\n"
+        "  -F/--outputformat [ascii|bin|nii|hdr|mc2|tx3|jnii|bnii]\n"
+        "  -Z/--zip [zlib|base64]   jnii/bnii payload: zlib-compressed (default) or raw\n"
         "  -b/--reflect [0|1|2|3] boundary: no reflect, Fresnel, absorb exterior, mirror\n"
         "  -U/--normalize [0|1]  normalize the output\n"
         "  -S/--save2pt [0|1]    save the fluence field\n"
@@ -326,8 +333,15 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
             char t = opt_value(argc, argv, &i)[0] | 0x20;
             c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
-        } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) c.outputformat = !strcmp(opt_value(argc, argv, &i), "bin");
-        else if (!strcmp(a, "-b") || !strcmp(a, "--reflect")) c.sim.isreflect = atoi(opt_value(argc, argv, &i));
+        } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) {
+            const char* f = opt_value(argc, argv, &i);
+            if ((c.outputformat = mmc_host_format_id(f)) < 0) die("unsupported output format '%s'", f);
+        } else if (!strcmp(a, "-Z") || !strcmp(a, "--zip")) {
+            const char* z = opt_value(argc, argv, &i);
+            if (!strcmp(z, "zlib")) c.zip = 1;
+            else if (!strcmp(z, "base64")) c.zip = 0;
+            else die("unsupported compression '%s' (zlib or base64)", z);
+        } else if (!strcmp(a, "-b") || !strcmp(a, "--reflect")) c.sim.isreflect = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-U") || !strcmp(a, "--normalize")) c.isnormalized = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-S") || !strcmp(a, "--save2pt")) c.issave2pt = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-d") || !strcmp(a, "--savedet")) c.sim.issavedet = atoi(opt_value(argc, argv, &i));
@@ -427,13 +441,9 @@ int main(int argc, char** argv) {
            100.0 * res.absorbweight / res.launchweight, normalizer);
     char fname[1024];
     if (c.issave2pt) {
-        if (c.outputformat) {
-            snprintf(fname, sizeof(fname), "%s.bin", c.session);
-            if (mmc_host_save_bin(fname, field, (int64_t)datalen * c.sim.maxgate)) die("can not write %s", fname);
-        } else {
-            snprintf(fname, sizeof(fname), "%s.dat", c.session);
-            if (mmc_host_save_dat(fname, field, datalen, c.sim.maxgate)) die("can not write %s", fname);
-        }
+        if (mmc_host_save_field(c.session, c.outputformat, field, datalen, c.sim.maxgate, 1, c.sim.tstep,
+                                c.sim.outputtype, c.zip))
+            die("can not write the output of session %s", c.session);
     }
     if (c.sim.issavedet) {
         int64_t saved = res.detected < c.sim.maxdetphoton ? res.detected : c.sim.maxdetphoton;

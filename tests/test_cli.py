@@ -23,6 +23,10 @@ def test_help_and_errors(tmp_path):
     r = run(["-f", str(bad)], tmp_path)
     assert r.returncode != 0 and "JSON error" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--zip", "lz4"], tmp_path)
+    assert r.returncode != 0 and "unsupported compression" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-F", "nifti"], tmp_path)
+    assert r.returncode != 0 and "unsupported output format" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--replaydet", "1"], tmp_path)
     assert r.returncode != 0 and "unsupported option" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-G", "-1"], tmp_path)
     assert r.returncode != 0 and "CPU path" in r.stderr
@@ -64,3 +68,33 @@ def test_onecube_mesh_files_nodal_binary(tmp_path):
     f = np.fromfile(tmp_path / "oc.bin", np.float64)
     assert f.shape == (8 * 10,)                       # 8 nodes x 10 gates
     assert np.all(f >= 0) and f.sum() > 0
+
+
+@pytest.mark.gpu
+def test_every_output_format_holds_the_same_field(tmp_path):
+    # one run per -F format; the fields agree to the FP32 atomic summation order
+    import json
+    import zlib
+    import base64
+    fields = {}
+    for fmt in ("bin", "nii", "hdr", "mc2", "tx3", "jnii"):
+        r = run(["-f", os.path.join(GOLDEN, "onecube", "onecube.json"), "-n", "20000", "-M", "P", "-C", "1",
+                 "-F", fmt, "-s", "oc_" + fmt], tmp_path)
+        assert r.returncode == 0, r.stderr
+        p = tmp_path / ("oc_" + fmt + "." + fmt)
+        if fmt in ("bin", "mc2"):
+            fields[fmt] = np.fromfile(p, np.float64)
+        elif fmt == "nii":
+            fields[fmt] = np.frombuffer(p.read_bytes()[352:], np.float64)
+        elif fmt == "hdr":
+            fields[fmt] = np.fromfile(tmp_path / "oc_hdr.img", np.float64)
+        elif fmt == "tx3":
+            fields[fmt] = np.frombuffer(p.read_bytes()[16:], np.float64)
+        else:
+            d = json.load(open(p))["NIFTIData"]
+            assert d["_ArraySize_"] == [8, 10]
+            fields[fmt] = np.frombuffer(zlib.decompress(base64.b64decode(d["_ArrayZipData_"])), np.float64)
+    ref = fields["bin"]
+    assert ref.shape == (80,) and ref.sum() > 0
+    for fmt, f in fields.items():
+        assert np.allclose(f, ref, rtol=1e-5, atol=1e-7 * ref.max()), fmt
