@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 2
+#define MMC_HIP_ABI_VERSION 3
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -170,6 +170,22 @@ int32_t mmc_hip_arena_release(mmc_hip_arena_handle a);
 int32_t mmc_hip_run_chunks(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_t nsim,
                            mmc_hip_arena_handle arena, const uint64_t* fieldoff,
                            const mmc_hip_tune* tune, mmc_hip_result* res);
+/* Launch slots (no reference counterpart): a stream with reusable device
+ * buffers.  mmc_hip_run_chunks_async enqueues the same launch as
+ * mmc_hip_run_chunks on the slot and returns at once; mmc_hip_slot_wait
+ * waits for it and adds its results into res (the same array, kept alive
+ * meanwhile); mmc_hip_slot_query returns 1 when the launch has finished, 0
+ * while it runs.  With two slots a server keeps the next launch queued
+ * behind the running one, so the next launch's workgroups start on the CUs
+ * the running launch's last photons leave idle.  One host thread per slot. */
+typedef struct mmc_hip_slotres* mmc_hip_slot_handle;
+int32_t mmc_hip_slot_create(mmc_hip_slot_handle* out);
+int32_t mmc_hip_slot_release(mmc_hip_slot_handle s);
+int32_t mmc_hip_run_chunks_async(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_t nsim,
+                                 mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                                 const mmc_hip_tune* tune, mmc_hip_result* res, mmc_hip_slot_handle slot);
+int32_t mmc_hip_slot_query(mmc_hip_slot_handle slot);
+int32_t mmc_hip_slot_wait(mmc_hip_slot_handle slot, mmc_hip_result* res);
 /* FP64 reduction of arena[off, off+n): adds it into host (if not NULL),
  * returns its sum, and zeroes the region for reuse. */
 int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, uint64_t n, double* host, double* sum);

@@ -99,6 +99,12 @@ SIGNATURES = {
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
                                        C.POINTER(TuneC), C.POINTER(ResultC)]),
     "mmc_hip_arena_collect": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, f64p, f64p]),
+    "mmc_hip_slot_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
+    "mmc_hip_slot_release": (C.c_int32, [C.c_void_p]),
+    "mmc_hip_run_chunks_async": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
+                                             C.POINTER(TuneC), C.POINTER(ResultC), C.c_void_p]),
+    "mmc_hip_slot_query": (C.c_int32, [C.c_void_p]),
+    "mmc_hip_slot_wait": (C.c_int32, [C.c_void_p, C.POINTER(ResultC)]),
     "mmc_host_volume": (None, [C.c_int32, f32p, C.c_int32, i32p, i32p, f32p, f32p]),
     "mmc_host_facenb": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p]),
     "mmc_host_planes": (None, [C.c_int32, f32p, i32p, f32p]),
@@ -119,7 +125,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 2        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 3        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

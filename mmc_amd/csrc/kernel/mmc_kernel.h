@@ -107,8 +107,8 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
 }
 
 /* OCC: minimum resident workgroups of kBlock threads per CU the register
- * allocation must allow (1 = no constraint; 5 trades registers for a fifth
- * workgroup).  STAGE: per-simulation tables staged in LDS (small batches) or
+ * allocation must allow (4 = 128 VGPRs, the default; 5 trades registers for
+ * a fifth workgroup).  STAGE: per-simulation tables staged in LDS (small batches) or
  * read through the caches (large batches, whose tables would crowd LDS). */
 template <bool NODAL, bool COMP, int OCC, bool STAGE>
 __global__ void transport(KArgs a);

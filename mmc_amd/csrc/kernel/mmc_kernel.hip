@@ -176,10 +176,23 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     int ctx = CTX_OUTER, fixcount = 0;
     float eabs = 0.f, eabs_c = 0.f;
     uint32_t raytet = 0, raytet_start = 0;
-    int cache_e = -1;                   /* element whose planes are in cNX..cfnb */
-    int4 cfnb = make_int4(0, 0, 0, 0);
+    int cache_e = -1;                   /* element whose geometry is in cNX..cfnb */
+    int4 cfnb = make_int4(0, 0, 0, 0), cee = cfnb;
     float4 cNX = make_float4(0.f, 0.f, 0.f, 0.f), cNY = cNX, cNZ = cNX, cDD = cNX;
+    /* pending nodal deposits (Plucker) to the nodes of element cache_e at
+     * field offset nbase */
+    float na0 = 0.f, na1 = 0.f, na2 = 0.f, na3 = 0.f;
+    uint64_t nbase = 0;
+    bool npend = false;
 
+    auto flush_nodal = [&]() {
+        /* x + 0 == x for the non-negative field: empty node sums are skipped */
+        if (na0 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.x - 1), na0);
+        if (na1 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.y - 1), na1);
+        if (na2 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.z - 1), na2);
+        if (na3 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.w - 1), na3);
+        npend = false;
+    };
     auto flush_sim = [&](int s) {
         if (s < 0) return;
         double* acc = a.simacc + (size_t)s * kAccCount;
@@ -346,20 +359,23 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         if (ctx >= 0) {
             /* ============================ ray-tetrahedron step ============ */
             const int e = eid - 1;
-            int4 fnb;
-            if (!NODAL) {
-                /* the element's planes and neighbours stay in registers while
-                 * the photon scatters inside it; reloaded after a crossing */
-                if (e != cache_e) {
+            /* the element's geometry and neighbours stay in registers while
+             * the photon scatters inside it; reloaded after a crossing
+             * (Badouel: the 4 planes; Plucker: the 12 node coordinates in
+             * cNX..cNZ and the node ids) */
+            if (e != cache_e) {
+                if (!NODAL) {
                     const float4* P = a.planes + 4 * (size_t)e;
                     cNX = P[0]; cNY = P[1]; cNZ = P[2]; cDD = P[3];
-                    cfnb = a.facenb[e];
-                    cache_e = e;
+                } else {
+                    const float4* Q = a.enode + 3 * (size_t)e;
+                    cNX = Q[0]; cNY = Q[1]; cNZ = Q[2];
+                    cee = a.elem[e];
                 }
-                fnb = cfnb;
-            } else {
-                fnb = a.facenb[e];
+                cfnb = a.facenb[e];
+                cache_e = e;
             }
+            const int4 fnb = cfnb;
             const float4 m = med(ctype);
             const float mua = m.x, mus = m.y, nidx = m.w;
             const float rc = nidx * R_C0F;
@@ -439,9 +455,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 if (faceid == -2) slen = 0.f;
             } else {
                 /* Plucker test over the 6 edges (src/mmc_raytrace.c:227-508) */
-                const int4 ee = a.elem[e];
-                const float4 q0 = a.enode[3 * (size_t)e], q1 = a.enode[3 * (size_t)e + 1],
-                             q2 = a.enode[3 * (size_t)e + 2];
+                const int4 ee = cee;
+                const float4 q0 = cNX, q1 = cNY, q2 = cNZ;
                 const float nxs[4] = {q0.x, q0.w, q1.z, q2.y};
                 const float nys[4] = {q0.y, q1.x, q1.w, q2.z};
                 const float nzs[4] = {q0.z, q1.y, q2.x, q2.w};
@@ -536,11 +551,22 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                 bo[2] = (1.f - ratio) * b2 + ratio * bo[2];
                                 bo[3] = (1.f - ratio) * b3 + ratio * bo[3];
                             }
-                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.x - 1), ww * (b0 + bo[0]));
-                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.y - 1), ww * (b1 + bo[1]));
-                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.z - 1), ww * (b2 + bo[2]));
-                            deposit<COMP>(fcopy, lcopy, base + (uint32_t)(ee.w - 1), ww * (b3 + bo[3]));
+                            /* the four nodal deposits of consecutive steps in
+                             * one element and gate add up in registers and
+                             * reach the field when the photon leaves the
+                             * element, changes gate or ends */
+                            const float d0 = ww * (b0 + bo[0]), d1 = ww * (b1 + bo[1]);
+                            const float d2 = ww * (b2 + bo[2]), d3 = ww * (b3 + bo[3]);
+                            if (npend && base != nbase) flush_nodal();
+                            if (npend) {
+                                na0 += d0; na1 += d1; na2 += d2; na3 += d3;
+                            } else {
+                                na0 = d0; na1 = d1; na2 = d2; na3 = d3;
+                                nbase = base;
+                                npend = true;
+                            }
                         }
+                        if (!isend && npend) flush_nodal();
                         if (isend) {
                             b0 = bo[0]; b1 = bo[1]; b2 = bo[2]; b3 = bo[3];
                         } else if (nb_id(nexteid) > 0 && faceidx < 4) {
@@ -730,6 +756,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
 
         /* ============================================ photon ends here ===== */
         {
+            if (NODAL && npend) flush_nodal();
             const SimCold& C = a.cold[sim];
             const float w0 = *LS(slot_w0);
             int exitdet = 0;
@@ -817,8 +844,8 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     template __global__ void transport<false, true, OCC, STAGE>(KArgs);      \
     template __global__ void transport<true, false, OCC, STAGE>(KArgs);      \
     template __global__ void transport<true, true, OCC, STAGE>(KArgs);
-MMC_INSTANTIATE(1, true)
+MMC_INSTANTIATE(4, true)
 MMC_INSTANTIATE(5, true)
-MMC_INSTANTIATE(1, false)
+MMC_INSTANTIATE(4, false)
 
 }  // namespace mmc

@@ -325,11 +325,35 @@ struct DevBuf {
     }
 };
 
+/* Device buffers that persist across launches and only grow. */
+struct Pooled {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        bytes = bytes ? bytes : 16;
+        if (bytes <= cap) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return set_err(-2, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        cap = bytes;
+        return 0;
+    }
+    ~Pooled() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 struct mmc_hip_arenares {
     float* field = nullptr;          /* nrep copies of n floats */
     float* lo = nullptr;
     uint64_t n = 0;
     int nrep = 1;
+    /* collection: its own non-blocking stream and grow-only buffers, so a
+     * collect neither waits for launches in flight nor frees device memory */
+    hipStream_t st = nullptr;
+    Pooled d64, dsum;
 };
 
 /* Field copies for a launch: one per XCD (8 on MI355X) while the copies fit
@@ -343,13 +367,66 @@ static int auto_replicas(uint64_t bytes_one_copy) {
     return 1;
 }
 
-/* One persistent launch over nsim simulations.  Without an arena the fields
- * are allocated here, reduced to FP64 and freed; with an arena they are
- * accumulated in place at the caller's offsets and left there. */
-static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+/* A launch slot: a stream, its timing events, the per-launch device buffers
+ * (reused by the next launch on the slot) and the host state of the launch
+ * in flight, which slot_finish needs to read the results back. */
+struct mmc_hip_slotres {
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Pooled d_hot, d_cold, d_med, d_cum, d_det, d_pat, d_acc, d_detcnt, d_queue, d_detdata, d_detph, d_rec;
+    /* launch in flight */
+    bool busy = false;
+    mmc_hip_mesh_handle m = nullptr;
+    mmc_hip_arenares* arena = nullptr;
+    int nsim = 0, nrep = 1, reclen = 0, savedet = 0;
+    bool comp = false, has_rec = false;
+    int64_t maxdet = 0, recfirst = 0, reccount = 0;
+    uint64_t fieldlen = 0;
+    std::vector<int64_t> cum;
+    std::vector<uint64_t> fieldoff;
+    std::vector<int64_t> datalen, maxdetph, photon_offset, nphoton;
+    std::vector<int32_t> maxgate;
+    DevBuf field, lo;                 /* own fields of a launch without an arena */
+    /* host sources of the launch's asynchronous copies, alive until it ends */
+    std::vector<SimHot> h_hot;
+    std::vector<SimCold> h_cold;
+    std::vector<float4> h_media, h_dets;
+    std::vector<float> h_pats;
+    std::chrono::steady_clock::time_point t_start;
+    ~mmc_hip_slotres() {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+static int slot_init(mmc_hip_slotres* sl) {
+    /* non-blocking: a slot's launch neither waits for nor holds up work on
+     * the null stream or on other slots */
+    HIPCHK(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&sl->e0));
+    HIPCHK(hipEventCreate(&sl->e1));
+    return 0;
+}
+
+/* Builds the tables of one persistent launch over nsim simulations, copies
+ * them and enqueues the kernel on the slot's stream; returns without
+ * waiting.  Without an arena the fields are allocated here and reduced to
+ * FP64 by slot_finish; with an arena they accumulate at the caller's
+ * offsets and stay there.  res is only read (record windows, detector
+ * buffers) and gets each simulation's reclen. */
+static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
                             const mmc_hip_tune* tune, mmc_hip_result* res, mmc_hip_arenares* arena,
                             const uint64_t* arena_off) {
-    auto t_start = std::chrono::steady_clock::now();
+    if (sl->busy) return set_err(-1, "launch slot busy: wait for its launch first");
+    sl->t_start = std::chrono::steady_clock::now();
+    /* own fields left by a launch that failed before it was enqueued */
+    if (sl->field.p) (void)hipFree(sl->field.p);
+    if (sl->lo.p) (void)hipFree(sl->lo.p);
+    sl->field.p = sl->lo.p = nullptr;
     if (!m || !sims || nsim < 1 || !res) return set_err(-1, "null argument");
     for (int i = 0; i < nsim; i++) {
         int rc = check_sim(m, sims + i, sims[0].method);
@@ -361,13 +438,19 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
     const bool comp = arena ? (arena->lo != nullptr) : (tune && tune->compensated);
 
-    /* ---- simulation tables ---- */
-    std::vector<SimHot> hot(nsim);
-    std::vector<SimCold> cold(nsim);
-    std::vector<float4> media((size_t)nsim * nmed);
-    std::vector<int64_t> cum(nsim + 1, 0);
-    std::vector<float4> dets;
-    std::vector<float> pats;
+    /* ---- simulation tables (kept in the slot until the launch ends) ---- */
+    std::vector<SimHot>& hot = sl->h_hot;
+    std::vector<SimCold>& cold = sl->h_cold;
+    std::vector<float4>& media = sl->h_media;
+    std::vector<int64_t>& cum = sl->cum;
+    std::vector<float4>& dets = sl->h_dets;
+    std::vector<float>& pats = sl->h_pats;
+    hot.assign(nsim, SimHot());
+    cold.assign(nsim, SimCold());
+    media.assign((size_t)nsim * nmed, float4());
+    cum.assign(nsim + 1, 0);
+    dets.clear();
+    pats.clear();
     uint64_t fieldlen = 0;
     std::vector<uint64_t> fieldoff(nsim);
     int savedet = 0, maxrec = 0;
@@ -458,21 +541,21 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     } else {
         nrep = auto_replicas((uint64_t)fieldlen * sizeof(float) * (comp ? 2 : 1));
     }
-    DevBuf dhot, dcold, dmed, dcum, ddet, dpat, dfield, dlo, dacc, ddetdata, ddetph, ddetcnt, dq, drec;
-    int64_t dummy = 0;
     int rc;
-    if ((rc = dalloc((SimHot**)&dhot.p, nsim, &dummy)) || (rc = dalloc((SimCold**)&dcold.p, nsim, &dummy)) ||
-        (rc = dalloc((float4**)&dmed.p, media.size(), &dummy)) || (rc = dalloc((int64_t**)&dcum.p, cum.size(), &dummy)) ||
-        (rc = dalloc((float4**)&ddet.p, dets.size(), &dummy)) || (rc = dalloc((float**)&dpat.p, pats.size(), &dummy)) ||
-        (!arena && (rc = dalloc((float**)&dfield.p, (size_t)fieldlen * nrep, &dummy))) ||
-        (rc = dalloc((double**)&dacc.p, (size_t)nsim * kAccCount, &dummy)) ||
-        (rc = dalloc((unsigned long long**)&ddetcnt.p, (size_t)nsim + 1, &dummy)) ||
-        (rc = dalloc((unsigned long long**)&dq.p, 1, &dummy)))
+    if ((rc = sl->d_hot.ensure(sizeof(SimHot) * nsim)) || (rc = sl->d_cold.ensure(sizeof(SimCold) * nsim)) ||
+        (rc = sl->d_med.ensure(sizeof(float4) * media.size())) || (rc = sl->d_cum.ensure(sizeof(int64_t) * cum.size())) ||
+        (rc = sl->d_det.ensure(sizeof(float4) * dets.size())) || (rc = sl->d_pat.ensure(sizeof(float) * pats.size())) ||
+        (rc = sl->d_acc.ensure(sizeof(double) * nsim * kAccCount)) ||
+        (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 1))) ||
+        (rc = sl->d_queue.ensure(sizeof(unsigned long long))))
         return rc;
-    if (!arena && comp && (rc = dalloc((float**)&dlo.p, (size_t)fieldlen * nrep, &dummy))) return rc;
+    if (!arena) {
+        if ((rc = dalloc((float**)&sl->field.p, (size_t)fieldlen * nrep, nullptr))) return rc;
+        if (comp && (rc = dalloc((float**)&sl->lo.p, (size_t)fieldlen * nrep, nullptr))) return rc;
+    }
     if (savedet && maxdet > 0) {
-        if ((rc = dalloc((float**)&ddetdata.p, (size_t)maxdet * reclen, &dummy)) ||
-            (rc = dalloc((int64_t**)&ddetph.p, (size_t)maxdet, &dummy)))
+        if ((rc = sl->d_detdata.ensure(sizeof(float) * (size_t)maxdet * reclen)) ||
+            (rc = sl->d_detph.ensure(sizeof(int64_t) * (size_t)maxdet)))
             return rc;
     }
     /* per-photon records: one window, taken from the first simulation's request */
@@ -483,30 +566,25 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
             reccount = std::min(res[i].reccount, sims[i].nphoton - res[i].recfirst);
             break;
         }
-    if (reccount > 0 && (rc = dalloc((mmc_hip_photon**)&drec.p, (size_t)reccount, &dummy))) return rc;
+    if (reccount > 0 && (rc = sl->d_rec.ensure(sizeof(mmc_hip_photon) * (size_t)reccount))) return rc;
 
-    hipStream_t st;
-    HIPCHK(hipStreamCreate(&st));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg{st};
-    HIPCHK(hipMemcpyAsync(dhot.p, hot.data(), sizeof(SimHot) * nsim, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dcold.p, cold.data(), sizeof(SimCold) * nsim, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dmed.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dcum.p, cum.data(), sizeof(int64_t) * cum.size(), hipMemcpyHostToDevice, st));
+    hipStream_t st = sl->st;
+    HIPCHK(hipMemcpyAsync(sl->d_hot.p, hot.data(), sizeof(SimHot) * nsim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl->d_cold.p, cold.data(), sizeof(SimCold) * nsim, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl->d_med.p, media.data(), sizeof(float4) * media.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sl->d_cum.p, cum.data(), sizeof(int64_t) * cum.size(), hipMemcpyHostToDevice, st));
     if (!dets.empty())
-        HIPCHK(hipMemcpyAsync(ddet.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sl->d_det.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
     if (!pats.empty())
-        HIPCHK(hipMemcpyAsync(dpat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sl->d_pat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
     if (!arena) {
-        HIPCHK(hipMemsetAsync(dfield.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
-        if (dlo.p) HIPCHK(hipMemsetAsync(dlo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
+        HIPCHK(hipMemsetAsync(sl->field.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
+        if (sl->lo.p) HIPCHK(hipMemsetAsync(sl->lo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
     }
-    HIPCHK(hipMemsetAsync(dacc.p, 0, sizeof(double) * nsim * kAccCount, st));
-    hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)dq.p, 1);
+    HIPCHK(hipMemsetAsync(sl->d_acc.p, 0, sizeof(double) * nsim * kAccCount, st));
+    hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)sl->d_queue.p, 1);
     hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 64) / 64), dim3(64), 0, st,
-                       (unsigned long long*)ddetcnt.p, nsim + 1);
+                       (unsigned long long*)sl->d_detcnt.p, nsim + 1);
 
     KArgs a;
     memset(&a, 0, sizeof(a));
@@ -527,27 +605,27 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     a.ne = m->ne;
     a.prop = m->prop;
     a.nsim = nsim;
-    a.hot = (const SimHot*)dhot.p;
-    a.cold = (const SimCold*)dcold.p;
-    a.media = (const float4*)dmed.p;
-    a.cum = (const int64_t*)dcum.p;
-    a.detpos = (const float4*)ddet.p;
-    a.pattern = (const float*)dpat.p;
-    a.field = arena ? arena->field : (float*)dfield.p;
-    a.field_lo = arena ? arena->lo : (float*)dlo.p;
+    a.hot = (const SimHot*)sl->d_hot.p;
+    a.cold = (const SimCold*)sl->d_cold.p;
+    a.media = (const float4*)sl->d_med.p;
+    a.cum = (const int64_t*)sl->d_cum.p;
+    a.detpos = (const float4*)sl->d_det.p;
+    a.pattern = (const float*)sl->d_pat.p;
+    a.field = arena ? arena->field : (float*)sl->field.p;
+    a.field_lo = arena ? arena->lo : (float*)sl->lo.p;
     a.nrep = nrep;
     a.repstride = arena ? arena->n : fieldlen;
-    a.simacc = (double*)dacc.p;
-    a.detdata = (float*)ddetdata.p;
-    a.detphoton = (int64_t*)ddetph.p;
-    a.detcount = (unsigned long long*)ddetcnt.p;
+    a.simacc = (double*)sl->d_acc.p;
+    a.detdata = (savedet && maxdet > 0) ? (float*)sl->d_detdata.p : nullptr;
+    a.detphoton = (savedet && maxdet > 0) ? (int64_t*)sl->d_detph.p : nullptr;
+    a.detcount = (unsigned long long*)sl->d_detcnt.p;
     a.maxdet = (savedet && maxdet > 0) ? maxdet : 0;
     a.reclen = reclen;
     a.savedet = savedet;
-    a.rec = (mmc_hip_photon*)drec.p;
+    a.rec = reccount > 0 ? (mmc_hip_photon*)sl->d_rec.p : nullptr;
     a.recfirst = recfirst;
     a.reccount = reccount;
-    a.queue = (unsigned long long*)dq.p;
+    a.queue = (unsigned long long*)sl->d_queue.p;
     a.total = total;
     a.chunk = (tune && tune->chunk > 0) ? tune->chunk : 1;
     a.maxsteps = MMC_MAX_STEPS;
@@ -562,19 +640,20 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     if (shmem > 160 * 1024) return set_err(-1, "batch of %d simulations needs %zu B of LDS", nsim, shmem);
 
     /* the register budget of the kernel variant follows the requested
-     * residency: 5 or more workgroups per CU select the variant compiled for
-     * 5; large batches read their tables through the caches (STAGE=false) */
+     * residency: 4 workgroups per CU by default (128 VGPRs), 5 or more select
+     * the variant compiled for 5; large batches read their tables through the
+     * caches (STAGE=false) */
     const int want = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : 0;
     const void* kfn = nullptr;
 #define MMC_PICK(O, S)                                                                               \
     kfn = nodal ? (comp ? (const void*)transport<true, true, O, S> : (const void*)transport<true, false, O, S>) \
                 : (comp ? (const void*)transport<false, true, O, S> : (const void*)transport<false, false, O, S>);
     if (!stage) {
-        MMC_PICK(1, false)
+        MMC_PICK(4, false)
     } else if (want >= 5) {
         MMC_PICK(5, true)
     } else {
-        MMC_PICK(1, true)
+        MMC_PICK(4, true)
     }
 #undef MMC_PICK
     int dev = 0, ncu = 0, maxbpc = 0;
@@ -584,52 +663,67 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
     int bpc = want > 0 ? std::min(want, std::max(maxbpc, 1)) : std::max(maxbpc, 1);
     int grid = ncu * bpc;
 
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, st));
+    HIPCHK(hipEventRecord(sl->e0, st));
     if (total > 0) {
         void* kargs[] = {&a};
         HIPCHK(hipLaunchKernel(kfn, dim3(grid), dim3(block), kargs, shmem, st));
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventRecord(sl->e1, st));
 
-    /* ---- final FP64 reduction and copies ---- */
+    /* state slot_finish needs */
+    sl->busy = true;
+    sl->m = m;
+    sl->arena = arena;
+    sl->nsim = nsim;
+    sl->nrep = nrep;
+    sl->reclen = reclen;
+    sl->savedet = savedet;
+    sl->comp = comp;
+    sl->maxdet = maxdet;
+    sl->recfirst = recfirst;
+    sl->reccount = reccount;
+    sl->fieldlen = fieldlen;
+    sl->fieldoff = fieldoff;
+    sl->datalen.resize(nsim);
+    sl->maxdetph.resize(nsim);
+    sl->photon_offset.resize(nsim);
+    sl->nphoton.resize(nsim);
+    sl->maxgate.resize(nsim);
+    for (int i = 0; i < nsim; i++) {
+        sl->datalen[i] = datalen_of(m, sims + i);
+        sl->maxdetph[i] = sims[i].maxdetphoton;
+        sl->photon_offset[i] = sims[i].photon_offset;
+        sl->nphoton[i] = sims[i].nphoton;
+        sl->maxgate[i] = sims[i].maxgate;
+    }
+    return 0;
+}
+
+/* Waits for the slot's launch and adds its results into res[0..nsim):
+ * energy tallies, ray-tet counts, kernel and wall times, detected records,
+ * per-photon records and (without an arena) the FP64 field and its sum. */
+static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
+    if (!sl->busy) return set_err(-1, "launch slot idle: nothing to wait for");
+    sl->busy = false;
+    struct FieldRelease {
+        mmc_hip_slotres* s;
+        ~FieldRelease() {
+            if (s->field.p) (void)hipFree(s->field.p);
+            if (s->lo.p) (void)hipFree(s->lo.p);
+            s->field.p = s->lo.p = nullptr;
+        }
+    } fr{sl};
+    const int nsim = sl->nsim;
+    hipStream_t st = sl->st;
     std::vector<double> acc((size_t)nsim * kAccCount);
     std::vector<unsigned long long> cnt(nsim + 1);
-    HIPCHK(hipMemcpyAsync(acc.data(), dacc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cnt.data(), ddetcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(acc.data(), sl->d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cnt.data(), sl->d_detcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     float kms = 0.f;
-    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-
-    /* FP64 final reduction of every simulation's field on the device; the
-     * host copy happens only when the caller passed a field array */
-    DevBuf d64, dsum;
-    if (arena) {
-        for (int i = 0; i < nsim; i++) {
-            mmc_hip_result& r = res[i];
-            const double* A = acc.data() + (size_t)i * kAccCount;
-            r.launchweight += A[kAccLaunch];
-            r.absorbweight += A[kAccAbsorb];
-            r.escapeweight += A[kAccEscape];
-            r.roulettegain += A[kAccRGain];
-            r.raytet += A[kAccRaytet];
-            r.detected += (int64_t)cnt[1 + i];
-            r.kernel_ms += kms;
-        }
-    }
-    uint64_t maxn = 0;
-    for (int i = 0; i < nsim; i++) maxn = std::max<uint64_t>(maxn, (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate);
-    if (!arena && ((rc = dalloc((double**)&d64.p, (size_t)std::max<uint64_t>(maxn, 1), &dummy)) ||
-                   (rc = dalloc((double**)&dsum.p, (size_t)nsim, &dummy))))
-        return rc;
-    if (!arena) HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
-    std::vector<double> sums(nsim, 0.0);
-    for (int i = 0; i < nsim && !arena; i++) {
+    HIPCHK(hipEventElapsedTime(&kms, sl->e0, sl->e1));
+    for (int i = 0; i < nsim; i++) {
         mmc_hip_result& r = res[i];
         const double* A = acc.data() + (size_t)i * kAccCount;
         r.launchweight += A[kAccLaunch];
@@ -639,54 +733,87 @@ static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int3
         r.raytet += A[kAccRaytet];
         r.detected += (int64_t)cnt[1 + i];
         r.kernel_ms += kms;
-        uint64_t n = (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate;
-        if (n == 0 || sims[i].nphoton == 0) continue;
-        int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
-        hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
-                           (float*)dfield.p + fieldoff[i], dlo.p ? (float*)dlo.p + fieldoff[i] : nullptr,
-                           (double*)d64.p, n, (double*)dsum.p + i, nrep, (uint64_t)fieldlen);
-        HIPCHK(hipGetLastError());
-        if (r.field) {
-            std::vector<double> hbuf(n);
-            HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
-        }
     }
-    if (!arena) {
+
+    /* FP64 final reduction of every simulation's own field on the device; the
+     * host copy happens only when the caller passed a field array */
+    if (!sl->arena) {
+        int dev = 0, ncu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        uint64_t maxn = 1;
+        for (int i = 0; i < nsim; i++) maxn = std::max<uint64_t>(maxn, (uint64_t)sl->datalen[i] * sl->maxgate[i]);
+        DevBuf d64, dsum;
+        int rc;
+        if ((rc = dalloc((double**)&d64.p, (size_t)maxn, nullptr)) || (rc = dalloc((double**)&dsum.p, (size_t)nsim, nullptr)))
+            return rc;
+        HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
+        for (int i = 0; i < nsim; i++) {
+            mmc_hip_result& r = res[i];
+            uint64_t n = (uint64_t)sl->datalen[i] * sl->maxgate[i];
+            if (n == 0 || sl->nphoton[i] == 0) continue;
+            int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
+            hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
+                               (float*)sl->field.p + sl->fieldoff[i],
+                               sl->lo.p ? (float*)sl->lo.p + sl->fieldoff[i] : nullptr, (double*)d64.p, n,
+                               (double*)dsum.p + i, sl->nrep, (uint64_t)sl->fieldlen);
+            HIPCHK(hipGetLastError());
+            if (r.field) {
+                std::vector<double> hbuf(n);
+                HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
+            }
+        }
+        std::vector<double> sums(nsim, 0.0);
         HIPCHK(hipMemcpyAsync(sums.data(), dsum.p, sizeof(double) * nsim, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         for (int i = 0; i < nsim; i++) res[i].fieldsum += sums[i];
     }
     /* detected photons: saved records are split per simulation by photon id range */
-    if (savedet && maxdet > 0 && cnt[0] > 0) {
-        int64_t nsaved = std::min<int64_t>((int64_t)cnt[0], maxdet);
+    if (sl->savedet && sl->maxdet > 0 && cnt[0] > 0) {
+        const int reclen = sl->reclen;
+        int64_t nsaved = std::min<int64_t>((int64_t)cnt[0], sl->maxdet);
         std::vector<float> dd((size_t)nsaved * reclen);
         std::vector<int64_t> dp((size_t)nsaved);
-        HIPCHK(hipMemcpy(dd.data(), ddetdata.p, sizeof(float) * dd.size(), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(dp.data(), ddetph.p, sizeof(int64_t) * dp.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(dd.data(), sl->d_detdata.p, sizeof(float) * dd.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(dp.data(), sl->d_detph.p, sizeof(int64_t) * dp.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         std::vector<int64_t> fill(nsim, 0);
         for (int64_t k = 0; k < nsaved; k++) {
             /* records carry the global photon index; cum[] maps it to its simulation */
-            int s = (int)(std::upper_bound(cum.begin(), cum.end(), dp[k]) - cum.begin()) - 1;
+            int s = (int)(std::upper_bound(sl->cum.begin(), sl->cum.end(), dp[k]) - sl->cum.begin()) - 1;
             if (s < 0 || s >= nsim) continue;
             mmc_hip_result& r = res[s];
-            if (!r.detdata || fill[s] >= sims[s].maxdetphoton) continue;
+            if (!r.detdata || fill[s] >= sl->maxdetph[s]) continue;
             memcpy(r.detdata + (size_t)fill[s] * r.reclen, dd.data() + (size_t)k * reclen, sizeof(float) * r.reclen);
-            if (r.detphoton) r.detphoton[fill[s]] = sims[s].photon_offset + (dp[k] - cum[s]);
+            if (r.detphoton) r.detphoton[fill[s]] = sl->photon_offset[s] + (dp[k] - sl->cum[s]);
             fill[s]++;
         }
     }
-    if (reccount > 0) {
+    if (sl->reccount > 0) {
         for (int i = 0; i < nsim; i++)
             if (res[i].photons && res[i].reccount > 0) {
-                HIPCHK(hipMemcpy(res[i].photons, drec.p, sizeof(mmc_hip_photon) * reccount, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpyAsync(res[i].photons, sl->d_rec.p, sizeof(mmc_hip_photon) * sl->reccount,
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
                 break;
             }
     }
-    double tms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    double tms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sl->t_start).count();
     for (int i = 0; i < nsim; i++) res[i].total_ms += tms;
     return 0;
+}
+
+/* Synchronous launch on a temporary slot. */
+static int32_t launch_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                            const mmc_hip_tune* tune, mmc_hip_result* res, mmc_hip_arenares* arena,
+                            const uint64_t* arena_off) {
+    mmc_hip_slotres sl;
+    int rc = slot_init(&sl);
+    if (!rc) rc = slot_enqueue(&sl, m, sims, nsim, tune, res, arena, arena_off);
+    if (!rc) rc = slot_finish(&sl, res);
+    return rc;
 }
 
 extern "C" int32_t mmc_hip_run_batch(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
@@ -703,7 +830,9 @@ extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, in
     a->nrep = replicas;
     const size_t total = (size_t)nfloat * replicas;
     int64_t dummy = 0;
-    int rc = dalloc(&a->field, total, &dummy);
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&a->st, hipStreamNonBlocking) != hipSuccess) rc = set_err(-3, "arena stream creation failed");
+    if (!rc) rc = dalloc(&a->field, total, &dummy);
     if (!rc && compensated) rc = dalloc(&a->lo, total, &dummy);
     if (!rc) {
         hipError_t e = hipMemset(a->field, 0, sizeof(float) * total);
@@ -713,6 +842,7 @@ extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, in
     if (rc) {
         if (a->field) (void)hipFree(a->field);
         if (a->lo) (void)hipFree(a->lo);
+        if (a->st) (void)hipStreamDestroy(a->st);
         delete a;
         return rc;
     }
@@ -722,6 +852,10 @@ extern "C" int32_t mmc_hip_arena_create(uint64_t nfloat, int32_t compensated, in
 
 extern "C" int32_t mmc_hip_arena_release(mmc_hip_arena_handle a) {
     if (!a) return 0;
+    if (a->st) {
+        (void)hipStreamSynchronize(a->st);
+        (void)hipStreamDestroy(a->st);
+    }
     if (a->field) (void)hipFree(a->field);
     if (a->lo) (void)hipFree(a->lo);
     delete a;
@@ -735,6 +869,45 @@ extern "C" int32_t mmc_hip_run_chunks(mmc_hip_mesh_handle m, const mmc_hip_sim* 
     return launch_batch(m, sims, nsim, tune, res, arena, fieldoff);
 }
 
+extern "C" int32_t mmc_hip_slot_create(mmc_hip_slot_handle* out) {
+    if (!out) return set_err(-1, "null argument");
+    mmc_hip_slotres* s = new mmc_hip_slotres();
+    int rc = slot_init(s);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_slot_release(mmc_hip_slot_handle s) {
+    delete s;
+    return 0;
+}
+
+extern "C" int32_t mmc_hip_run_chunks_async(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                                            mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                                            const mmc_hip_tune* tune, mmc_hip_result* res,
+                                            mmc_hip_slot_handle slot) {
+    if (!arena || !fieldoff || !slot) return set_err(-1, "null arena or slot");
+    return slot_enqueue(slot, m, sims, nsim, tune, res, arena, fieldoff);
+}
+
+extern "C" int32_t mmc_hip_slot_query(mmc_hip_slot_handle slot) {
+    if (!slot) return set_err(-1, "null slot");
+    if (!slot->busy) return 1;
+    hipError_t e = hipEventQuery(slot->e1);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return set_err(-100 - (int)e, "slot query failed: %s", hipGetErrorString(e));
+}
+
+extern "C" int32_t mmc_hip_slot_wait(mmc_hip_slot_handle slot, mmc_hip_result* res) {
+    if (!slot || !res) return set_err(-1, "null argument");
+    return slot_finish(slot, res);
+}
+
 extern "C" int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, uint64_t n, double* host,
                                          double* sum) {
     if (!a || off + n > a->n) return set_err(-1, "collect outside the arena");
@@ -742,34 +915,24 @@ extern "C" int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, u
         if (sum) *sum = 0.0;
         return 0;
     }
-    double *d64 = nullptr, *dsum = nullptr;
-    int64_t dummy = 0;
-    int rc = dalloc(&d64, (size_t)n, &dummy);
-    if (!rc) rc = dalloc(&dsum, 1, &dummy);
-    if (rc) {
-        if (d64) (void)hipFree(d64);
-        return rc;
-    }
-    hipError_t e = hipMemset(dsum, 0, sizeof(double));
+    int rc;
+    if ((rc = a->d64.ensure(sizeof(double) * n)) || (rc = a->dsum.ensure(sizeof(double)))) return rc;
     int dev = 0, ncu = 0;
-    if (!e) e = hipGetDevice(&dev);
-    if (!e) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (!e) {
-        int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)std::max(ncu, 1) * 16);
-        hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, 0, a->field + off,
-                           a->lo ? a->lo + off : nullptr, d64, n, dsum, a->nrep, a->n);
-        e = hipGetLastError();
-    }
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipMemsetAsync(a->dsum.p, 0, sizeof(double), a->st));
+    int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)std::max(ncu, 1) * 16);
+    hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, a->st, a->field + off,
+                       a->lo ? a->lo + off : nullptr, (double*)a->d64.p, n, (double*)a->dsum.p, a->nrep, a->n);
+    HIPCHK(hipGetLastError());
     std::vector<double> hbuf;
     double s = 0.0;
-    if (!e && host) {
+    if (host) {
         hbuf.resize(n);
-        e = hipMemcpy(hbuf.data(), d64, sizeof(double) * n, hipMemcpyDeviceToHost);
+        HIPCHK(hipMemcpyAsync(hbuf.data(), a->d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, a->st));
     }
-    if (!e) e = hipMemcpy(&s, dsum, sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d64);
-    (void)hipFree(dsum);
-    if (e) return set_err(-3, "collect failed: %s", hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(&s, a->dsum.p, sizeof(double), hipMemcpyDeviceToHost, a->st));
+    HIPCHK(hipStreamSynchronize(a->st));
     if (host)
         for (uint64_t k = 0; k < n; k++) host[k] += hbuf[k];
     if (sum) *sum = s;

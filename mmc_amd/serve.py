@@ -41,7 +41,8 @@ class Request:
     # filled by the server
     flen: int = 0
     off: int = -1
-    done: int = 0
+    assigned: int = 0                 # photons handed to launches
+    done: int = 0                     # photons whose launches have finished
     admitted: float = -1.0
     finished: float = -1.0
     launches: int = 0
@@ -140,13 +141,17 @@ class Scheduler:
     def plan(self):
         """Pick the mesh of the oldest admitted request and give every admitted
         request on that mesh a share of the launch's photon budget."""
-        if not self.active:
+        open_ = self.plannable()
+        if not open_:
             return None, []
-        mesh = min(self.active, key=lambda r: (r.arrival, r.rid)).mesh
-        group = [r for r in self.active if r.mesh == mesh]
+        mesh = min(open_, key=lambda r: (r.arrival, r.rid)).mesh
+        group = [r for r in open_ if r.mesh == mesh]
         share = max(self.min_chunk, self.slice_photons // len(group))
-        chunks = [(r, min(share, r.cfg.nphoton - r.done)) for r in group]
+        chunks = [(r, min(share, r.cfg.nphoton - r.assigned)) for r in group]
         return mesh, [(r, n) for r, n in chunks if n > 0]
+
+    def plannable(self) -> List[Request]:
+        return [r for r in self.active if r.assigned < r.cfg.nphoton]
 
     def complete(self, r: Request, now: float):
         self.active.remove(r)
@@ -173,43 +178,74 @@ class Server:
         self.tune = tune if tune is not None else tune_struct()
         self.launches = 0
         self.kernel_ms = 0.0
+        # two launch slots: the next launch waits queued behind the running
+        # one and its workgroups fill the CUs the running launch's tail frees
+        self.slots = []
+        for _ in range(2):
+            sh = C.c_void_p()
+            _lib.check(L.mmc_hip_slot_create(C.byref(sh)), "mmc_hip_slot_create")
+            self.slots.append(sh)
+        self.inflight = deque()       # (slot, chunks, results, keep-alive)
 
     def close(self):
+        L = _lib.lib()
+        while self.inflight:
+            self._finish()
+        for sh in self.slots:
+            L.mmc_hip_slot_release(sh)
+        self.slots = []
         for r in self.res.values():
             r.release()
         if self.arena:
-            _lib.lib().mmc_hip_arena_release(self.arena)
+            L.mmc_hip_arena_release(self.arena)
             self.arena = None
 
     def _launch(self, mesh: str, chunks):
+        """Enqueue one launch on a free slot (returns at once)."""
         L = _lib.lib()
+        busy = {id(x[0]) for x in self.inflight}
+        slot = next(sh for sh in self.slots if id(sh) not in busy)
         rm = self.res[mesh]
         n = len(chunks)
         sims = (SimC * n)()
         offs = np.zeros(n, np.uint64)
-        keep = []
+        keep = [sims, offs]
         for i, (r, cnt) in enumerate(chunks):
             c = copy.copy(r.cfg)
             c.nphoton = cnt
-            c.photon_offset = r.cfg.photon_offset + r.done
-            s = c.cstruct(rm.mesh)
-            keep.append(s._keep)
-            sims[i] = s
+            c.photon_offset = r.cfg.photon_offset + r.assigned
+            st = c.cstruct(rm.mesh)
+            keep.append(st._keep)
+            sims[i] = st
             offs[i] = r.off
         res = (ResultC * n)()
-        _lib.check(L.mmc_hip_run_chunks(rm.handle, sims, n, self.arena, ptr(offs, C.c_uint64), C.byref(self.tune),
-                                        res), "mmc_hip_run_chunks")
+        _lib.check(L.mmc_hip_run_chunks_async(rm.handle, sims, n, self.arena, ptr(offs, C.c_uint64),
+                                              C.byref(self.tune), res, slot), "mmc_hip_run_chunks_async")
+        for r, cnt in chunks:
+            r.assigned += cnt
+            r.launches += 1
+        self.inflight.append((slot, chunks, res, keep))
+        self.launches += 1
+
+    def _finish(self) -> List[Request]:
+        """Wait for the oldest launch in flight; returns the requests whose
+        last photons it ran."""
+        L = _lib.lib()
+        slot, chunks, res, keep = self.inflight.popleft()
+        _lib.check(L.mmc_hip_slot_wait(slot, res), "mmc_hip_slot_wait")
+        finished = []
         for i, (r, cnt) in enumerate(chunks):
             q = res[i]
             r.done += cnt
-            r.launches += 1
             r.launchweight += q.launchweight
             r.absorbweight += q.absorbweight
             r.escapeweight += q.escapeweight
             r.raytet += q.raytet
             r.detected += q.detected
-        self.launches += 1
+            if r.done >= r.cfg.nphoton:
+                finished.append(r)
         self.kernel_ms += res[0].kernel_ms
+        return finished
 
     def _collect(self, r: Request):
         L = _lib.lib()
@@ -225,27 +261,37 @@ class Server:
 
     def serve(self, requests: List[Request]) -> dict:
         """Run a recorded arrival stream (arrival times in seconds from now)
-        to completion; returns latency and throughput statistics."""
+        to completion; returns latency and throughput statistics.  Up to two
+        launches are in flight; the host plans and enqueues the next launch
+        while the current one runs."""
+        L = _lib.lib()
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         done = []
-        while pending or self.sched.waiting or self.sched.active:
+        while pending or self.sched.waiting or self.sched.active or self.inflight:
             while pending and pending[0].arrival <= now():
                 r = pending.popleft()
                 self.sched.arrive(r, self.meshes[r.mesh])
             self.sched.admit(now())
-            mesh, chunks = self.sched.plan()
-            if not chunks:
+            if len(self.inflight) < len(self.slots):
+                mesh, chunks = self.sched.plan()
+                if chunks:
+                    self._launch(mesh, chunks)
+                    continue
+            if not self.inflight:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
                 continue
-            self._launch(mesh, chunks)
-            for r, _ in chunks:
-                if r.done >= r.cfg.nphoton:
-                    self._collect(r)
-                    self.sched.complete(r, now())
-                    done.append(r)
+            # both slots busy, or nothing more to plan: take the oldest launch
+            # once it is done (polling, so that arrivals are admitted meanwhile)
+            if len(self.inflight) < len(self.slots) and pending and L.mmc_hip_slot_query(self.inflight[0][0]) == 0:
+                time.sleep(0.0005)
+                continue
+            for r in self._finish():
+                self._collect(r)
+                self.sched.complete(r, now())
+                done.append(r)
         wall = now()
         return summarize(done, wall, self.launches, self.kernel_ms)
 

@@ -37,7 +37,7 @@ def test_admission_by_field_bytes_and_fifo_order():
     assert [r.rid for r in s.waiting] == [1, 2]
     mesh, chunks = s.plan()
     assert mesh == "a" and [(r.rid, n) for r, n in chunks] == [(0, 1000)]
-    reqs[0].done = 5000
+    reqs[0].assigned = reqs[0].done = 5000
     s.complete(reqs[0], 1.0)
     admitted = s.admit(1.0)
     assert [r.rid for r in admitted] == [1, 2]   # FIFO: 1 first, then 2 fits too
@@ -61,3 +61,18 @@ def test_fair_share_of_the_slice():
     s.admit(0.0)
     _, chunks = s.plan()
     assert [n for _, n in chunks] == [250, 150, 250, 250]
+
+
+def test_plan_skips_requests_whose_photons_are_all_in_flight():
+    # with launches in flight, planning uses the photons not yet handed out
+    s = Scheduler(arena_floats=10**6, datalen={"a": 10, "b": 10}, slice_photons=1000, min_chunk=100)
+    r0, r1 = mkreq(0, "a", 1, 800, 0.0), mkreq(1, "b", 1, 5000, 0.1)
+    for r in (r0, r1):
+        s.arrive(r)
+    s.admit(0.2)
+    mesh, chunks = s.plan()
+    assert mesh == "a" and [(r.rid, n) for r, n in chunks] == [(0, 800)]
+    r0.assigned += 800                # launched, not finished
+    mesh, chunks = s.plan()
+    assert mesh == "b" and [(r.rid, n) for r, n in chunks] == [(1, 1000)]
+    assert r0 in s.active             # completes only when its launch finishes
