@@ -182,32 +182,47 @@ def bench_serve(args):
 
 def bench_large(args):
     """The large-scale workload: the 24.0M-tet jittered shells mesh, a 60x60 mm
-    planar source with 64x64 Bernoulli patterns, 100 gates of 0.05 ns,
-    Plucker nodal fluence with FP64 final reduction, 32 detectors, run as 8
-    pattern requests (args.large_photons in total) through the serving loop."""
+    planar source with 8 64x64 Bernoulli patterns batched onto every photon
+    (srcnum = 8), 100 gates of 0.05 ns, Plucker nodal fluence with FP64 final
+    reduction (8 slots per node and gate), 32 detectors, args.large_photons
+    photons in launches of args.large_slice."""
     import mmc_amd as M
-    from mmc_amd import serve as S
     from mmc_amd import workloads as W
     t = time.time()
     mesh = W.shells_mesh(1.0)
     log(f"shells mesh: {mesh.ne} tets, {mesh.nn} nodes, {len(mesh.srcelem)} launch candidates, "
         f"built in {time.time() - t:.1f} s")
-    sims = W.shells_sims(8, int(args.large_photons))
-    reqs = [S.Request(rid=i, mesh="shells", cfg=c, arrival=0.0) for i, c in enumerate(sims)]
-    srv = S.Server({"shells": mesh}, arena_bytes=16 << 30, slice_photons=int(args.large_slice))
-    log(f"mesh resident: {srv.res['shells'].device_bytes / 2**30:.2f} GiB")
-    stats = srv.serve(reqs)
-    srv.close()
-    steps = sum(r.raytet for r in reqs)
-    stats["steps_per_photon"] = steps / stats["photons"]
-    stats["detected"] = int(sum(r.detected for r in reqs))
-    stats["absorbed_fraction"] = sum(r.absorbweight for r in reqs) / sum(r.launchweight for r in reqs)
-    log(f"large: {stats}")
+    rm = M.ResidentMesh(mesh)
+    log(f"mesh resident: {rm.device_bytes / 2**30:.2f} GiB")
+    total = int(args.large_photons)
+    sl = int(args.large_slice)
+    photons = steps = absorbed = launched = kern = 0.0
+    detected = 0
+    t = time.perf_counter()
+    done = 0
+    while done < total:
+        n = min(sl, total - done)
+        cfg = W.shells_shared_sim(total)
+        cfg.nphoton, cfg.photon_offset, cfg.e0 = n, done, int(mesh.srcelem[0])
+        r = rm.run_batch([cfg], want_field=False)[0]
+        done += n
+        photons += n
+        steps += r.raytet
+        launched += r.launchweight
+        absorbed += r.absorbweight
+        kern += r.kernel_ms
+        detected += r.detected
+        log(f"large: {done:.3g}/{total:.3g} photons, launch {r.kernel_ms:.0f} ms")
+    wall = (time.perf_counter() - t) * 1e3
+    rm.release()
     return {"workload": f"jittered 200x200x100 mm box, 1 mm cells x6 = {mesh.ne / 1e6:.1f}M tets, 5 tissues in "
-                        "nested ellipsoidal shells, 60x60 mm planar source with 64x64 Bernoulli(0.5) patterns, "
-                        f"{args.large_photons:.0e} photons as 8 pattern requests, 100 gates x 0.05 ns, Plucker "
-                        "nodal fluence (FP32 atomics, FP64 reduction), 32 detectors",
-            "tets": int(mesh.ne), "nodes": int(mesh.nn), **stats}
+                        "nested ellipsoidal shells, 60x60 mm planar source with 8 64x64 Bernoulli(0.5) patterns "
+                        f"sharing every photon (srcnum 8), {total:.0e} photons, 100 gates x 0.05 ns, Plucker nodal "
+                        "fluence (FP32 atomics into 8 slots, FP64 reduction), 32 detectors",
+            "tets": int(mesh.ne), "nodes": int(mesh.nn), "photons": int(photons), "launches": -(-total // sl),
+            "wall_ms": wall, "kernel_ms": kern, "photons_per_ms": photons / wall,
+            "pattern_photons_per_ms": 8 * photons / wall, "steps_per_photon": steps / photons,
+            "absorbed_fraction": absorbed / launched, "detected": detected}
 
 
 def copy_probe(nbytes: int):
@@ -235,7 +250,7 @@ def main():
     ap.add_argument("--photons", type=int, default=100_000_000)
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "all", "large"])
     ap.add_argument("--large-photons", type=float, default=1e9)
-    ap.add_argument("--large-slice", type=float, default=2.4e7)
+    ap.add_argument("--large-slice", type=float, default=2.5e8)
     ap.add_argument("--serve-requests", type=int, default=8)
     ap.add_argument("--serve-max-photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")

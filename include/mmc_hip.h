@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 3
+#define MMC_HIP_ABI_VERSION 4
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -77,7 +77,10 @@ typedef struct mmc_hip_sim {
     int32_t e0;              /* initial element (cfg->e0) */
     float bary0[4];          /* its barycentric coordinates (cfg->bary0) */
     float srcpos[4], srcdir[4], srcparam1[4], srcparam2[4];
-    const float* srcpattern; /* srcparam1.w*srcparam2.w weights (pattern source) */
+    const float* srcpattern; /* pattern source: srcparam1.w*srcparam2.w pixels x srcnum patterns,
+                                pattern k of pixel p at [p*srcnum + k]; with srcnum > 1 each
+                                photon is shared by all patterns and the field holds srcnum
+                                slots per value, [gate][node or element][pattern] */
     float tstart, tend, tstep;
     int32_t maxgate;
     float nout, minenergy, roulettesize;
@@ -113,6 +116,8 @@ typedef struct mmc_hip_result {
     int64_t recfirst, reccount;
     double kernel_ms;        /* transport kernels only */
     double total_ms;         /* including uploads, reductions and copies */
+    double* srcweight;       /* pattern sources with srcnum > 1: launched energy of each pattern,
+                                then absorbed energy of each pattern (2*srcnum, +=), or NULL */
 } mmc_hip_result;
 
 /* Launch shape for the persistent kernel; zeros pick the defaults. */

@@ -62,6 +62,13 @@ double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
                           const int32_t* elem, const int32_t* type, const float* med,
                           const float* evol, const float* nvol, double* field,
                           double absorbweight, double launchweight);
+/* The same for pattern slot `pair` of a field shared by srcnum patterns
+ * (values strided by srcnum; mesh_normalize's pair argument,
+ * src/mmc_mesh.c:2154), with that pattern's absorbed and launched energy. */
+double mmc_host_normalize_pair(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
+                               const int32_t* elem, const int32_t* type, const float* med,
+                               const float* evol, const float* nvol, double* field,
+                               double absorbweight, double launchweight, int32_t pair);
 
 /* Writers.  Return 0 on success. */
 int32_t mmc_host_save_dat(const char* fname, const double* field, int32_t datalen,

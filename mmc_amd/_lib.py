@@ -70,7 +70,7 @@ class ResultC(C.Structure):
         ("escapeweight", C.c_double), ("roulettegain", C.c_double), ("raytet", C.c_double),
         ("detdata", f32p), ("detphoton", i64p), ("detected", C.c_int64), ("reclen", C.c_int32),
         ("photons", C.c_void_p), ("recfirst", C.c_int64), ("reccount", C.c_int64),
-        ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("srcweight", f64p),
     ]
 
 
@@ -116,6 +116,8 @@ SIGNATURES = {
     "mmc_host_number_exterior": (C.c_int32, [C.c_int32, i32p]),
     "mmc_host_normalize": (C.c_double, [C.POINTER(SimC), C.c_int32, C.c_int32, i32p, i32p, f32p,
                                         f32p, f32p, f64p, C.c_double, C.c_double]),
+    "mmc_host_normalize_pair": (C.c_double, [C.POINTER(SimC), C.c_int32, C.c_int32, i32p, i32p, f32p,
+                                             f32p, f32p, f64p, C.c_double, C.c_double, C.c_int32]),
     "mmc_host_save_dat": (C.c_int32, [C.c_char_p, f64p, C.c_int32, C.c_int32]),
     "mmc_host_save_bin": (C.c_int32, [C.c_char_p, f64p, C.c_int64]),
     "mmc_host_save_mch": (C.c_int32, [C.c_char_p, C.c_void_p, f32p, C.c_int64]),
@@ -125,7 +127,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 3        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 4        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

@@ -234,44 +234,58 @@ int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb) {
     return nf;
 }
 
-double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne, const int32_t* elem,
-                          const int32_t* type, const float* med, const float* evol,
-                          const float* nvol, double* field, double absorbweight,
-                          double launchweight) {
+double mmc_host_normalize_pair(const mmc_hip_sim* sim, int32_t nn, int32_t ne, const int32_t* elem,
+                               const int32_t* type, const float* med, const float* evol,
+                               const float* nvol, double* field, double absorbweight,
+                               double launchweight, int32_t pair) {
     /* mesh_normalize takes the energies as float (src/mmc_mesh.c:2154) */
     float Eabsorb = (float)absorbweight, Etotal = (float)launchweight;
-    int32_t datalen = sim->basisorder ? nn : ne;
-    size_t total = (size_t)datalen * sim->maxgate;
+    const int32_t datalen = sim->basisorder ? nn : ne;
+    const int32_t S = (sim->srctype == MMC_SRC_PATTERN && sim->srcnum > 1) ? sim->srcnum : 1;
+    if (pair < 0 || pair >= S) return 0.0;
+    /* value j of gate g of this pattern slot: field[(g*datalen + j)*S + pair] */
+#define F(g, j) field[((size_t)(g) * datalen + (j)) * S + pair]
     double deposit = 0.0, k;
     if (sim->outputtype == MMC_OUT_ENERGY) {
         k = 1.f / Etotal;
-        for (size_t i = 0; i < total; i++) field[i] *= k;
+        for (int32_t g = 0; g < sim->maxgate; g++)
+            for (int32_t j = 0; j < datalen; j++) F(g, j) *= k;
         return k;
     }
     if (sim->basisorder) {
         for (int32_t g = 0; g < sim->maxgate; g++)
             for (int32_t j = 0; j < nn; j++)
-                if (nvol[j] > 0.f) field[(size_t)g * nn + j] /= nvol[j];
+                if (nvol[j] > 0.f) F(g, j) /= nvol[j];
         for (int32_t i = 0; i < ne; i++) {
             const int32_t* e = elem + 4 * (size_t)i;
             double sum = 0.0;
             for (int32_t g = 0; g < sim->maxgate; g++)
-                for (int c = 0; c < 4; c++) sum += (float)field[(size_t)g * nn + e[c] - 1];
+                for (int c = 0; c < 4; c++) sum += (float)F(g, e[c] - 1);
             deposit += sum * evol[i] * med[4 * type[i]];
         }
         k = Eabsorb / (Etotal * deposit * 0.25f);
     } else {
         for (int32_t i = 0; i < ne; i++)
-            for (int32_t g = 0; g < sim->maxgate; g++) deposit += field[(size_t)g * ne + i];
+            for (int32_t g = 0; g < sim->maxgate; g++) deposit += F(g, i);
         for (int32_t i = 0; i < ne; i++) {
             double em = evol[i] * med[4 * type[i]];
-            for (int32_t g = 0; g < sim->maxgate; g++) field[(size_t)g * ne + i] /= em;
+            for (int32_t g = 0; g < sim->maxgate; g++) F(g, i) /= em;
         }
         k = Eabsorb / (Etotal * deposit);
     }
     if (sim->outputtype == MMC_OUT_FLUX) k /= sim->tstep;
-    for (size_t i = 0; i < total; i++) field[i] *= k;
+    for (int32_t g = 0; g < sim->maxgate; g++)
+        for (int32_t j = 0; j < datalen; j++) F(g, j) *= k;
+#undef F
     return k;
+}
+
+double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne, const int32_t* elem,
+                          const int32_t* type, const float* med, const float* evol,
+                          const float* nvol, double* field, double absorbweight,
+                          double launchweight) {
+    return mmc_host_normalize_pair(sim, nn, ne, elem, type, med, evol, nvol, field, absorbweight,
+                                   launchweight, 0);
 }
 
 int32_t mmc_host_save_dat(const char* fname, const double* field, int32_t datalen,

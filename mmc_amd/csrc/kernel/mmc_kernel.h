@@ -15,6 +15,7 @@ namespace mmc {
 constexpr int kBlock = 256;          /* default threads per workgroup (4 waves) */
 constexpr int kMaxProp = 15;         /* media labels 1..kMaxProp per mesh */
 constexpr int kLaneSlots = 8;        /* per-lane LDS accumulators (energy, Kahan terms) */
+constexpr int kMaxSrc = 8;           /* patterns shared by one photon (srcnum) */
 
 /* Per-simulation parameters read on every step or crossing; staged in LDS
  * (48 B).  flags: bit 0 roulette, 1 energy output, 2 save detectors,
@@ -27,7 +28,8 @@ struct SimHot {
     int32_t maxgate, flags;
     float nout;                      /* refractive index outside the mesh */
     uint64_t fieldoff;               /* float offset of this simulation's field */
-    uint64_t pad1;
+    int32_t srcnum;                  /* field slots per value: patterns sharing each photon */
+    int32_t patbase;                 /* this simulation's patterns in KArgs::pattern */
 };
 
 /* Per-simulation parameters read at launch / exit / deposit (global, cached). */
@@ -43,8 +45,11 @@ struct SimCold {
     uint64_t fieldoff;               /* float offset of this simulation's field */
 };
 
-/* Accumulators per simulation (FP64, final reduction only). */
+/* Accumulators per simulation (FP64, final reduction only), followed by the
+ * launched and absorbed energy of each shared pattern. */
 enum SimAcc { kAccLaunch = 0, kAccAbsorb, kAccEscape, kAccRGain, kAccRaytet, kAccCount };
+constexpr int kAccSrcLaunch = kAccCount, kAccSrcAbsorb = kAccCount + kMaxSrc;
+constexpr int kAccStride = kAccCount + 2 * kMaxSrc;
 
 struct KArgs {
     /* mesh, structure of arrays */
@@ -76,7 +81,7 @@ struct KArgs {
     float* field_lo;                 /* compensation buffer (TwoSum atomics) or null */
     uint64_t repstride;
     int32_t nrep, pad_rep;
-    double* simacc;                  /* nsim*kAccCount */
+    double* simacc;                  /* nsim*kAccStride */
     float* detdata;
     int64_t* detphoton;
     unsigned long long* detcount;    /* nsim+1 counters: [0] global slot counter */
@@ -89,6 +94,7 @@ struct KArgs {
     int64_t total;
     int32_t chunk;
     uint32_t maxsteps;               /* per-photon bound on ray-tetrahedron tests */
+    int32_t maxsrc;                  /* largest srcnum of the launch (per-lane pattern slots) */
 };
 
 /* Per-photon xorshift128+ state: splitmix64 finalizer over (seed, photon id).

@@ -143,7 +143,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     const int slot_w0 = kLaneSlots + 2 * a.prop;
     const int slot_rg = slot_w0 + 1;
     const int slot_ns = slot_w0 + 2;
-    const int nslots = slot_w0 + 3;
+    /* shared patterns (srcnum > 1): the photon's weight in each pattern, and
+     * this lane's launched and absorbed energy per pattern */
+    const int slot_pw = slot_w0 + 3;
+    const int slot_pl = slot_pw + a.maxsrc;
+    const int slot_pa = slot_pl + a.maxsrc;
+    const int nslots = slot_w0 + 3 + (a.maxsrc > 1 ? 3 * a.maxsrc : 0);
 
     for (int i = tid; i < nstage; i += nthr) s_hot[i] = a.hot[i];
     for (int i = tid; i < nstage * nmed; i += nthr) s_med[i] = a.media[i];
@@ -182,20 +187,36 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     /* pending nodal deposits (Plucker) to the nodes of element cache_e at
      * field offset nbase */
     float na0 = 0.f, na1 = 0.f, na2 = 0.f, na3 = 0.f;
-    uint64_t nbase = 0;
+    uint64_t nfoff = 0;                 /* field offset of the simulation */
+    uint32_t nloc = 0;                  /* gate * nn */
     bool npend = false;
+    int nsrc = 1;                       /* patterns sharing the photon (field slots) */
+    int posidx = 0;                     /* launch pixel of a pattern source */
 
+    /* deposit v at value loc of the simulation's field (offset foff); with
+     * shared patterns the value has nsrc slots and slot k receives v times
+     * the photon's weight in pattern k (src/mmc_raytrace.c:1586-1612) */
+    auto depo = [&](uint64_t foff, uint32_t loc, float v) {
+        if (nsrc == 1) {
+            deposit<COMP>(fcopy, lcopy, foff + loc, v);
+        } else {
+            for (int k = 0; k < nsrc; k++) {
+                const float pk = *LS(slot_pw + k);
+                if (pk != 0.f) deposit<COMP>(fcopy, lcopy, foff + (uint64_t)loc * nsrc + k, v * pk);
+            }
+        }
+    };
     auto flush_nodal = [&]() {
         /* x + 0 == x for the non-negative field: empty node sums are skipped */
-        if (na0 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.x - 1), na0);
-        if (na1 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.y - 1), na1);
-        if (na2 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.z - 1), na2);
-        if (na3 != 0.f) deposit<COMP>(fcopy, lcopy, nbase + (uint32_t)(cee.w - 1), na3);
+        if (na0 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.x - 1), na0);
+        if (na1 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.y - 1), na1);
+        if (na2 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.z - 1), na2);
+        if (na3 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.w - 1), na3);
         npend = false;
     };
     auto flush_sim = [&](int s) {
         if (s < 0) return;
-        double* acc = a.simacc + (size_t)s * kAccCount;
+        double* acc = a.simacc + (size_t)s * kAccStride;
         float v;
         v = *LS(0) - *LS(1); if (v != 0.f) atomicAdd(acc + kAccLaunch, (double)v);
         v = *LS(2) - *LS(3); if (v != 0.f) atomicAdd(acc + kAccAbsorb, (double)v);
@@ -203,6 +224,13 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         v = *LS(6);          if (v != 0.f) atomicAdd(acc + kAccRGain, (double)v);
         if (raytet) atomicAdd(acc + kAccRaytet, (double)raytet);
         for (int k = 0; k < kLaneSlots; k++) *LS(k) = 0.f;
+        if (a.maxsrc > 1)
+            for (int k = 0; k < a.maxsrc; k++) {
+                v = *LS(slot_pl + k); if (v != 0.f) atomicAdd(acc + kAccSrcLaunch + k, (double)v);
+                v = *LS(slot_pa + k); if (v != 0.f) atomicAdd(acc + kAccSrcAbsorb + k, (double)v);
+                *LS(slot_pl + k) = 0.f;
+                *LS(slot_pa + k) = 0.f;
+            }
         raytet = 0;
     };
 
@@ -247,6 +275,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     const SimCold& C = a.cold[sim];
                     uint64_t pid = (uint64_t)(C.photon_first + (g - s_cum[sim]));
                     photon_seed(C.seed, pid, r0, r1);
+                    nsrc = C.srcnum;
+                    posidx = 0;
                     /* launchphoton (src/mmc_raytrace.c:2346-2671) */
                     px = C.srcpos.x; py = C.srcpos.y; pz = C.srcpos.z;
                     vx = C.srcdir.x; vy = C.srcdir.y; vz = C.srcdir.z;
@@ -268,7 +298,14 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             int xs = C.patnx, ys = C.patny;
                             int iy = min((int)(ry * ys), ys - 1);
                             int ix = min((int)(rx * xs), xs - 1);
-                            w = (C.srcnum == 1) ? a.pattern[C.patbase + iy * xs + ix] : 1.f;
+                            posidx = iy * xs + ix;
+                            w = (C.srcnum == 1) ? a.pattern[C.patbase + posidx] : 1.f;
+                            if (nsrc > 1)
+                                for (int k = 0; k < nsrc; k++) {
+                                    const float pk = a.pattern[C.patbase + posidx * nsrc + k];
+                                    *LS(slot_pw + k) = pk;
+                                    *LS(slot_pl + k) += pk;
+                                }
                         }
                         px = EPSF * vx + 1.f * px;
                         py = EPSF * vy + 1.f * py;
@@ -441,14 +478,14 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                          * adds that zero (src/mmc_raytrace.c:1562-1612); x + 0
                          * == x for the non-negative field, so the atomic is
                          * skipped, which halves the atomics */
-                        if (oldw != 0.f) deposit<COMP>(fcopy, lcopy, foff + oldidx, oldw);
+                        if (oldw != 0.f) depo(foff, oldidx, oldw);
                         oldidx = newidx;
                         oldw = ww;
                     } else {
                         oldw += ww;
                     }
                     if (faceid == -2 || !isend) {
-                        deposit<COMP>(fcopy, lcopy, foff + newidx, oldw);
+                        depo(foff, newidx, oldw);
                         oldw = 0.f;
                     }
                 }
@@ -543,7 +580,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             if (!(hot.flags & kFlagEnergy))
                                 ww = (mua < EPSF) ? (currweight * Lmove) : ((currweight - w) / mua);
                             int g = (int)((tof - hot.tstart) * hot.rtstep);
-                            uint64_t base = hot.fieldoff + (uint64_t)max(min(g, hot.maxgate - 1), 0) * a.nn;
+                            const uint32_t loc = (uint32_t)max(min(g, hot.maxgate - 1), 0) * (uint32_t)a.nn;
                             ww *= 0.5f;
                             if (isend) {
                                 bo[0] = (1.f - ratio) * b0 + ratio * bo[0];
@@ -557,12 +594,13 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                              * element, changes gate or ends */
                             const float d0 = ww * (b0 + bo[0]), d1 = ww * (b1 + bo[1]);
                             const float d2 = ww * (b2 + bo[2]), d3 = ww * (b3 + bo[3]);
-                            if (npend && base != nbase) flush_nodal();
+                            if (npend && loc != nloc) flush_nodal();
                             if (npend) {
                                 na0 += d0; na1 += d1; na2 += d2; na3 += d3;
                             } else {
                                 na0 = d0; na1 = d1; na2 = d2; na3 = d3;
-                                nbase = base;
+                                nfoff = hot.fieldoff;
+                                nloc = loc;
                                 npend = true;
                             }
                         }
@@ -781,7 +819,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             row[col++] = px; row[col++] = py; row[col++] = pz;
                             row[col++] = vx; row[col++] = vy; row[col++] = vz;
                         }
-                        row[col] = w0;
+                        /* shared patterns record the launch pixel instead of
+                         * the launch weight (src/mmc_raytrace.c:1824) */
+                        row[col] = (nsrc > 1) ? __int_as_float(posidx) : w0;
                         if (a.detphoton) a.detphoton[slot] = gid;
                     }
                 }
@@ -790,6 +830,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
             float rg = *LS(slot_rg);
             if (reason == MMC_END_ROULETTE) rg -= w;
             kahan_lds(LS(2), LS(3), eabs);
+            if (nsrc > 1)
+                for (int k = 0; k < nsrc; k++) *LS(slot_pa + k) += eabs * *LS(slot_pw + k);
             kahan_lds(LS(4), LS(5), wend);
             *LS(6) += rg;
             if (a.rec && gid >= a.recfirst && gid < a.recfirst + a.reccount) {

@@ -280,8 +280,12 @@ extern "C" int32_t mmc_hip_mesh_release(mmc_hip_mesh_handle h) {
 
 extern "C" int64_t mmc_hip_mesh_bytes(mmc_hip_mesh_handle h) { return h ? h->bytes : 0; }
 
+/* field slots per value: the patterns that share each photon */
+static int slots_of(const mmc_hip_sim* s) { return s->srctype == MMC_SRC_PATTERN ? s->srcnum : 1; }
+
+/* values per gate (nodes or elements) times the pattern slots */
 static int64_t datalen_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
-    return s->basisorder ? m->nn : m->ne;
+    return (int64_t)(s->basisorder ? m->nn : m->ne) * slots_of(s);
 }
 
 extern "C" int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* s) {
@@ -307,7 +311,8 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
     if (s->srctype == MMC_SRC_PATTERN) {
         int nx = (int)s->srcparam1[3], ny = (int)s->srcparam2[3];
         if (!s->srcpattern || nx < 1 || ny < 1) return set_err(-1, "pattern source without pattern");
-        if (s->srcnum != 1) return set_err(-1, "only srcnum=1 patterns are supported");
+        if (s->srcnum < 1 || s->srcnum > kMaxSrc)
+            return set_err(-1, "srcnum %d outside 1..%d patterns", s->srcnum, kMaxSrc);
     }
     if (s->issavedet && s->detnum > 0 && !s->detpos) return set_err(-1, "detectors without positions");
     if (s->outputtype != MMC_OUT_FLUX && s->outputtype != MMC_OUT_FLUENCE && s->outputtype != MMC_OUT_ENERGY)
@@ -330,11 +335,15 @@ struct Pooled {
     void* p = nullptr;
     size_t cap = 0;
     int ensure(size_t bytes) {
-        bytes = bytes ? bytes : 16;
-        if (bytes <= cap) return 0;
+        if (bytes <= cap && p) return 0;
+        /* grow geometrically from 64 KiB: a free can wait for the device, so
+         * steady-state launches must not reallocate */
+        size_t want = std::max<size_t>(bytes, 64 << 10);
+        if (cap) want = std::max(want, 2 * cap);
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        bytes = want;
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) return set_err(-2, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
         cap = bytes;
@@ -385,7 +394,7 @@ struct mmc_hip_slotres {
     std::vector<int64_t> cum;
     std::vector<uint64_t> fieldoff;
     std::vector<int64_t> datalen, maxdetph, photon_offset, nphoton;
-    std::vector<int32_t> maxgate;
+    std::vector<int32_t> maxgate, nslot;
     DevBuf field, lo;                 /* own fields of a launch without an arena */
     /* host sources of the launch's asynchronous copies, alive until it ends */
     std::vector<SimHot> h_hot;
@@ -453,7 +462,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     pats.clear();
     uint64_t fieldlen = 0;
     std::vector<uint64_t> fieldoff(nsim);
-    int savedet = 0, maxrec = 0;
+    int savedet = 0, maxrec = 0, maxsrc = 1;
     int64_t maxdet = 0;
     for (int i = 0; i < nsim; i++) {
         const mmc_hip_sim& s = sims[i];
@@ -475,6 +484,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                   (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
                   ((s.isspecular & 3) << kSpecShift);
         h.nout = s.nout;
+        h.srcnum = slots_of(&s);
+        maxsrc = std::max(maxsrc, h.srcnum);
         const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate;
         if (arena) {
             if (arena_off[i] + flen > arena->n)
@@ -497,7 +508,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         c.roulettesize = s.roulettesize;
         c.e0 = s.e0;
         c.srctype = s.srctype;
-        c.srcnum = s.srcnum;
+        c.srcnum = slots_of(&s);
         c.isreflect = s.isreflect;
         c.isspecular = s.isspecular;
         c.voidtime = s.voidtime;
@@ -512,7 +523,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             c.patnx = (int)s.srcparam1[3];
             c.patny = (int)s.srcparam2[3];
             c.patbase = (int)pats.size();
-            pats.insert(pats.end(), s.srcpattern, s.srcpattern + (size_t)c.patnx * c.patny);
+            h.patbase = c.patbase;
+            pats.insert(pats.end(), s.srcpattern, s.srcpattern + (size_t)c.patnx * c.patny * c.srcnum);
         }
         c.nphoton = s.nphoton;
         c.photon_first = s.photon_offset;
@@ -545,7 +557,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     if ((rc = sl->d_hot.ensure(sizeof(SimHot) * nsim)) || (rc = sl->d_cold.ensure(sizeof(SimCold) * nsim)) ||
         (rc = sl->d_med.ensure(sizeof(float4) * media.size())) || (rc = sl->d_cum.ensure(sizeof(int64_t) * cum.size())) ||
         (rc = sl->d_det.ensure(sizeof(float4) * dets.size())) || (rc = sl->d_pat.ensure(sizeof(float) * pats.size())) ||
-        (rc = sl->d_acc.ensure(sizeof(double) * nsim * kAccCount)) ||
+        (rc = sl->d_acc.ensure(sizeof(double) * nsim * kAccStride)) ||
         (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 1))) ||
         (rc = sl->d_queue.ensure(sizeof(unsigned long long))))
         return rc;
@@ -581,7 +593,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         HIPCHK(hipMemsetAsync(sl->field.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
         if (sl->lo.p) HIPCHK(hipMemsetAsync(sl->lo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
     }
-    HIPCHK(hipMemsetAsync(sl->d_acc.p, 0, sizeof(double) * nsim * kAccCount, st));
+    HIPCHK(hipMemsetAsync(sl->d_acc.p, 0, sizeof(double) * nsim * kAccStride, st));
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)sl->d_queue.p, 1);
     hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 64) / 64), dim3(64), 0, st,
                        (unsigned long long*)sl->d_detcnt.p, nsim + 1);
@@ -629,8 +641,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.total = total;
     a.chunk = (tune && tune->chunk > 0) ? tune->chunk : 1;
     a.maxsteps = MMC_MAX_STEPS;
+    a.maxsrc = maxsrc;
 
-    const int nslots = kLaneSlots + 2 * m->prop + 3;
+    const int nslots = kLaneSlots + 2 * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that
      * large batches keep 4+ workgroups per CU */
     const size_t simtab = (sizeof(SimHot) + sizeof(float4) * (size_t)nmed) * nsim;
@@ -690,12 +703,14 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     sl->photon_offset.resize(nsim);
     sl->nphoton.resize(nsim);
     sl->maxgate.resize(nsim);
+    sl->nslot.resize(nsim);
     for (int i = 0; i < nsim; i++) {
         sl->datalen[i] = datalen_of(m, sims + i);
         sl->maxdetph[i] = sims[i].maxdetphoton;
         sl->photon_offset[i] = sims[i].photon_offset;
         sl->nphoton[i] = sims[i].nphoton;
         sl->maxgate[i] = sims[i].maxgate;
+        sl->nslot[i] = slots_of(sims + i);
     }
     return 0;
 }
@@ -716,7 +731,7 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
     } fr{sl};
     const int nsim = sl->nsim;
     hipStream_t st = sl->st;
-    std::vector<double> acc((size_t)nsim * kAccCount);
+    std::vector<double> acc((size_t)nsim * kAccStride);
     std::vector<unsigned long long> cnt(nsim + 1);
     HIPCHK(hipMemcpyAsync(acc.data(), sl->d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(cnt.data(), sl->d_detcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
@@ -725,7 +740,7 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
     HIPCHK(hipEventElapsedTime(&kms, sl->e0, sl->e1));
     for (int i = 0; i < nsim; i++) {
         mmc_hip_result& r = res[i];
-        const double* A = acc.data() + (size_t)i * kAccCount;
+        const double* A = acc.data() + (size_t)i * kAccStride;
         r.launchweight += A[kAccLaunch];
         r.absorbweight += A[kAccAbsorb];
         r.escapeweight += A[kAccEscape];
@@ -733,6 +748,11 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
         r.raytet += A[kAccRaytet];
         r.detected += (int64_t)cnt[1 + i];
         r.kernel_ms += kms;
+        if (r.srcweight && sl->nslot[i] > 1)
+            for (int k = 0; k < sl->nslot[i]; k++) {
+                r.srcweight[k] += A[kAccSrcLaunch + k];
+                r.srcweight[sl->nslot[i] + k] += A[kAccSrcAbsorb + k];
+            }
     }
 
     /* FP64 final reduction of every simulation's own field on the device; the

@@ -114,7 +114,7 @@ class Scheduler:
         return self.nodelen[r.mesh] if r.cfg.basisorder else self.datalen[r.mesh]
 
     def field_len(self, r: Request) -> int:
-        return self.rows(r) * r.cfg.gates()[0]
+        return self.rows(r) * r.cfg.gates()[0] * r.cfg.srcnum
 
     def arrive(self, r: Request, mesh: Optional[TetMesh] = None):
         if mesh is not None and r.cfg.e0 <= 0 and r.cfg.srctype == "pencil":
@@ -160,7 +160,7 @@ class Scheduler:
 
 
 class Server:
-    def __init__(self, meshes: Dict[str, TetMesh], arena_bytes: int = 8 << 30, slice_photons: int = 2_000_000,
+    def __init__(self, meshes: Dict[str, TetMesh], arena_bytes: int = 8 << 30, slice_photons: int = 8_000_000,
                  tune=None, compensated: bool = False, replicas: int = 8):
         """arena_bytes of FP32 field space for admitted requests, held as
         `replicas` per-XCD copies (device memory = arena_bytes * replicas,
@@ -251,7 +251,8 @@ class Server:
         L = _lib.lib()
         s = C.c_double()
         if r.want_field:
-            r.field = np.zeros((r.cfg.gates()[0], self.sched.rows(r)), np.float64)
+            shape = (r.cfg.gates()[0], self.sched.rows(r)) + ((r.cfg.srcnum,) if r.cfg.srcnum > 1 else ())
+            r.field = np.zeros(shape, np.float64)
             host = ptr(r.field, C.c_double)
         else:
             host = C.cast(None, f64p)

@@ -40,7 +40,7 @@ class SimConfig:
     srcdir: Sequence[float] = (0.0, 0.0, 1.0)
     srcparam1: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
     srcparam2: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
-    srcpattern: Optional[np.ndarray] = None
+    srcpattern: Optional[np.ndarray] = None   # (ny, nx), or (srcnum, ny, nx) patterns sharing photons
     tstart: float = 0.0
     tend: float = 5e-9
     tstep: float = 5e-9
@@ -57,6 +57,13 @@ class SimConfig:
     def __post_init__(self):
         if self.basisorder is None:
             self.basisorder = 1 if METHODS[self.method] == 0 else 0
+
+    @property
+    def srcnum(self) -> int:
+        """Patterns sharing each photon (field slots per value)."""
+        if self.srctype == "pattern" and self.srcpattern is not None and np.ndim(self.srcpattern) == 3:
+            return int(np.shape(self.srcpattern)[0])
+        return 1
 
     def gates(self):
         """maxgate and the adjusted tend, in float32 as mcx_validatecfg."""
@@ -75,7 +82,7 @@ class SimConfig:
         s.outputtype = OUTPUTS[self.outputtype]
         s.isreflect, s.isspecular, s.voidtime = int(self.isreflect), int(self.isspecular), int(self.voidtime)
         s.srctype = SOURCES[self.srctype]
-        s.srcnum = 1
+        s.srcnum = self.srcnum
         pos = list(self.srcpos) + [0.0] * (4 - len(self.srcpos))
         dirv = list(self.srcdir) + [0.0] * (4 - len(self.srcdir))
         if s.srctype == 0:
@@ -91,7 +98,10 @@ class SimConfig:
         s.srcparam2[:] = list(self.srcparam2)[:4]
         keep = []
         if self.srcpattern is not None:
-            pat = np.ascontiguousarray(self.srcpattern, np.float32).ravel()
+            pat = np.asarray(self.srcpattern, np.float32)
+            if pat.ndim == 3:             # (srcnum, ny, nx) -> pixel-major, pattern fastest
+                pat = np.transpose(pat, (1, 2, 0))
+            pat = np.ascontiguousarray(pat).ravel()
             keep.append(pat)
             s.srcpattern = ptr(pat, C.c_float)
         else:
@@ -147,13 +157,17 @@ class SimResult:
     kernel_ms: float
     total_ms: float
     normalizer: float = 1.0
+    srcweight: Optional[np.ndarray] = None    # (2, srcnum): launched, absorbed energy per pattern
 
 
 class _ResultBuf:
     def __init__(self, mesh: TetMesh, cfg: SimConfig, want_field=True, records=None):
         maxgate = cfg.gates()[0]
         datalen = mesh.nn if cfg.basisorder else mesh.ne
-        self.field = np.zeros((maxgate, datalen), np.float64) if want_field else None
+        S = cfg.srcnum
+        shape = (maxgate, datalen, S) if S > 1 else (maxgate, datalen)
+        self.field = np.zeros(shape, np.float64) if want_field else None
+        self.srcweight = np.zeros((2, S), np.float64) if S > 1 else None
         ndet = len(cfg.detpos) if (cfg.issavedet and cfg.detpos is not None) else 0
         self.reclen = (1 + 2 * mesh.prop + 6 * int(cfg.issaveexit) + 1) if ndet else 0
         nd = min(int(cfg.maxdetphoton), int(cfg.nphoton)) if ndet else 0
@@ -165,6 +179,7 @@ class _ResultBuf:
         r.field = ptr(self.field, C.c_double) if want_field else C.cast(None, f64p)
         r.detdata = ptr(self.detdata, C.c_float) if ndet else C.cast(None, f32p)
         r.detphoton = ptr(self.detphoton, C.c_int64) if ndet else C.cast(None, i64p)
+        r.srcweight = ptr(self.srcweight, C.c_double) if self.srcweight is not None else C.cast(None, f64p)
         if records is not None:
             first, count = records
             count = max(0, min(int(count), int(cfg.nphoton) - int(first)))
@@ -182,7 +197,7 @@ class _ResultBuf:
             detected=int(r.detected),
             detdata=self.detdata[:n] if self.detdata is not None else None,
             detphoton=self.detphoton[:n] if self.detphoton is not None else None,
-            photons=self.photons, kernel_ms=r.kernel_ms, total_ms=r.total_ms)
+            photons=self.photons, kernel_ms=r.kernel_ms, total_ms=r.total_ms, srcweight=self.srcweight)
 
 
 def tune_struct(block=0, blocks_per_cu=0, compensated=0, chunk=1, replicas=0) -> TuneC:
@@ -208,13 +223,18 @@ def run(mesh: TetMesh, cfg: SimConfig, records=None, normalize=False, want_field
     return res
 
 
-def normalize_field(mesh: TetMesh, cfg: SimConfig, res: SimResult) -> float:
+def normalize_field(mesh: TetMesh, cfg: SimConfig, res: SimResult):
+    """mesh_normalize in place; with shared patterns each pattern slot is
+    normalized with its own energies and the factors are returned as an array."""
     L = _lib.lib()
     sc = cfg.cstruct(mesh)
-    return L.mmc_host_normalize(C.byref(sc), mesh.nn, mesh.ne, ptr(mesh.elem, C.c_int32),
-                                ptr(mesh.etype, C.c_int32), ptr(mesh.med, C.c_float),
-                                ptr(mesh.evol, C.c_float), ptr(mesh.nvol, C.c_float),
-                                ptr(res.field, C.c_double), res.absorbweight, res.launchweight)
+    args = (C.byref(sc), mesh.nn, mesh.ne, ptr(mesh.elem, C.c_int32), ptr(mesh.etype, C.c_int32),
+            ptr(mesh.med, C.c_float), ptr(mesh.evol, C.c_float), ptr(mesh.nvol, C.c_float),
+            ptr(res.field, C.c_double))
+    if cfg.srcnum == 1:
+        return L.mmc_host_normalize(*args, res.absorbweight, res.launchweight)
+    return np.array([L.mmc_host_normalize_pair(*args, res.srcweight[1, k], res.srcweight[0, k], k)
+                     for k in range(cfg.srcnum)])
 
 
 class ResidentMesh:

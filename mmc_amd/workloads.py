@@ -12,7 +12,8 @@ shells      jittered 200x200x100 mm box, 1 mm cells x6 tets (24.0M tets; the
             stated ~22M tets implies ~1 mm cells, not 0.6 mm which would give
             111M), five tissues in nested ellipsoidal shells; a 60x60 mm planar
             source with a 64x64 Bernoulli(0.5) pattern, 100 gates of 0.05 ns,
-            Plucker nodal fluence, 32 detectors; run as 8 pattern requests
+            Plucker nodal fluence, 32 detectors; the 8 patterns either share
+            every photon (srcnum = 8) or run as 8 pattern requests
 """
 from __future__ import annotations
 
@@ -87,6 +88,23 @@ def shells_detectors() -> np.ndarray:
     ang = np.arange(32) * (2 * np.pi / 32)
     return np.stack([100 + 45 * np.cos(ang), 100 + 45 * np.sin(ang), np.zeros(32), np.full(32, 1.5)],
                     1).astype(np.float32)
+
+
+def shells_patterns(npat: int = 8, seed: int = 5) -> np.ndarray:
+    """The 8 Bernoulli(0.5) 64x64 patterns over the 60x60 mm window."""
+    rng = np.random.default_rng(seed)
+    return (rng.random((npat, 64, 64)) < 0.5).astype(np.float32)
+
+
+def shells_shared_sim(photons: int = 1_000_000_000, npat: int = 8, seed: int = 900) -> SimConfig:
+    """The large run with its 8 patterns batched onto every photon
+    (srcnum = 8, photon sharing): one trajectory serves all patterns and the
+    nodal field holds 8 slots per node and gate."""
+    return SimConfig(nphoton=photons, seed=seed, method="P", srctype="pattern",
+                     srcpos=(70.0, 70.0, 0.0), srcdir=(0.0, 0.0, 1.0),
+                     srcparam1=(60.0, 0.0, 0.0, 64.0), srcparam2=(0.0, 60.0, 0.0, 64.0),
+                     srcpattern=shells_patterns(npat), tstart=0.0, tend=5e-9, tstep=5e-11, outputtype="x",
+                     issavedet=1, detpos=shells_detectors(), maxdetphoton=1_000_000)
 
 
 def shells_sims(nreq: int = 8, photons: int = 1_000_000_000, seed: int = 5):

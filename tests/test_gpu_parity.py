@@ -162,6 +162,39 @@ def test_area_sources_per_photon(srctype):
     assert abs(g.launchweight - o.launchweight) < 1e-3
 
 
+@pytest.mark.parametrize("method", ["S", "P"])
+def test_shared_patterns_equal_one_run_per_pattern(method):
+    # photon sharing (srcnum > 1, src/mmc_raytrace.c:1586-1612, 1812-1835):
+    # slot k of a run whose photons carry all patterns equals the run with
+    # pattern k alone.  With 0/1 patterns and several gates (no roulette) the
+    # single-pattern photons have weight p_k and the same trajectories, so
+    # the deposits agree value by value up to the FP32 atomic order.
+    node, elem, et, med = M.meshgen.cube60(4.0)
+    nodal = method == "P"
+    base = M.TetMesh(node, elem, et, med).prepare(nodal=nodal)
+    c = M.meshgen.centroids(base.node, base.elem)
+    srcelem = (np.where(c[:, 2] < 4.0)[0] + 1).astype(np.int32)
+    tm = M.TetMesh(node, elem, et, med, srcelem=srcelem).prepare(nodal=nodal)
+    pats = (np.random.default_rng(5).random((3, 8, 8)) < 0.5).astype(np.float32)
+    kw = dict(nphoton=20000, method=method, srctype="pattern", srcpos=(20.0, 20.0, 0.0), srcdir=(0, 0, 1),
+              srcparam1=(20.0, 0, 0, 8), srcparam2=(0, 20.0, 0, 8), tend=5e-9, tstep=5e-10, seed=31,
+              e0=int(srcelem[0]))
+    tune = M.sim.tune_struct(compensated=1)
+    shared = M.run(tm, M.SimConfig(srcpattern=pats, **kw), tune=tune)
+    assert shared.field.shape[-1] == 3 and shared.launchweight == kw["nphoton"]
+    for k in range(3):
+        one = M.run(tm, M.SimConfig(srcpattern=pats[k], **kw), tune=tune)
+        f = shared.field[..., k]
+        assert np.linalg.norm(f - one.field) <= 1e-6 * np.linalg.norm(one.field)
+        assert shared.srcweight[0, k] == one.launchweight
+        assert abs(shared.srcweight[1, k] - one.absorbweight) <= 1e-6 * one.launchweight
+    # per-pattern normalization uses each pattern's energies
+    k_shared = M.sim.normalize_field(tm, M.SimConfig(srcpattern=pats, **kw), shared)
+    one = M.run(tm, M.SimConfig(srcpattern=pats[2], **kw), tune=tune, normalize=True)
+    assert abs(k_shared[2] / one.normalizer - 1) < 1e-5
+    assert np.linalg.norm(shared.field[..., 2] - one.field) <= 1e-5 * np.linalg.norm(one.field)
+
+
 def test_batch_result_equals_result_alone(slab):
     _, tm, _ = slab
     sims = W.slab_batch_sims(3, seed=1, lo=2000, hi=6000)
