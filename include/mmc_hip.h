@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 4
+#define MMC_HIP_ABI_VERSION 5
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -36,6 +36,12 @@ extern "C" {
 #define MMC_OUT_FLUX     0
 #define MMC_OUT_FLUENCE  1
 #define MMC_OUT_ENERGY   2
+/* replay outputs (enum TOutputType, src/mmc_utils.h:106): photons relaunched
+ * from saved seeds deposit the detected-weight-scaled Jacobian for mua (J),
+ * weighted path length (L) or weighted scattering count (P) */
+#define MMC_OUT_JACOBIAN 3
+#define MMC_OUT_WL       4
+#define MMC_OUT_WP       5
 /* boundary conditions (enum TBoundary, src/mmc_utils.h:116) */
 #define MMC_BC_NOREFLECT 0
 #define MMC_BC_REFLECT   1
@@ -89,6 +95,14 @@ typedef struct mmc_hip_sim {
     const float* detpos;     /* detnum*4 {x,y,z,r} */
     int64_t maxdetphoton;
     const float* med;        /* optional (prop+1)*4 media replacing mesh->med */
+    /* replay (seed = SEED_FROM_FILE, src/mmc_mesh.c:815-898): photon i of the
+     * simulation starts from photonseed[2i], photonseed[2i+1] (xorshift128+
+     * state; photon_offset selects the first) instead of the seeded hash;
+     * replayweight/replaytime give each photon's detected weight and time of
+     * flight for outputs J/L/P.  NULL otherwise. */
+    const uint64_t* photonseed;
+    const float* replayweight;
+    const float* replaytime;
 } mmc_hip_sim;
 
 /* Per-photon end state, written for photon ids in [recfirst, recfirst+reccount). */

@@ -45,6 +45,7 @@ class SimC(C.Structure):
         ("issavedet", C.c_int32), ("issaveexit", C.c_int32),
         ("detnum", C.c_int32), ("detpos", f32p), ("maxdetphoton", C.c_int64),
         ("med", f32p),
+        ("photonseed", C.POINTER(C.c_uint64)), ("replayweight", f32p), ("replaytime", f32p),
     ]
 
 
@@ -127,7 +128,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 4        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 5        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

@@ -12,6 +12,7 @@
  * <session>.mch (history header + detected-photon records,
  * src/mmc_utils.c:4309-4342).
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -45,6 +46,11 @@ typedef struct {
     float detpos[4 * MAXDET];
     float* pattern;
     int gpuid;
+    /* replay (-E file.mch) and seed saving (-q 1) */
+    char seedfile[1024];
+    int issaveseed, replaydet;
+    uint64_t* rseeds;
+    float *rweight, *rtime;
 } cli_cfg;
 
 static void die(const char* fmt, const char* arg) {
@@ -269,17 +275,68 @@ static void load_mesh_files(cli_cfg* c) {
     }
 }
 
+/* mesh_loadseedfile (src/mmc_mesh.c:815-898): the saved photons of detector
+ * replaydet (0: all) with their seeds; for the J/L/P outputs (and any
+ * replaydet) the replay weight w0 * prod exp(-mua_j * col_j * unitinmm) and
+ * time sum n_j * col_j * R_C0 over record columns j = 2..maxmedia+1 and media
+ * 1..maxmedia, in float as the reference computes them. */
+static void load_seedfile(cli_cfg* c, int prop) {
+    FILE* fp = fopen(c->seedfile, "rb");
+    if (!fp) die("can not open the history file %s", c->seedfile);
+    mmc_mch_header his;
+    if (fread(&his, sizeof(his), 1, fp) != 1 || memcmp(his.magic, "MCXH", 4))
+        die("%s is not a detected-photon history file", c->seedfile);
+    if (his.savedphoton == 0 || his.seedbyte == 0) die("%s holds no photon seeds", c->seedfile);
+    if (his.seedbyte != 16) die("%s", "only 16-byte xorshift128+ seeds can be replayed");
+    if ((int)his.maxmedia != prop) die("%s", "the history file was generated with a different media setting");
+    size_t n = his.savedphoton, col = his.colcount;
+    float* ppath = (float*)malloc(sizeof(float) * n * col);
+    uint64_t* seeds = (uint64_t*)malloc(sizeof(uint64_t) * 2 * n);
+    if (fread(ppath, sizeof(float) * col, n, fp) != n || fread(seeds, 16, n, fp) != n)
+        die("error when reading %s", c->seedfile);
+    fclose(fp);
+    c->rseeds = (uint64_t*)malloc(sizeof(uint64_t) * 2 * n);
+    c->rweight = (float*)malloc(sizeof(float) * n);
+    c->rtime = (float*)malloc(sizeof(float) * n);
+    const float R_C0 = 3.335640951981520e-12f; /* src/mmc_const.h:45 */
+    size_t m = 0;
+    for (size_t i = 0; i < n; i++) {
+        const float* r = ppath + i * col;
+        if (c->replaydet != 0 && c->replaydet != (int)r[0]) continue;
+        c->rseeds[2 * m] = seeds[2 * i];
+        c->rseeds[2 * m + 1] = seeds[2 * i + 1];
+        float w = r[col - 1], t = 0.f;
+        for (int j = 2; j < prop + 2; j++) w *= expf(-c->med[4 * (j - 1)] * r[j] * his.unitinmm);
+        for (int j = 2; j < prop + 2; j++) t += c->med[4 * (j - 1) + 3] * r[j] * R_C0;
+        c->rweight[m] = w;
+        c->rtime[m] = t;
+        m++;
+    }
+    free(ppath);
+    free(seeds);
+    if (m == 0) die("%s", "no saved photon of the replayed detector");
+    c->sim.nphoton = (int64_t)m;
+    c->sim.photon_offset = 0;
+    c->sim.photonseed = c->rseeds;
+    c->sim.replayweight = c->rweight;
+    c->sim.replaytime = c->rtime;
+    if (c->sim.outputtype >= MMC_OUT_JACOBIAN || c->replaydet > 0) c->sim.minenergy = 0.f;
+}
+
 static void usage(void) {
     printf(
         "mmc (MI355X HIP backend) -- mesh Monte Carlo photon transport\n"
         "usage: mmc -f input.json [options]\n"
         "  -f/--input FILE       JSON input (Session/Forward/Domain/Mesh/Optode)\n"
         "  -n/--photon N         photons to simulate\n"
-        "  -E/--seed N           RNG seed (negative: time-based)\n"
+        "  -E/--seed N|file.mch  RNG seed (negative: time-based), or replay the saved photons of an .mch\n"
+        "  -q/--saveseed [0|1]   save the seeds of detected photons in the .mch (for replay)\n"
+        "  -P/--replaydet N      replay only the photons of detector N (0: all)\n"
         "  -s/--session NAME     output file prefix\n"
         "  -M/--method [P|S]     Plucker (nodal, -C 1) or branch-less Badouel (element, -C 0)\n"
         "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
-        "  -O/--outputtype [X|F|E] flux, fluence or energy deposit\n"
+        "  -O/--outputtype [X|F|E|J|L|P] flux, fluence, energy deposit; replay: Jacobian,\n"
+        "                        weighted path length, weighted scattering count\n"
         "  -F/--outputformat [ascii|bin|nii|hdr|mc2|tx3|jnii|bnii]\n"
         "  -Z/--zip [zlib|base64]   jnii/bnii payload: zlib-compressed (default) or raw\n"
         "  -b/--reflect [0|1|2|3] boundary: no reflect, Fresnel, absorb exterior, mirror\n"
@@ -326,13 +383,20 @@ int main(int argc, char** argv) {
         const char* a = argv[i];
         if (!strcmp(a, "-f") || !strcmp(a, "--input")) i++;
         else if (!strcmp(a, "-n") || !strcmp(a, "--photon")) c.sim.nphoton = (int64_t)atof(opt_value(argc, argv, &i));
-        else if (!strcmp(a, "-E") || !strcmp(a, "--seed")) c.seed = atol(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-E") || !strcmp(a, "--seed")) {
+            const char* v = opt_value(argc, argv, &i);
+            size_t n = strlen(v);
+            if (n > 4 && !strcmp(v + n - 4, ".mch")) snprintf(c.seedfile, sizeof(c.seedfile), "%s", v);
+            else c.seed = atol(v);
+        } else if (!strcmp(a, "-q") || !strcmp(a, "--saveseed")) c.issaveseed = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-P") || !strcmp(a, "--replaydet")) c.replaydet = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-s") || !strcmp(a, "--session")) snprintf(c.session, sizeof(c.session), "%s", opt_value(argc, argv, &i));
         else if (!strcmp(a, "-M") || !strcmp(a, "--method")) c.method = opt_value(argc, argv, &i)[0] | 0x20;
         else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
             char t = opt_value(argc, argv, &i)[0] | 0x20;
-            c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
+            c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : t == 'j' ? MMC_OUT_JACOBIAN
+                             : t == 'l' ? MMC_OUT_WL : t == 'p' ? MMC_OUT_WP : MMC_OUT_FLUX;
         } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) {
             const char* f = opt_value(argc, argv, &i);
             if ((c.outputformat = mmc_host_format_id(f)) < 0) die("unsupported output format '%s'", f);
@@ -361,7 +425,7 @@ int main(int argc, char** argv) {
         } else die("unknown or unsupported option %s", a);
     }
     if (c.gpuid < 0) die("%s", "the CPU path (-G -1) is not part of this build");
-    if (c.sim.nphoton <= 0) die("%s", "cfg.nphoton must be a positive number");
+    if (c.sim.nphoton <= 0 && !c.seedfile[0]) die("%s", "cfg.nphoton must be a positive number");
     if (c.sim.tstart > c.sim.tend || c.sim.tstep == 0.f) die("%s", "incorrect time gate settings or missing tstart/tend/tstep fields");
     /* mcx_validatecfg, src/mmc_utils.c:3500-3530 */
     if (c.sim.tstep > c.sim.tend - c.sim.tstart) c.sim.tstep = c.sim.tend - c.sim.tstart;
@@ -387,6 +451,8 @@ int main(int argc, char** argv) {
             c.med[4 * i + 1] *= c.unitinmm;
         }
     int prop = c.nmed - 1;
+    if (c.seedfile[0]) load_seedfile(&c, prop);
+    else if (c.sim.outputtype >= MMC_OUT_JACOBIAN) die("%s", "the J/L/P outputs replay photons: give -E file.mch");
 
     /* mmc_prep: volumes, neighbours, surface correction, exterior faces, planes */
     int32_t* elem = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)c.ne);
@@ -422,6 +488,7 @@ int main(int argc, char** argv) {
     int reclen = 2 * prop + 6 * (c.sim.issaveexit > 0) + 2;
     if (c.sim.issavedet) {
         res.detdata = (float*)calloc((size_t)c.sim.maxdetphoton * reclen, sizeof(float));
+        res.detphoton = (int64_t*)calloc((size_t)c.sim.maxdetphoton, sizeof(int64_t));
         res.reclen = reclen;
     }
     char dev[256] = "";
@@ -462,12 +529,35 @@ int main(int argc, char** argv) {
         his.srcnum = 1;
         his.respin = 1;
         his.savedetflag = c.sim.issaveexit ? 0x77 : 0x47;
+        his.seedbyte = c.issaveseed ? 16 : 0;
         snprintf(fname, sizeof(fname), "%s.mch", c.session);
         if (mmc_host_save_mch(fname, &his, res.detdata, saved)) die("can not write %s", fname);
+        if (c.issaveseed) {
+            /* the initial xorshift128+ state of each saved photon: the saved
+             * state of a replayed photon, else the seeded hash of its id */
+            FILE* fp = fopen(fname, "ab");
+            if (!fp) die("can not append seeds to %s", fname);
+            for (int64_t k = 0; k < saved; k++) {
+                uint64_t st[2];
+                int64_t id = res.detphoton[k];
+                if (c.rseeds) {
+                    st[0] = c.rseeds[2 * id];
+                    st[1] = c.rseeds[2 * id + 1];
+                } else {
+                    mmc_hip_photon_seeds(c.sim.seed, id, 1, st);
+                }
+                if (fwrite(st, sizeof(uint64_t), 2, fp) != 2) die("can not append seeds to %s", fname);
+            }
+            fclose(fp);
+        }
         printf("detected %lld photons\n", (long long)res.detected);
     }
     free(field);
     free(res.detdata);
+    free(res.detphoton);
+    free(c.rseeds);
+    free(c.rweight);
+    free(c.rtime);
     free(elem);
     free(type);
     free(evol);

@@ -246,6 +246,14 @@ double mmc_host_normalize_pair(const mmc_hip_sim* sim, int32_t nn, int32_t ne, c
     /* value j of gate g of this pattern slot: field[(g*datalen + j)*S + pair] */
 #define F(g, j) field[((size_t)(g) * datalen + (j)) * S + pair]
     double deposit = 0.0, k;
+    if (sim->outputtype == MMC_OUT_JACOBIAN || sim->outputtype == MMC_OUT_WL || sim->outputtype == MMC_OUT_WP) {
+        /* replay outputs: per replayed photon for the Jacobian, per unit of
+         * detected weight for the weighted path and scattering count */
+        k = (sim->outputtype == MMC_OUT_JACOBIAN) ? 1.f / (1e-4f * (float)sim->nphoton) : 1.f / Etotal;
+        for (int32_t g = 0; g < sim->maxgate; g++)
+            for (int32_t j = 0; j < datalen; j++) F(g, j) *= k;
+        return k;
+    }
     if (sim->outputtype == MMC_OUT_ENERGY) {
         k = 1.f / Etotal;
         for (int32_t g = 0; g < sim->maxgate; g++)

@@ -22,6 +22,11 @@ constexpr int kMaxSrc = 8;           /* patterns shared by one photon (srcnum) *
  * 3 voidtime, bits 4-5 boundary condition (MMC_BC_*), bits 6-7 isspecular. */
 enum { kFlagRoulette = 1, kFlagEnergy = 2, kFlagSaveDet = 4, kFlagVoidTime = 8 };
 constexpr int kReflShift = 4, kSpecShift = 6;
+/* bits 8-9: replay output (0 none, 1 Jacobian, 2 weighted path, 3 weighted
+ * scattering count); kFlagEnergy marks outputs deposited as raw weight
+ * (energy, weighted path, weighted scattering count) rather than fluence */
+constexpr int kReplayShift = 8;
+enum { kReplayNone = 0, kReplayJ = 1, kReplayWL = 2, kReplayWP = 3 };
 struct SimHot {
     float tstart, tend, rtstep, minenergy;
     float roulettesize;
@@ -40,6 +45,7 @@ struct SimCold {
     int32_t patbase, patnx, patny, srcnum;
     int32_t isreflect, isspecular, voidtime, outputtype;
     int32_t issavedet, issaveexit, pad0, pad1;
+    int64_t replaybase;              /* first of this simulation's photons in KArgs::seeds, or -1 */
     int64_t nphoton, photon_first;
     uint64_t seed;
     uint64_t fieldoff;               /* float offset of this simulation's field */
@@ -95,6 +101,11 @@ struct KArgs {
     int32_t chunk;
     uint32_t maxsteps;               /* per-photon bound on ray-tetrahedron tests */
     int32_t maxsrc;                  /* largest srcnum of the launch (per-lane pattern slots) */
+    /* replayed photons: initial xorshift128+ states, detected weights and
+     * times, per photon of the launch (SimCold::replaybase) */
+    const uint64_t* seeds;
+    const float* rweight;
+    const float* rtime;
 };
 
 /* Per-photon xorshift128+ state: splitmix64 finalizer over (seed, photon id).

@@ -35,6 +35,7 @@ namespace mmc {
 #define ID_UNDEF   0x7FFFFFFF                 /* src/mmc_mesh.h:66 */
 #define MAX_TRIAL  3                          /* src/mmc_raytrace.h:44 */
 #define FIX_PHOTON 1e-3f                      /* src/mmc_raytrace.h:45 */
+#define DELTA_MUA  1e-4f                      /* src/mmc_mesh.h:73 */
 
 enum { CTX_OUTER = 0, CTX_INNER = 1, CTX_INNERFIX = 2 };
 
@@ -108,6 +109,22 @@ __device__ __forceinline__ void deposit(float* f, float* lo, uint64_t idx, float
         float err = (old - (s - bp)) + (v - bp);
         if (err != 0.f) atomicAdd(lo + idx, err);
     }
+}
+
+/* Weight before the step as the deposit sees it: the photon's own weight
+ * before attenuation, or in replay the detected weight rw times the step's
+ * Jacobian factor exp(-dmua L), path length L or scattering fraction
+ * L mus / slen0, added to the attenuated weight w
+ * (src/mmc_raytrace.c:1523-1543). */
+__device__ __forceinline__ float replay_weight(int rmode, float currweight, float w, float L, float mus,
+                                               float slen0, float rw) {
+    if (rmode == kReplayNone) return currweight;
+    float c;
+    if (rmode == kReplayJ) c = mmc_expf(-DELTA_MUA * L);
+    else if (rmode == kReplayWL) c = L;
+    else c = (slen0 < EPSF) ? 1.f : L * mus / slen0;
+    c *= rw;
+    return c + w;
 }
 
 /* Physical XCD (accelerator complex die) executing this wave, 0..7 on MI355X. */
@@ -191,6 +208,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     uint32_t nloc = 0;                  /* gate * nn */
     bool npend = false;
     int nsrc = 1;                       /* patterns sharing the photon (field slots) */
+    float rw = 1.f, rt = 0.f;           /* replay: detected weight and time of the photon */
+    float slen0 = 0.f;                  /* scattering length at the last scattering event */
     int posidx = 0;                     /* launch pixel of a pattern source */
 
     /* deposit v at value loc of the simulation's field (offset foff); with
@@ -274,7 +293,19 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     alive = true;
                     const SimCold& C = a.cold[sim];
                     uint64_t pid = (uint64_t)(C.photon_first + (g - s_cum[sim]));
-                    photon_seed(C.seed, pid, r0, r1);
+                    if (C.replaybase >= 0) {
+                        /* replay: the saved initial state of this photon */
+                        const int64_t k = C.replaybase + (g - s_cum[sim]);
+                        r0 = a.seeds[2 * k];
+                        r1 = a.seeds[2 * k + 1];
+                        rw = a.rweight ? a.rweight[k] : 1.f;
+                        rt = a.rtime ? a.rtime[k] : 0.f;
+                    } else {
+                        photon_seed(C.seed, pid, r0, r1);
+                        rw = 1.f;
+                        rt = 0.f;
+                    }
+                    slen0 = 0.f;
                     nsrc = C.srcnum;
                     posidx = 0;
                     /* launchphoton (src/mmc_raytrace.c:2346-2671) */
@@ -374,7 +405,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     *LS(slot_ns) = 0.f;
                     if (a.savedet)
                         for (int k = 0; k < 2 * a.prop; k++) *LS(ppbase + k) = 0.f;
-                    kahan_lds(LS(0), LS(1), w);
+                    /* replay with weighted path / scattering outputs counts the
+                     * detected weight as launched energy (src/mmc_raytrace.c:1815) */
+                    {
+                        const int rmode = (a.hot[sim].flags >> kReplayShift) & 3;
+                        kahan_lds(LS(0), LS(1), (rmode == kReplayWL || rmode == kReplayWP) ? rw : w);
+                    }
                     if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */
                 }
             }
@@ -383,6 +419,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         if (!alive) continue;
 
         const SimHot hot = STAGE ? s_hot[sim] : a.hot[sim];
+        const int rmode = (hot.flags >> kReplayShift) & 3;
         const size_t mbase = (size_t)sim * nmed;
         auto med = [&](int k) -> float4 { return STAGE ? s_med[mbase + k] : a.media[mbase + k]; };
         int reason = 0;
@@ -455,9 +492,11 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     w *= mmc_expf(-mua * Lmove);
                     slen -= Lmove * mus;
-                    float ww = currweight - w;
+                    const float cw = replay_weight(rmode, currweight, w, Lmove, mus, slen0, rw);
+                    float ww = cw - w;
                     tof += Lmove * rc;
-                    int g = (int)((tof - hot.tstart) * hot.rtstep);
+                    int g = (rmode == kReplayWL || rmode == kReplayWP) ? (int)(rt * hot.rtstep)
+                                                                       : (int)((tof - hot.tstart) * hot.rtstep);
                     int tshift = max(min(g, hot.maxgate - 1), 0) * a.ne;
                     if (mua > 0.f) {
                         float y = ww - eabs_c, tt = eabs + y;
@@ -465,7 +504,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         eabs = tt;
                     }
                     if (!(hot.flags & kFlagEnergy))
-                        ww = (mua < EPSF) ? (currweight * Lmove) : ((currweight - w) / mua);
+                        ww = (mua < EPSF) ? (cw * Lmove) : ((cw - w) / mua);
                     px = px + vx * Lmove;
                     py = py + vy * Lmove;
                     pz = pz + vz * Lmove;
@@ -569,7 +608,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     pz += Lmove * vz;
                     if (Lp0 > EPSF) {
                         tof += Lmove * rc;
-                        float ww = currweight - w;
+                        const float cw = replay_weight(rmode, currweight, w, Lmove, mus, slen0, rw);
+                        float ww = cw - w;
                         if (mua > 0.f || !(hot.flags & kFlagEnergy)) {
                             float ratio = Lmove / Lp0;
                             if (mua > 0.f) {
@@ -578,8 +618,10 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                 eabs = tt;
                             }
                             if (!(hot.flags & kFlagEnergy))
-                                ww = (mua < EPSF) ? (currweight * Lmove) : ((currweight - w) / mua);
-                            int g = (int)((tof - hot.tstart) * hot.rtstep);
+                                ww = (mua < EPSF) ? (cw * Lmove) : ((cw - w) / mua);
+                            int g = (rmode == kReplayWL || rmode == kReplayWP)
+                                        ? (int)(rt * hot.rtstep)
+                                        : (int)((tof - hot.tstart) * hot.rtstep);
                             const uint32_t loc = (uint32_t)max(min(g, hot.maxgate - 1), 0) * (uint32_t)a.nn;
                             ww *= 0.5f;
                             if (isend) {
@@ -782,6 +824,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     vx = qx; vy = qy; vz = qz;
                     slen = nextslen;
+                    slen0 = nextslen;
                     if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
                     *LS(slot_ns) += 1.f;
                     ctx = CTX_OUTER;

@@ -315,8 +315,14 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
             return set_err(-1, "srcnum %d outside 1..%d patterns", s->srcnum, kMaxSrc);
     }
     if (s->issavedet && s->detnum > 0 && !s->detpos) return set_err(-1, "detectors without positions");
-    if (s->outputtype != MMC_OUT_FLUX && s->outputtype != MMC_OUT_FLUENCE && s->outputtype != MMC_OUT_ENERGY)
+    if (s->outputtype < MMC_OUT_FLUX || s->outputtype > MMC_OUT_WP)
         return set_err(-1, "unsupported output type %d", s->outputtype);
+    if (s->outputtype >= MMC_OUT_JACOBIAN) {
+        if (!s->photonseed || !s->replayweight)
+            return set_err(-1, "replay outputs need the photons' seeds and detected weights");
+        if (s->outputtype != MMC_OUT_JACOBIAN && !s->replaytime)
+            return set_err(-1, "weighted path/scattering outputs need the detected times");
+    }
     if (s->isreflect < MMC_BC_NOREFLECT || s->isreflect > MMC_BC_MIRROR)
         return set_err(-1, "unsupported boundary condition %d", s->isreflect);
     if (s->isspecular < 0 || s->isspecular > 2) return set_err(-1, "isspecular must be 0, 1 or 2");
@@ -383,6 +389,7 @@ struct mmc_hip_slotres {
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     Pooled d_hot, d_cold, d_med, d_cum, d_det, d_pat, d_acc, d_detcnt, d_queue, d_detdata, d_detph, d_rec;
+    Pooled d_seeds, d_rw, d_rt;
     /* launch in flight */
     bool busy = false;
     mmc_hip_mesh_handle m = nullptr;
@@ -401,6 +408,8 @@ struct mmc_hip_slotres {
     std::vector<SimCold> h_cold;
     std::vector<float4> h_media, h_dets;
     std::vector<float> h_pats;
+    std::vector<uint64_t> h_seeds;
+    std::vector<float> h_rw, h_rt;
     std::chrono::steady_clock::time_point t_start;
     ~mmc_hip_slotres() {
         if (st) {
@@ -454,6 +463,12 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     std::vector<int64_t>& cum = sl->cum;
     std::vector<float4>& dets = sl->h_dets;
     std::vector<float>& pats = sl->h_pats;
+    std::vector<uint64_t>& seeds = sl->h_seeds;
+    std::vector<float>& rws = sl->h_rw;
+    std::vector<float>& rts = sl->h_rt;
+    seeds.clear();
+    rws.clear();
+    rts.clear();
     hot.assign(nsim, SimHot());
     cold.assign(nsim, SimCold());
     media.assign((size_t)nsim * nmed, float4());
@@ -478,8 +493,12 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         h.maxgate = s.maxgate;
         /* roulette only for single-gate runs (src/mmc_raytrace.c:2033) */
         float span = (s.tend - s.tstart) * h.rtstep;
-        h.flags = ((s.minenergy > 0.f && span <= 1.f) ? kFlagRoulette : 0) |
-                  (s.outputtype == MMC_OUT_ENERGY ? kFlagEnergy : 0) |
+        const int rmode = s.outputtype == MMC_OUT_JACOBIAN ? kReplayJ
+                          : s.outputtype == MMC_OUT_WL ? kReplayWL
+                          : s.outputtype == MMC_OUT_WP ? kReplayWP : kReplayNone;
+        h.flags = ((s.minenergy > 0.f && span <= 1.f && rmode == kReplayNone) ? kFlagRoulette : 0) |
+                  ((s.outputtype == MMC_OUT_ENERGY || rmode == kReplayWL || rmode == kReplayWP) ? kFlagEnergy : 0) |
+                  (rmode << kReplayShift) |
                   ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0) |
                   (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
                   ((s.isspecular & 3) << kSpecShift);
@@ -530,6 +549,16 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         c.photon_first = s.photon_offset;
         c.seed = s.seed;
         c.fieldoff = fieldoff[i];
+        c.replaybase = -1;
+        if (s.photonseed) {
+            c.replaybase = (int64_t)(seeds.size() / 2);
+            const uint64_t* sp = s.photonseed + 2 * (size_t)s.photon_offset;
+            seeds.insert(seeds.end(), sp, sp + 2 * (size_t)s.nphoton);
+            for (int64_t k = 0; k < s.nphoton; k++) {
+                rws.push_back(s.replayweight ? s.replayweight[s.photon_offset + k] : 1.f);
+                rts.push_back(s.replaytime ? s.replaytime[s.photon_offset + k] : 0.f);
+            }
+        }
         const float* md = s.med ? s.med : m->med.data();
         for (int k = 0; k < nmed; k++)
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
@@ -561,6 +590,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 1))) ||
         (rc = sl->d_queue.ensure(sizeof(unsigned long long))))
         return rc;
+    if (!seeds.empty() &&
+        ((rc = sl->d_seeds.ensure(sizeof(uint64_t) * seeds.size())) || (rc = sl->d_rw.ensure(sizeof(float) * rws.size())) ||
+         (rc = sl->d_rt.ensure(sizeof(float) * rts.size()))))
+        return rc;
     if (!arena) {
         if ((rc = dalloc((float**)&sl->field.p, (size_t)fieldlen * nrep, nullptr))) return rc;
         if (comp && (rc = dalloc((float**)&sl->lo.p, (size_t)fieldlen * nrep, nullptr))) return rc;
@@ -589,6 +622,11 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         HIPCHK(hipMemcpyAsync(sl->d_det.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
     if (!pats.empty())
         HIPCHK(hipMemcpyAsync(sl->d_pat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
+    if (!seeds.empty()) {
+        HIPCHK(hipMemcpyAsync(sl->d_seeds.p, seeds.data(), sizeof(uint64_t) * seeds.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sl->d_rw.p, rws.data(), sizeof(float) * rws.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sl->d_rt.p, rts.data(), sizeof(float) * rts.size(), hipMemcpyHostToDevice, st));
+    }
     if (!arena) {
         HIPCHK(hipMemsetAsync(sl->field.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
         if (sl->lo.p) HIPCHK(hipMemsetAsync(sl->lo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
@@ -642,6 +680,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.chunk = (tune && tune->chunk > 0) ? tune->chunk : 1;
     a.maxsteps = MMC_MAX_STEPS;
     a.maxsrc = maxsrc;
+    a.seeds = seeds.empty() ? nullptr : (const uint64_t*)sl->d_seeds.p;
+    a.rweight = seeds.empty() ? nullptr : (const float*)sl->d_rw.p;
+    a.rtime = seeds.empty() ? nullptr : (const float*)sl->d_rt.p;
 
     const int nslots = kLaneSlots + 2 * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that

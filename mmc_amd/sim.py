@@ -19,7 +19,8 @@ from ._lib import PHOTON_DTYPE, ResultC, SimC, TuneC, f32p, f64p, i64p, ptr
 from .mesh import TetMesh
 
 METHODS = {"P": 0, "S": 3, "plucker": 0, "blbadouel": 3}
-OUTPUTS = {"x": 0, "f": 1, "e": 2, "flux": 0, "fluence": 1, "energy": 2}
+OUTPUTS = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5, "flux": 0, "fluence": 1, "energy": 2,
+           "jacobian": 3, "wl": 4, "wp": 5}
 SOURCES = {"pencil": 0, "planar": 4, "pattern": 5}
 
 
@@ -53,6 +54,11 @@ class SimConfig:
     maxdetphoton: int = 1000000
     e0: int = 0
     med: Optional[np.ndarray] = None       # per-simulation media override
+    # replay: initial xorshift128+ state of each photon (n, 2) and, for the
+    # J/L/P outputs, its detected weight and time of flight (n,)
+    photonseed: Optional[np.ndarray] = None
+    replayweight: Optional[np.ndarray] = None
+    replaytime: Optional[np.ndarray] = None
 
     def __post_init__(self):
         if self.basisorder is None:
@@ -125,6 +131,22 @@ class SimConfig:
             s.med = ptr(md, C.c_float)
         else:
             s.med = C.cast(None, f32p)
+        if self.photonseed is not None:
+            sd = np.ascontiguousarray(self.photonseed, np.uint64).reshape(-1, 2)
+            if len(sd) < int(self.photon_offset) + int(self.nphoton):
+                raise ValueError("fewer saved seeds than replayed photons")
+            keep.append(sd)
+            s.photonseed = sd.ctypes.data_as(C.POINTER(C.c_uint64))
+        else:
+            s.photonseed = C.cast(None, C.POINTER(C.c_uint64))
+        for name in ("replayweight", "replaytime"):
+            v = getattr(self, name)
+            if v is not None:
+                a = np.ascontiguousarray(v, np.float32).ravel()
+                keep.append(a)
+                setattr(s, name, ptr(a, C.c_float))
+            else:
+                setattr(s, name, C.cast(None, f32p))
         s._keep = keep
         return s
 

@@ -365,7 +365,41 @@ typedef struct {
     float rtstep;
     int framelen;
     double raytet;
+    float rw, rt;              /* replay: the photon's detected weight and time */
 } octx;
+
+#define DELTA_MUA 1e-4f        /* src/mmc_mesh.h:73 */
+
+/* mmc_is_fluence_like, src/mmc_raytrace.c:57-59 */
+static inline int fluence_like(int ot) { return ot != ORC_OT_ENERGY && ot != ORC_OT_WP && ot != ORC_OT_WL; }
+
+/* replay weights of src/mmc_raytrace.c:1523-1543: with the J/L/P outputs the
+ * deposit sees the detected weight times exp(-DELTA_MUA*L), L or
+ * L*mus/slen0, added to the attenuated weight */
+static inline float replay_cw(const octx* c, float currweight, const oray* r, float mus) {
+    int ot = c->cfg->outputtype;
+    float cw;
+    if (ot == ORC_OT_JACOBIAN) {
+        cw = O_EXPF(-DELTA_MUA * r->Lmove);
+    } else if (ot == ORC_OT_WL) {
+        cw = r->Lmove;
+    } else if (ot == ORC_OT_WP) {
+        if (r->slen0 < EPS) cw = 1;
+        else cw = r->Lmove * mus / r->slen0;
+    } else {
+        return currweight;
+    }
+    cw *= c->rw;
+    cw += r->weight;
+    return cw;
+}
+
+/* time gate: the detected time for the weighted path/scattering outputs */
+static inline int replay_gate(const octx* c, float photontimer) {
+    int ot = c->cfg->outputtype;
+    if (ot == ORC_OT_WL || ot == ORC_OT_WP) return (int)(c->rt * c->rtstep);
+    return (int)((photontimer - c->cfg->tstart) * c->rtstep);
+}
 
 static inline const float* med_of(const orc_mesh* m, int t) { return m->med + 4 * (size_t)t; }
 
@@ -437,15 +471,16 @@ static float orc_badouel(oray* r, octx* c) {
         float totalloss = O_EXPF(-mua * r->Lmove);
         r->weight *= totalloss;
         r->slen -= r->Lmove * mus;
+        currweight = replay_cw(c, currweight, r, mus);
 
         {
             int framelen = c->framelen;
             float ww = currweight - r->weight;
             r->photontimer += r->Lmove * rc;
-            int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+            int g = replay_gate(c, r->photontimer);
             int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * framelen;
             if (mua > 0.f) r->Eabsorb += ww;
-            if (cfg->outputtype != ORC_OT_ENERGY)
+            if (fluence_like(cfg->outputtype))
                 ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
 
             r->p0.x = r->p0.x + r->vec.x * r->Lmove;
@@ -547,6 +582,7 @@ static float orc_plucker(oray* r, octx* c) {
         }
         r->weight *= O_EXPF(-mua * r->Lmove);
         r->slen -= r->Lmove * mus;
+        currweight = replay_cw(c, currweight, r, mus);
         r->p0.x += r->Lmove * r->vec.x;
         r->p0.y += r->Lmove * r->vec.y;
         r->p0.z += r->Lmove * r->vec.z;
@@ -554,21 +590,21 @@ static float orc_plucker(oray* r, octx* c) {
         if (!cfg->basisorder) {
             float ww = currweight - r->weight;
             r->Eabsorb += ww;
-            if (cfg->outputtype != ORC_OT_ENERGY)
+            if (fluence_like(cfg->outputtype))
                 ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
             r->photontimer += r->Lmove * rc;
-            int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+            int g = replay_gate(c, r->photontimer);
             int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * m->ne;
             add_w(c->weight, (size_t)eid + tshift, ww);
         } else if (Lp0 > EPS) {
             r->photontimer += r->Lmove * rc;
             float ww = currweight - r->weight;
-            if (mua > 0.f || cfg->outputtype != ORC_OT_ENERGY) {
+            if (mua > 0.f || fluence_like(cfg->outputtype)) {
                 float ratio = r->Lmove / Lp0;
                 if (mua > 0.f) r->Eabsorb += ww;
-                if (cfg->outputtype != ORC_OT_ENERGY)
+                if (fluence_like(cfg->outputtype))
                     ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
-                int g = (int)((r->photontimer - cfg->tstart) * c->rtstep);
+                int g = replay_gate(c, r->photontimer);
                 int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * m->nn;
                 ww *= 0.5f;
                 if (r->isend)
@@ -823,7 +859,9 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
     if (orc_launch(&r, c, ran)) return -1;
     float w0 = r.weight;
     ppath[reclen - 2] = r.weight;
-    *launchw += r.weight;
+    /* replay with weighted path / scattering outputs: the detected weight
+     * is the launched energy (src/mmc_raytrace.c:1815-1817) */
+    *launchw += (cfg->outputtype == ORC_OT_WL || cfg->outputtype == ORC_OT_WP) ? c->rw : r.weight;
 
 #define PPATH_ADD(cond)                                                            \
     do {                                                                          \
@@ -1023,6 +1061,8 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
             if (out->rec && id >= cfg->recfirst && id < cfg->recfirst + cfg->reccount)
                 rec = out->rec + (id - cfg->recfirst);
             double lw = 0.0, aw = 0.0;
+            c.rw = cfg->replayweight ? cfg->replayweight[id] : 1.f;
+            c.rt = cfg->replaytime ? cfg->replaytime[id] : 0.f;
             if (orc_onephoton(id, &c, ran, &det, &lw, &aw, rec) < 0) err = 1;
             lsum += lw;
             asum += aw;
@@ -1042,11 +1082,18 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
     return err ? -1 : 0;
 }
 
-/* mesh_normalize, src/mmc_mesh.c:2154-2279 (single source, no RF/replay). */
+/* mesh_normalize, src/mmc_mesh.c:2154-2279 (single source, no RF). */
 double orc_normalize(const orc_cfg* cfg, const orc_mesh* mesh, double* weight, float Eabsorb,
                      float Etotal) {
     int datalen = cfg->basisorder ? mesh->nn : mesh->ne;
     double energydeposit = 0.f, normalizor;
+    if (cfg->photonseed && (cfg->outputtype == ORC_OT_JACOBIAN || cfg->outputtype == ORC_OT_WL ||
+                            cfg->outputtype == ORC_OT_WP)) {
+        float nz = 1.f / (DELTA_MUA * cfg->nphoton);
+        if (cfg->outputtype == ORC_OT_WL || cfg->outputtype == ORC_OT_WP) nz = 1.f / Etotal;
+        for (size_t i = 0; i < (size_t)cfg->maxgate * datalen; i++) weight[i] *= nz;
+        return nz;
+    }
     if (cfg->outputtype == ORC_OT_ENERGY) {
         normalizor = 1.f / Etotal;
         for (size_t i = 0; i < (size_t)cfg->maxgate * datalen; i++) weight[i] *= normalizor;

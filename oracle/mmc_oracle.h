@@ -31,6 +31,9 @@ extern "C" {
 #define ORC_OT_FLUX    0
 #define ORC_OT_FLUENCE 1
 #define ORC_OT_ENERGY  2
+#define ORC_OT_JACOBIAN 3     /* replay outputs, enum TOutputType (src/mmc_utils.h:106) */
+#define ORC_OT_WL       4
+#define ORC_OT_WP       5
 /* boundary = enum TBoundary (src/mmc_utils.h:116) */
 #define ORC_BC_NOREFLECT 0
 #define ORC_BC_REFLECT   1
@@ -78,6 +81,8 @@ typedef struct {
     const float* detpos;              /* detnum*4: x,y,z,r */
     uint32_t seed;
     const uint64_t* photonseed;       /* nphoton*2 xorshift128+ states, or NULL */
+    const float* replayweight;        /* replay J/L/P: detected weight per photon, or NULL */
+    const float* replaytime;          /* replay L/P: detected time of flight per photon, or NULL */
     int nthreads;                     /* <=0: OpenMP default */
     int64_t recfirst, reccount;       /* photon-record window [recfirst, recfirst+reccount) */
 } orc_cfg;

@@ -41,6 +41,7 @@ class OCfg(C.Structure):
                 ("tstart", C.c_float), ("tend", C.c_float), ("tstep", C.c_float), ("maxgate", C.c_int),
                 ("nout", C.c_float), ("minenergy", C.c_float), ("roulettesize", C.c_float),
                 ("detnum", C.c_int), ("detpos", f32p), ("seed", C.c_uint32), ("photonseed", u64p),
+                ("replayweight", f32p), ("replaytime", f32p),
                 ("nthreads", C.c_int), ("recfirst", C.c_int64), ("reccount", C.c_int64)]
 
 
@@ -157,7 +158,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     c.nphoton = int(sim.nphoton)
     c.method = {"P": 0, "S": 3}[sim.method]
     c.basisorder = int(sim.basisorder)
-    c.outputtype = {"x": 0, "f": 1, "e": 2}[sim.outputtype]
+    c.outputtype = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5}[sim.outputtype]
     c.isreflect, c.isspecular, c.voidtime = int(sim.isreflect), int(sim.isspecular), int(sim.voidtime)
     c.issavedet = int(bool(sim.issavedet and sim.detpos is not None and len(sim.detpos)))
     c.issaveexit = int(sim.issaveexit) if c.issavedet else 0
@@ -204,6 +205,14 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
         c.photonseed = p(ps, C.c_uint64)
     else:
         c.photonseed = C.cast(None, u64p)
+    for name in ("replayweight", "replaytime"):
+        v = getattr(sim, name, None)
+        if v is not None:
+            a = np.ascontiguousarray(v, np.float32).ravel()
+            keep.append(a)
+            setattr(c, name, p(a, C.c_float))
+        else:
+            setattr(c, name, C.cast(None, f32p))
     c.nthreads = int(nthreads)
     c.recfirst, c.reccount = (int(records[0]), int(records[1])) if records else (0, 0)
     c._keep = keep

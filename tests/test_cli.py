@@ -26,8 +26,10 @@ def test_help_and_errors(tmp_path):
     assert r.returncode != 0 and "unsupported compression" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-F", "nifti"], tmp_path)
     assert r.returncode != 0 and "unsupported output format" in r.stderr
-    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--replaydet", "1"], tmp_path)
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--momentum", "1"], tmp_path)
     assert r.returncode != 0 and "unsupported option" in r.stderr
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-E", str(tmp_path / "none.mch")], tmp_path)
+    assert r.returncode != 0
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-G", "-1"], tmp_path)
     assert r.returncode != 0 and "CPU path" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-M", "H"], tmp_path)
@@ -98,3 +100,29 @@ def test_every_output_format_holds_the_same_field(tmp_path):
     assert ref.shape == (80,) and ref.sum() > 0
     for fmt, f in fields.items():
         assert np.allclose(f, ref, rtol=1e-5, atol=1e-7 * ref.max()), fmt
+
+
+@pytest.mark.gpu
+def test_saved_seeds_replay_the_detected_photons(tmp_path):
+    # -q 1 appends each saved photon's seed; -E file.mch relaunches them
+    # (src/mmc_mesh.c:815-898) and -O L deposits the weighted path length
+    cube2 = os.path.join(GOLDEN, "cube2", "cube2.json")
+    r = run(["-f", cube2, "-n", "20000", "-M", "S", "-C", "0", "-d", "1", "-q", "1", "-s", "orig"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    head, rec = read_mch(tmp_path / "orig.mch")
+    n, col = head[7], head[4]
+    raw = (tmp_path / "orig.mch").read_bytes()
+    assert head[9] == 16 and len(raw) == 64 + 4 * n * col + 16 * n     # seedbyte, seed block
+    r = run(["-f", cube2, "-M", "S", "-C", "0", "-d", "1", "-E", str(tmp_path / "orig.mch"), "-s", "rep"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    head2, rec2 = read_mch(tmp_path / "rep.mch")
+    assert head2[7] == n                                   # every replayed photon is detected again
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert np.array_equal(key(rec2), key(rec))
+    r = run(["-f", cube2, "-M", "S", "-C", "0", "-E", str(tmp_path / "orig.mch"), "-O", "L", "-U", "0",
+             "-F", "bin", "-s", "wl"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    f = np.fromfile(tmp_path / "wl.bin", np.float64)
+    # one medium: replay weight w0*exp(-mua*ppath), deposit = weight * path
+    w = rec[:, -1].astype(np.float64) * np.exp(-0.005 * rec[:, 2].astype(np.float64))
+    assert abs(f.sum() / (w * rec[:, 2]).sum() - 1) < 1e-4

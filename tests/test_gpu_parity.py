@@ -268,3 +268,40 @@ def test_served_results_equal_results_alone():
         assert abs(r.raytet - alone.raytet) == 0
         rel = np.linalg.norm(r.field - alone.field) / np.linalg.norm(alone.field)
         assert rel < 1e-6, rel
+
+
+def test_replay_of_saved_seeds_reproduces_the_detected_photons(slab):
+    # seed replay (src/mmc_mesh.c:815-898): relaunching the detected photons
+    # from their initial states gives the same detection records
+    from mmc_amd import replay as R
+    _, tm, _ = slab
+    cfg = W.slab_sim(30000, seed=23)
+    g = M.run(tm, cfg)
+    assert g.detected > 100
+    seeds = R.detected_seeds(cfg, g)
+    order = np.argsort(g.detphoton)
+    rcfg = M.SimConfig(**{**cfg.__dict__, "nphoton": len(seeds), "photonseed": seeds[order]})
+    r = M.run(tm, rcfg, records=(0, len(seeds)))
+    assert r.detected == len(seeds)
+    ri = np.argsort(r.detphoton)
+    assert np.array_equal(r.detdata[ri], g.detdata[order])
+
+
+@pytest.mark.parametrize("out", ["j", "l", "p"])
+def test_replay_outputs_match_the_oracle(slab, out):
+    # Jacobian, weighted path and weighted scattering count of the replayed
+    # detected photons against the oracle's restatement of the same formulas
+    from mmc_amd import replay as R
+    arrays, tm, om = slab
+    cfg = W.slab_sim(30000, seed=29)
+    g = M.run(tm, cfg)
+    seeds = R.detected_seeds(cfg, g)
+    mch = R.MchFile(len(arrays[3]) - 1, len(cfg.detpos), g.detdata.shape[1], cfg.nphoton, g.detected, 1.0, 1.0,
+                    g.detdata, seeds)
+    rcfg = R.replay_config(cfg, mch, arrays[3], out)
+    rcfg.issavedet = 0
+    gr = M.run(tm, rcfg, tune=M.sim.tune_struct(compensated=1), normalize=True)
+    o = O.run(om, rcfg, photonseed=rcfg.photonseed, math="portable", normalize=True)
+    assert gr.launchweight == pytest.approx(o.launchweight, rel=1e-6)
+    assert np.linalg.norm(gr.field - o.field) <= 1e-5 * np.linalg.norm(o.field)
+    assert abs(gr.normalizer / o.normalizer - 1) < 1e-6
