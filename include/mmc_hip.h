@@ -23,11 +23,15 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 5
+#define MMC_HIP_ABI_VERSION 6
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
 #define MMC_RT_BLBADOUEL 3
+/* dual-grid MMC (DMMC, "-M G", the reference's GPU default): branch-less
+ * Badouel tracing with the deposits on a voxel grid over the mesh's bounding
+ * box (src/mmc_raytrace.c:1615-1673, src/mmc_mesh.c:349-381) */
+#define MMC_RT_BLBADOUELGRID 4
 /* source types (enum TSrcType, src/mmc_utils.h:101) */
 #define MMC_SRC_PENCIL   0
 #define MMC_SRC_PLANAR   4
@@ -42,6 +46,12 @@ extern "C" {
 #define MMC_OUT_JACOBIAN 3
 #define MMC_OUT_WL       4
 #define MMC_OUT_WP       5
+/* debug flag that records photon trajectories (dlTraj, "-D M",
+ * src/mmc_utils.h:92-94) */
+#define MMC_DL_TRAJ      8192
+/* floats per trajectory record: photon id (uint32 bits), x, y, z, weight,
+ * element id (int32 bits) (MCX_DEBUG_REC_LEN, src/mmc_const.h:47) */
+#define MMC_TRAJ_RECLEN  6
 /* boundary conditions (enum TBoundary, src/mmc_utils.h:116) */
 #define MMC_BC_NOREFLECT 0
 #define MMC_BC_REFLECT   1
@@ -75,8 +85,8 @@ typedef struct mmc_hip_sim {
     int64_t nphoton;
     uint64_t seed;           /* photon i starts from mmc_hip_photon_seeds(seed, i) */
     int64_t photon_offset;   /* id of the first photon (respin / chunked runs) */
-    int32_t method;          /* MMC_RT_PLUCKER or MMC_RT_BLBADOUEL */
-    int32_t basisorder;      /* 0 element field, 1 nodal field */
+    int32_t method;          /* MMC_RT_PLUCKER, MMC_RT_BLBADOUEL or MMC_RT_BLBADOUELGRID */
+    int32_t basisorder;      /* 0 element field, 1 nodal field (ignored by the dual grid) */
     int32_t outputtype;
     int32_t isreflect, isspecular, voidtime;
     int32_t srctype, srcnum;
@@ -103,6 +113,13 @@ typedef struct mmc_hip_sim {
     const uint64_t* photonseed;
     const float* replayweight;
     const float* replaytime;
+    /* dual grid: voxel edge in mesh units (cfg->steps.x, "--gridsize",
+     * Domain.Step); the grid covers the mesh's bounding box widened by 1e-6
+     * (mmc_hip_sim_grid / mmc_host_grid give its origin and dimensions) */
+    float gridstep;
+    float unitinmm;          /* mesh length unit in mm (normalization of grid fields); 0 = 1 */
+    int32_t debuglevel;      /* MMC_DL_TRAJ: record each photon's launch, scattering and end
+                                positions (savedebugdata, src/mmc_raytrace.c:1739-1756) */
 } mmc_hip_sim;
 
 /* Per-photon end state, written for photon ids in [recfirst, recfirst+reccount). */
@@ -114,7 +131,8 @@ typedef struct mmc_hip_photon {
 } mmc_hip_photon;
 
 typedef struct mmc_hip_result {
-    double* field;           /* datalen*maxgate, datalen = nn (basisorder 1) or ne; accumulated (+=);
+    double* field;           /* datalen*maxgate, datalen = nn (basisorder 1), ne, or the voxels of
+                                the dual grid (nx*ny*nz, x fastest); accumulated (+=);
                                 NULL keeps the field on the device (fieldsum is still filled) */
     double fieldsum;         /* sum of the raw (unnormalized) field of this call, FP64 */
     double launchweight;     /* total launched weight   (cfg->energytot) */
@@ -132,6 +150,12 @@ typedef struct mmc_hip_result {
     double total_ms;         /* including uploads, reductions and copies */
     double* srcweight;       /* pattern sources with srcnum > 1: launched energy of each pattern,
                                 then absorbed energy of each pattern (2*srcnum, +=), or NULL */
+    float* traj;             /* maxtraj records of MMC_TRAJ_RECLEN floats (cfg->exportdebugdata), or
+                                NULL; one buffer per launch, filled through the first result of a
+                                batch that provides one, in device order (each photon's records in
+                                order) */
+    int64_t maxtraj;         /* capacity in records (cfg->maxjumpdebug) */
+    int64_t ntraj;           /* records produced (may exceed maxtraj: the rest were dropped) */
 } mmc_hip_result;
 
 /* Launch shape for the persistent kernel; zeros pick the defaults. */
@@ -172,6 +196,10 @@ int32_t mmc_hip_run_batch(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_
                           const mmc_hip_tune* tune, mmc_hip_result* res);
 /* Device bytes the outputs of a simulation occupy (admission accounting). */
 int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
+/* Origin and voxel counts of a dual-grid simulation's output on a resident
+ * mesh (the grid cfg->dim / mesh->nmin of mesh_createdualmesh,
+ * src/mmc_mesh.c:349-381); 0 on success. */
+int32_t mmc_hip_sim_grid(mmc_hip_mesh_handle h, const mmc_hip_sim* sim, float nmin[3], int32_t dim[3]);
 
 /* ---- continuous batching (no reference counterpart): simulations are run in
  *      photon chunks whose FP32 fields stay in a caller-managed device arena

@@ -57,6 +57,11 @@ void mmc_host_surface_nvol(int32_t nn, int32_t ne, const int32_t* elem, const in
                            int32_t isreflect, float* nvol);
 /* Numbers exterior faces -1,-2,... in element-major order; returns the count. */
 int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb);
+/* Dual-grid extent (mesh_createdualmesh, src/mmc_mesh.c:349-381): the
+ * node bounding box widened by 1e-6 gives the grid origin nmin and dim =
+ * (int)(extent / step) + 1 voxels per axis.  Returns 0, or -1 for a bad
+ * step. */
+int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], int32_t dim[3]);
 /* Normalizes a field in place and returns the factor (mesh_normalize). */
 double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
                           const int32_t* elem, const int32_t* type, const float* med,
@@ -69,6 +74,10 @@ double mmc_host_normalize_pair(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
                                const int32_t* elem, const int32_t* type, const float* med,
                                const float* evol, const float* nvol, double* field,
                                double absorbweight, double launchweight, int32_t pair);
+/* The same for a dual-grid field of ncells voxels per gate (fluence per
+ * unit launched energy and unitinmm^3, src/mmc_mesh.c:2205-2206). */
+double mmc_host_normalize_grid(const mmc_hip_sim* sim, int64_t ncells, double* field, double absorbweight,
+                               double launchweight, int32_t pair);
 
 /* Writers.  Return 0 on success. */
 int32_t mmc_host_save_dat(const char* fname, const double* field, int32_t datalen,
@@ -86,12 +95,18 @@ int32_t mmc_host_save_mch(const char* fname, const mmc_mch_header* his, const fl
 #define MMC_FMT_TX3   5   /* <session>.tx3, GL_RGBA32F + int dim[3] + doubles */
 #define MMC_FMT_JNII  6   /* <session>.jnii, JNIfTI JSON (zlib + base64, or raw base64) */
 #define MMC_FMT_BNII  7   /* <session>.bnii, JNIfTI binary JSON (BJData draft 2) */
-/* Saves a [srcnum][maxgate][datalen] double field as <session>.<ext>
+/* Saves a [maxgate][datalen][srcnum] double field as <session>.<ext>
  * (mesh_saveweight + mcx_savedata, src/mmc_mesh.c:1726-1800,
  * src/mmc_utils.c:919-993); compress selects zlib for jnii/bnii. */
 int32_t mmc_host_save_field(const char* session, int32_t format, const double* field, int32_t datalen,
                             int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype,
                             int32_t compress);
+/* Saves a dual-grid field ([maxgate][nz][ny][nx][srcnum] doubles over dim
+ * voxels of edge step) in the same formats (mcx_savedata with cfg->dim,
+ * src/mmc_utils.c:919-993). */
+int32_t mmc_host_save_grid(const char* session, int32_t format, const double* field, const int32_t dim[3],
+                           float step, int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype,
+                           int32_t compress);
 /* Index of a -F name ("ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii",
  * "bnii"), or -1. */
 int32_t mmc_host_format_id(const char* name);

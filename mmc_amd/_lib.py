@@ -46,6 +46,7 @@ class SimC(C.Structure):
         ("detnum", C.c_int32), ("detpos", f32p), ("maxdetphoton", C.c_int64),
         ("med", f32p),
         ("photonseed", C.POINTER(C.c_uint64)), ("replayweight", f32p), ("replaytime", f32p),
+        ("gridstep", C.c_float), ("unitinmm", C.c_float), ("debuglevel", C.c_int32),
     ]
 
 
@@ -72,6 +73,7 @@ class ResultC(C.Structure):
         ("detdata", f32p), ("detphoton", i64p), ("detected", C.c_int64), ("reclen", C.c_int32),
         ("photons", C.c_void_p), ("recfirst", C.c_int64), ("reccount", C.c_int64),
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("srcweight", f64p),
+        ("traj", f32p), ("maxtraj", C.c_int64), ("ntraj", C.c_int64),
     ]
 
 
@@ -95,6 +97,7 @@ SIGNATURES = {
     "mmc_hip_run_batch": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.POINTER(TuneC),
                                       C.POINTER(ResultC)]),
     "mmc_hip_sim_output_bytes": (C.c_int64, [C.c_void_p, C.POINTER(SimC)]),
+    "mmc_hip_sim_grid": (C.c_int32, [C.c_void_p, C.POINTER(SimC), f32p, i32p]),
     "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "mmc_hip_arena_release": (C.c_int32, [C.c_void_p]),
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
@@ -115,6 +118,9 @@ SIGNATURES = {
     "mmc_host_surface_nvol": (None, [C.c_int32, C.c_int32, i32p, i32p, i32p, f32p, C.c_int32,
                                      C.c_int32, f32p]),
     "mmc_host_number_exterior": (C.c_int32, [C.c_int32, i32p]),
+    "mmc_host_grid": (C.c_int32, [f32p, C.c_int32, C.c_float, f32p, i32p]),
+    "mmc_host_normalize_grid": (C.c_double, [C.POINTER(SimC), C.c_int64, f64p, C.c_double, C.c_double,
+                                             C.c_int32]),
     "mmc_host_normalize": (C.c_double, [C.POINTER(SimC), C.c_int32, C.c_int32, i32p, i32p, f32p,
                                         f32p, f32p, f64p, C.c_double, C.c_double]),
     "mmc_host_normalize_pair": (C.c_double, [C.POINTER(SimC), C.c_int32, C.c_int32, i32p, i32p, f32p,
@@ -124,11 +130,13 @@ SIGNATURES = {
     "mmc_host_save_mch": (C.c_int32, [C.c_char_p, C.c_void_p, f32p, C.c_int64]),
     "mmc_host_save_field": (C.c_int32, [C.c_char_p, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_float, C.c_int32, C.c_int32]),
+    "mmc_host_save_grid": (C.c_int32, [C.c_char_p, C.c_int32, f64p, i32p, C.c_float, C.c_int32, C.c_int32,
+                                       C.c_float, C.c_int32, C.c_int32]),
     "mmc_host_format_id": (C.c_int32, [C.c_char_p]),
 }
 
 
-ABI_VERSION = 5        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 6        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

@@ -30,11 +30,12 @@ typedef struct {
     char session[256];
     char meshid[512];
     char rootdir[1024];
-    char method;           /* 'p' or 's' */
+    char method;           /* 'p', 's' or 'g' */
     int isnormalized, issave2pt, outputformat; /* MMC_FMT_* (enum TOutputFormat order) */
     int zip;                        /* jnii/bnii: 1 zlib (default), 0 raw */
     long seed;
     float unitinmm;
+    float gridstep;        /* dual-grid voxel edge (cfg->steps.x) */
     /* mesh given inline in the JSON */
     float* node;
     int nn;
@@ -49,6 +50,12 @@ typedef struct {
     /* replay (-E file.mch) and seed saving (-q 1) */
     char seedfile[1024];
     int issaveseed, replaydet;
+    /* --dumpjson [file|-]: write the settings as JSON and exit */
+    int dumpjson;
+    char jsonfile[1024];
+    /* -D M: photon trajectories, at most maxjumpdebug records */
+    int savetraj;
+    long long maxjumpdebug;
     uint64_t* rseeds;
     float *rweight, *rtime;
 } cli_cfg;
@@ -60,6 +67,8 @@ static void die(const char* fmt, const char* arg) {
     exit(1);
 }
 
+static void debug_flags(cli_cfg* c, const char* v);
+
 static void defaults(cli_cfg* c) {
     memset(c, 0, sizeof(*c));
     /* mcx_initcfg, src/mmc_utils.c:207-363 */
@@ -69,7 +78,11 @@ static void defaults(cli_cfg* c) {
     c->isnormalized = 1;
     c->issave2pt = 1;
     c->sim.basisorder = 1;
-    c->method = 'p';
+    /* the GPU builds of the reference default to the dual grid
+     * (src/mmc_utils.c:241-243) */
+    c->method = 'g';
+    c->gridstep = 1.f;
+    c->maxjumpdebug = 10000000;       /* src/mmc_utils.c:342 */
     c->sim.minenergy = 1e-6f;
     c->sim.roulettesize = 10.f;
     c->sim.nout = 1.f;
@@ -169,6 +182,11 @@ static void load_json(cli_cfg* c, const char* path) {
             }
         }
         if ((v = mj_get(Domain, "LengthUnit"))) c->unitinmm = (float)mj_num(v, 1);
+        /* Domain.Step: the dual grid's voxel edge (src/mmc_utils.c:1791-1801) */
+        if ((v = mj_get(Domain, "Step"))) {
+            if (v->type == MJ_ARRAY && mj_size(v) >= 1) c->gridstep = (float)mj_num(mj_at(v, 0), 1);
+            else die("%s", "Domain::Step has incorrect element numbers");
+        }
         if ((v = mj_get(Domain, "MeshID"))) snprintf(c->meshid, sizeof(c->meshid), "%s", mj_str(v, ""));
     }
     if (Optode) {
@@ -217,6 +235,11 @@ static void load_json(cli_cfg* c, const char* path) {
             if (c->outputformat < 0) die("unsupported Session.OutputFormat '%s'", mj_str(v, ""));
         }
         if ((v = mj_get(Session, "RayTracer"))) c->method = mj_str(v, "p")[0] | 0x20;
+        if ((v = mj_get(Session, "DebugFlag"))) {
+            char num[32];
+            if (v->type == MJ_NUMBER) snprintf(num, sizeof(num), "%d", (int)v->num);
+            debug_flags(c, v->type == MJ_NUMBER ? num : mj_str(v, ""));
+        }
         if ((v = mj_get(Session, "OutputType"))) {
             char t = mj_str(v, "x")[0] | 0x20;
             c->sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
@@ -229,6 +252,152 @@ static void load_json(cli_cfg* c, const char* path) {
         c->sim.nout = (float)mj_num(mj_get(Forward, "N0"), c->sim.nout);
     }
     mj_free(root);
+}
+
+/* mcx_loadconfig (src/mmc_utils.c:2425-2670): the legacy text input, one
+ * setting per line followed by an optional comment: photons, seed, source
+ * position, direction (optional w), time gates, mesh id, initial element,
+ * detector count and radius, detectors (optional own radius), then
+ * optionally the source type, its two parameter sets and, for a pattern,
+ * the pattern file and count. */
+/* the rest of the current line (the comment after a setting); 1 if read */
+static int line_rest(FILE* fp, char* buf, int len) { return fgets(buf, len, fp) != NULL; }
+
+static void load_inp(cli_cfg* c, const char* path) {
+    static const char* const names[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier",
+                                        "arcsine", "disk", "fourierx", "fourierx2d", "zgaussian", "line", "slit",
+                                        ""};
+    FILE* fp = fopen(path, "rt");
+    if (!fp) die("can not read input file %s", path);
+    char rest[4096], word[1024];
+    long long nphoton = 0;
+    int seed = 0, detnum = 0;
+    float detr = 0.f, v;
+    float* sp = c->sim.srcpos;
+    float* sd = c->sim.srcdir;
+#define INP_READ(n, ...) \
+    do { \
+        if (fscanf(fp, __VA_ARGS__) != (n)) die("%s", "the .inp file is incomplete"); \
+    } while (0)
+#define INP_REST() line_rest(fp, rest, sizeof(rest))
+    INP_READ(1, "%lld", &nphoton);
+    INP_REST();
+    INP_READ(1, "%d", &seed);
+    INP_REST();
+    INP_READ(3, "%f %f %f", sp, sp + 1, sp + 2);
+    INP_REST();
+    INP_READ(3, "%f %f %f", sd, sd + 1, sd + 2);
+    if (INP_REST() && sscanf(rest, "%f", &v) == 1) sd[3] = v;
+    INP_READ(3, "%f %f %f", &c->sim.tstart, &c->sim.tend, &c->sim.tstep);
+    INP_REST();
+    INP_READ(1, "%1023s", word);
+    snprintf(c->meshid, sizeof(c->meshid), "%.511s", word);
+    INP_REST();
+    INP_READ(1, "%d", &c->sim.e0);
+    INP_REST();
+    INP_READ(2, "%d %f", &detnum, &detr);
+    INP_REST();
+    if (detnum < 0 || detnum > MAXDET) die("%s", "detector count out of range");
+    for (int i = 0; i < detnum; i++) {
+        float* d = c->detpos + 4 * i;
+        INP_READ(3, "%f %f %f", d, d + 1, d + 2);
+        d[3] = (INP_REST() && sscanf(rest, "%f", &v) == 1) ? v : detr;
+    }
+    c->sim.detnum = detnum;
+    if (fscanf(fp, "%1023s", word) == 1 && word[0]) {
+        int t = keyidx(word, names);
+        if (t < 0) die("%s", "the specified source type is not supported");
+        c->sim.srctype = t;
+        INP_REST();
+        float* p1 = c->sim.srcparam1;
+        float* p2 = c->sim.srcparam2;
+        INP_READ(4, "%f %f %f %f", p1, p1 + 1, p1 + 2, p1 + 3);
+        INP_REST();
+        if (fscanf(fp, "%f %f %f %f", p2, p2 + 1, p2 + 2, p2 + 3) == 4) {
+            INP_REST();
+            if (t == MMC_SRC_PATTERN && p1[3] * p2[3] > 0) {
+                int srcnum = 1;
+                if (fscanf(fp, "%1023s %d", word, &srcnum) < 1) die("%s", "missing source pattern file");
+                if (srcnum < 1) die("%s", "the number of patterns cannot be smaller than 1");
+                c->sim.srcnum = srcnum;
+                size_t n = (size_t)(p1[3] * p2[3]) * (size_t)srcnum;
+                c->pattern = (float*)calloc(n, sizeof(float));
+                FILE* pf = fopen(word, "rb");
+                if (!pf) {
+                    char alt[4096];
+                    snprintf(alt, sizeof(alt), "%s%s", c->rootdir, word);
+                    pf = fopen(alt, "rb");
+                }
+                if (!pf) die("%s", "source pattern file can not be opened");
+                if (fread(c->pattern, sizeof(float), n, pf) != n) die("%s", "source pattern file is too short");
+                fclose(pf);
+                c->sim.srcpattern = c->pattern;
+            }
+        }
+        if (detnum == 1 && c->detpos[3] == 0.f) die("%s", "wide-field detectors are not supported in this build");
+    }
+#undef INP_READ
+#undef INP_REST
+    fclose(fp);
+    c->sim.nphoton = (int64_t)nphoton;
+    c->seed = seed;
+}
+
+/* mcx_savejdata (src/mmc_utils.c:2710-2850): the settings as a JSON input
+ * with the mesh inline, readable by -f; '-' writes to stdout */
+static void dump_json(const cli_cfg* c) {
+    static const char* const names[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern"};
+    static const char otype[] = "xfejlp";
+    FILE* fp = (!c->jsonfile[0] || !strcmp(c->jsonfile, "-")) ? stdout : fopen(c->jsonfile, "wt");
+    if (!fp) die("can not write %s", c->jsonfile);
+    const mmc_hip_sim* s = &c->sim;
+    fprintf(fp, "{\n\t\"Session\": {\n\t\t\"ID\": \"%s\",\n\t\t\"Photons\": %lld,\n\t\t\"RNGSeed\": %u,\n",
+            c->session, (long long)s->nphoton, (unsigned)c->seed);
+    fprintf(fp, "\t\t\"DoMismatch\": %d,\n\t\t\"DoSaveVolume\": %d,\n\t\t\"DoNormalize\": %d,\n"
+                "\t\t\"DoPartialPath\": %d,\n\t\t\"DoSaveExit\": %d,\n\t\t\"DoSaveSeed\": %d,\n"
+                "\t\t\"DoSpecular\": %d,\n\t\t\"DebugFlag\": %d,\n\t\t\"BasisOrder\": %d,\n",
+            s->isreflect, c->issave2pt, c->isnormalized, s->issavedet, s->issaveexit, c->issaveseed, s->isspecular,
+            c->savetraj ? MMC_DL_TRAJ : 0, s->basisorder);
+    const char* fmts[] = {"ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii", "bnii"};
+    fprintf(fp, "\t\t\"RayTracer\": \"%c\",\n\t\t\"OutputFormat\": \"%s\",\n\t\t\"OutputType\": \"%c\"\n\t},\n",
+            c->method, fmts[c->outputformat], otype[s->outputtype >= 0 && s->outputtype <= 5 ? s->outputtype : 0]);
+    fprintf(fp, "\t\"Forward\": {\n\t\t\"T0\": %.9g,\n\t\t\"T1\": %.9g,\n\t\t\"Dt\": %.9g,\n\t\t\"N0\": %.9g\n\t},\n",
+            (double)s->tstart, (double)s->tend, (double)s->tstep, (double)s->nout);
+    fprintf(fp, "\t\"Domain\": {\n\t\t\"LengthUnit\": %.9g,\n\t\t\"Step\": [%.9g, %.9g, %.9g],\n\t\t\"Media\": [",
+            (double)c->unitinmm, (double)c->gridstep, (double)c->gridstep, (double)c->gridstep);
+    for (int i = 0; i < c->nmed; i++)
+        fprintf(fp, "%s\n\t\t\t{\"mua\": %.9g, \"mus\": %.9g, \"g\": %.9g, \"n\": %.9g}", i ? "," : "",
+                (double)c->med[4 * i], (double)c->med[4 * i + 1], (double)c->med[4 * i + 2], (double)c->med[4 * i + 3]);
+    fprintf(fp, "\n\t\t]\n\t},\n\t\"Mesh\": {\n\t\t\"MeshID\": \"%s\",\n\t\t\"InitElem\": %d,\n\t\t\"MeshNode\": [",
+            c->meshid, s->e0);
+    for (int i = 0; i < c->nn; i++)
+        fprintf(fp, "%s\n\t\t\t[%.9g, %.9g, %.9g]", i ? "," : "", (double)c->node[3 * i], (double)c->node[3 * i + 1],
+                (double)c->node[3 * i + 2]);
+    fprintf(fp, "\n\t\t],\n\t\t\"MeshElem\": [");
+    for (int i = 0; i < c->ne; i++) {
+        const int* e = c->elem5 + 5 * i;
+        fprintf(fp, "%s\n\t\t\t[%d, %d, %d, %d, %d]", i ? "," : "", e[0], e[1], e[2], e[3], e[4]);
+    }
+    fprintf(fp, "\n\t\t]\n\t},\n\t\"Optode\": {\n\t\t\"Source\": {\n\t\t\t\"Type\": \"%s\",\n",
+            names[s->srctype >= 0 && s->srctype <= 5 ? s->srctype : 0]);
+    fprintf(fp, "\t\t\t\"Pos\": [%.9g, %.9g, %.9g],\n\t\t\t\"Dir\": [%.9g, %.9g, %.9g, %.9g],\n",
+            (double)s->srcpos[0], (double)s->srcpos[1], (double)s->srcpos[2], (double)s->srcdir[0],
+            (double)s->srcdir[1], (double)s->srcdir[2], (double)s->srcdir[3]);
+    fprintf(fp, "\t\t\t\"Param1\": [%.9g, %.9g, %.9g, %.9g],\n\t\t\t\"Param2\": [%.9g, %.9g, %.9g, %.9g]",
+            (double)s->srcparam1[0], (double)s->srcparam1[1], (double)s->srcparam1[2], (double)s->srcparam1[3],
+            (double)s->srcparam2[0], (double)s->srcparam2[1], (double)s->srcparam2[2], (double)s->srcparam2[3]);
+    if (s->srctype == MMC_SRC_PATTERN && c->pattern) {
+        size_t n = (size_t)(s->srcparam1[3] * s->srcparam2[3]) * (size_t)(s->srcnum > 0 ? s->srcnum : 1);
+        fprintf(fp, ",\n\t\t\t\"Pattern\": [");
+        for (size_t i = 0; i < n; i++) fprintf(fp, "%s%.9g", i ? ", " : "", (double)c->pattern[i]);
+        fprintf(fp, "]");
+    }
+    fprintf(fp, "\n\t\t},\n\t\t\"Detector\": [");
+    for (int i = 0; i < s->detnum; i++)
+        fprintf(fp, "%s\n\t\t\t{\"Pos\": [%.9g, %.9g, %.9g], \"R\": %.9g}", i ? "," : "", (double)c->detpos[4 * i],
+                (double)c->detpos[4 * i + 1], (double)c->detpos[4 * i + 2], (double)c->detpos[4 * i + 3]);
+    fprintf(fp, "\n\t\t]\n\t}\n}\n");
+    if (fp != stdout) fclose(fp);
 }
 
 /* ------------------------------------------------------------------ mesh files */
@@ -331,9 +500,15 @@ static void usage(void) {
         "  -n/--photon N         photons to simulate\n"
         "  -E/--seed N|file.mch  RNG seed (negative: time-based), or replay the saved photons of an .mch\n"
         "  -q/--saveseed [0|1]   save the seeds of detected photons in the .mch (for replay)\n"
+        "  -D/--debug FLAGS      M: save photon trajectories to <session>.mct (other letters of the\n"
+        "                        reference only change its logging and are accepted)\n"
+        "  --maxjumpdebug N      trajectory positions kept (default 10000000)\n"
+        "  --dumpjson [FILE|-]   write the settings (mesh inline) as JSON input and exit\n"
         "  -P/--replaydet N      replay only the photons of detector N (0: all)\n"
         "  -s/--session NAME     output file prefix\n"
-        "  -M/--method [P|S]     Plucker (nodal, -C 1) or branch-less Badouel (element, -C 0)\n"
+        "  -M/--method [G|P|S]   dual grid (voxel output, default), Plucker (nodal, -C 1) or\n"
+        "                        branch-less Badouel (element, -C 0)\n"
+        "  --gridsize D          dual-grid voxel edge in mesh units (default 1)\n"
         "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
         "  -O/--outputtype [X|F|E|J|L|P] flux, fluence, energy deposit; replay: Jacobian,\n"
         "                        weighted path length, weighted scattering count\n"
@@ -351,6 +526,20 @@ static void usage(void) {
         "  -u/--unitinmm U       mesh length unit in mm\n"
         "  -G/--gpu 1            the GPU (one GPU only)\n"
         "  -c/--compute hip      backend (hip only)\n");
+}
+
+/* -D / Session.DebugFlag: letters of debugflag[] (src/mmc_utils.c:139) or a
+ * number; M (dlTraj) records trajectories, the rest select log output */
+static void debug_flags(cli_cfg* c, const char* v) {
+    if (v[0] >= '0' && v[0] <= '9') {
+        if (atoi(v) & MMC_DL_TRAJ) c->savetraj = 1;
+        return;
+    }
+    for (const char* p = v; *p; p++) {
+        char u = (char)(*p & ~0x20);
+        if (!strchr("SCBWDIOXATRPEM", u)) die("unknown debug flag '%s'", v);
+        if (u == 'M') c->savetraj = 1;
+    }
 }
 
 static const char* opt_value(int argc, char** argv, int* i) {
@@ -373,11 +562,13 @@ int main(int argc, char** argv) {
         usage();
         return input ? 0 : 1;
     }
-    load_json(&c, input);
     {
         const char* slash = strrchr(input, '/');
         if (slash) snprintf(c.rootdir, sizeof(c.rootdir), "%.*s/", (int)(slash - input), input);
     }
+    /* JSON input, else the legacy .inp text (mcx_readconfig, src/mmc_utils.c:1475-1500) */
+    if (strstr(input, ".json") || strstr(input, ".JSON")) load_json(&c, input);
+    else load_inp(&c, input);
     /* flags given on the command line win over the file */
     for (int i = 1; i < argc; i++) {
         const char* a = argv[i];
@@ -389,10 +580,19 @@ int main(int argc, char** argv) {
             if (n > 4 && !strcmp(v + n - 4, ".mch")) snprintf(c.seedfile, sizeof(c.seedfile), "%s", v);
             else c.seed = atol(v);
         } else if (!strcmp(a, "-q") || !strcmp(a, "--saveseed")) c.issaveseed = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-D") || !strcmp(a, "--debug")) debug_flags(&c, opt_value(argc, argv, &i));
+        else if (!strcmp(a, "--maxjumpdebug")) c.maxjumpdebug = atoll(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "--dumpjson")) {
+            /* --dumpjson [file]: stdout unless a file name follows */
+            c.dumpjson = 1;
+            if (i + 1 < argc && (argv[i + 1][0] != '-' || !strcmp(argv[i + 1], "-")))
+                snprintf(c.jsonfile, sizeof(c.jsonfile), "%s", argv[++i]);
+        }
         else if (!strcmp(a, "-P") || !strcmp(a, "--replaydet")) c.replaydet = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-s") || !strcmp(a, "--session")) snprintf(c.session, sizeof(c.session), "%s", opt_value(argc, argv, &i));
         else if (!strcmp(a, "-M") || !strcmp(a, "--method")) c.method = opt_value(argc, argv, &i)[0] | 0x20;
         else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "--gridsize")) c.gridstep = (float)atof(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
             char t = opt_value(argc, argv, &i)[0] | 0x20;
             c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : t == 'j' ? MMC_OUT_JACOBIAN
@@ -433,14 +633,27 @@ int main(int argc, char** argv) {
     c.sim.tend = c.sim.tstart + c.sim.tstep * c.sim.maxgate;
     if (c.seed < 0) c.seed = (long)time(NULL);
     c.sim.seed = (uint64_t)(uint32_t)c.seed;
-    c.sim.method = (c.method == 'p') ? MMC_RT_PLUCKER : (c.method == 's') ? MMC_RT_BLBADOUEL : -1;
-    if (c.sim.method < 0) die("%s", "only -M P and -M S are available in this build");
+    c.sim.method = (c.method == 'p')   ? MMC_RT_PLUCKER
+                   : (c.method == 's') ? MMC_RT_BLBADOUEL
+                   : (c.method == 'g') ? MMC_RT_BLBADOUELGRID : -1;
+    if (c.sim.method < 0) die("%s", "only -M G, -M P and -M S are available in this build");
+    /* the dual grid deposits per voxel (src/mmc_utils.c:3554-3556) */
+    if (c.sim.method == MMC_RT_BLBADOUELGRID) {
+        c.sim.basisorder = 0;
+        if (!(c.gridstep > 0.f)) die("%s", "the dual-grid voxel edge (--gridsize, Domain.Step) must be positive");
+    }
+    c.sim.gridstep = c.gridstep;
+    c.sim.unitinmm = c.unitinmm;
     if (c.sim.issavedet && c.sim.detnum == 0) c.sim.issavedet = 0;
     if (!c.sim.issavedet) c.sim.issaveexit = 0;
     c.sim.detpos = c.detpos;
 
     if (!c.node) load_mesh_files(&c);
     if (!c.med) die("%s", "no media given (Domain.Media or prop_<id>.dat)");
+    if (c.dumpjson) {
+        dump_json(&c);
+        return 0;
+    }
     c.med[0] = 0.f;
     c.med[1] = 0.f;
     c.med[2] = 1.f;
@@ -481,10 +694,26 @@ int main(int argc, char** argv) {
     }
     mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med, NULL, 0};
     int datalen = c.sim.basisorder ? c.nn : c.ne;
+    float gridmin[3] = {0.f, 0.f, 0.f};
+    int32_t griddim[3] = {0, 0, 0};
+    const int isgrid = c.sim.method == MMC_RT_BLBADOUELGRID;
+    if (isgrid) {
+        if (mmc_host_grid(c.node, c.nn, c.gridstep, gridmin, griddim)) die("%s", "bad dual-grid voxel edge");
+        int64_t nc = (int64_t)griddim[0] * griddim[1] * griddim[2];
+        if (nc >= (1ll << 31)) die("%s", "the dual grid has more than 2^31 voxels (raise --gridsize)");
+        datalen = (int)nc;
+    }
     double* field = (double*)calloc((size_t)datalen * c.sim.maxgate, sizeof(double));
     mmc_hip_result res;
     memset(&res, 0, sizeof(res));
     res.field = field;
+    if (c.savetraj) {
+        if (c.maxjumpdebug < 1) die("%s", "--maxjumpdebug must be positive");
+        c.sim.debuglevel |= MMC_DL_TRAJ;
+        res.maxtraj = c.maxjumpdebug;
+        res.traj = (float*)calloc((size_t)c.maxjumpdebug * MMC_TRAJ_RECLEN, sizeof(float));
+        if (!res.traj) die("%s", "can not allocate the trajectory buffer");
+    }
     int reclen = 2 * prop + 6 * (c.sim.issaveexit > 0) + 2;
     if (c.sim.issavedet) {
         res.detdata = (float*)calloc((size_t)c.sim.maxdetphoton * reclen, sizeof(float));
@@ -494,22 +723,29 @@ int main(int argc, char** argv) {
     char dev[256] = "";
     mmc_hip_device_info(dev, sizeof(dev), NULL);
     printf("simulating %lld photons on %s (seed %u, method %c, %d gates)\n", (long long)c.sim.nphoton, dev,
-           (unsigned)c.sim.seed, c.method == 'p' ? 'P' : 'S', c.sim.maxgate);
+           (unsigned)c.sim.seed, c.method == 'p' ? 'P' : c.method == 's' ? 'S' : 'G', c.sim.maxgate);
+    if (isgrid)
+        printf("dual grid: %d x %d x %d voxels of %g from (%g, %g, %g)\n", griddim[0], griddim[1], griddim[2],
+               (double)c.gridstep, (double)gridmin[0], (double)gridmin[1], (double)gridmin[2]);
     if (mmc_hip_run(&mesh, &c.sim, NULL, &res) != 0) die("%s", mmc_hip_last_error());
     printf("speed ...\t%.2f photon/ms, %.0f ray-tetrahedron tests (%.2f test/ms)\n",
            (double)c.sim.nphoton / res.total_ms, res.raytet, res.raytet / res.kernel_ms);
     double normalizer = 1.0;
     if (c.isnormalized) {
         mmc_hip_sim s = c.sim;
-        normalizer = mmc_host_normalize(&s, c.nn, c.ne, elem, type, c.med, evol, nvol, field, res.absorbweight,
-                                        res.launchweight);
+        normalizer = isgrid ? mmc_host_normalize_grid(&s, datalen, field, res.absorbweight, res.launchweight, 0)
+                            : mmc_host_normalize(&s, c.nn, c.ne, elem, type, c.med, evol, nvol, field,
+                                                 res.absorbweight, res.launchweight);
     }
     printf("total simulated energy: %.0f\tabsorbed: %5.5f%%\tnormalizor=%g\n", res.launchweight,
            100.0 * res.absorbweight / res.launchweight, normalizer);
     char fname[1024];
     if (c.issave2pt) {
-        if (mmc_host_save_field(c.session, c.outputformat, field, datalen, c.sim.maxgate, 1, c.sim.tstep,
-                                c.sim.outputtype, c.zip))
+        int rc = isgrid ? mmc_host_save_grid(c.session, c.outputformat, field, griddim, c.gridstep, c.sim.maxgate,
+                                             1, c.sim.tstep, c.sim.outputtype, c.zip)
+                        : mmc_host_save_field(c.session, c.outputformat, field, datalen, c.sim.maxgate, 1,
+                                              c.sim.tstep, c.sim.outputtype, c.zip);
+        if (rc)
             die("can not write the output of session %s", c.session);
     }
     if (c.sim.issavedet) {
@@ -552,6 +788,33 @@ int main(int argc, char** argv) {
         }
         printf("detected %lld photons\n", (long long)res.detected);
     }
+    if (c.savetraj) {
+        /* the trajectory file: the history header with colcount 6 and
+         * detected = 0 (src/mmc_host.c:452-459), named .mct by the filetag
+         * rule of mesh_savedetphoton (src/mmc_mesh.c:1982) */
+        int64_t kept = res.ntraj < c.maxjumpdebug ? res.ntraj : c.maxjumpdebug;
+        if (res.ntraj > c.maxjumpdebug)
+            printf("WARNING: %lld trajectory positions, more than --maxjumpdebug %lld; the rest were dropped\n",
+                   (long long)res.ntraj, (long long)c.maxjumpdebug);
+        mmc_mch_header his;
+        memset(&his, 0, sizeof(his));
+        memcpy(his.magic, "MCXH", 4);
+        his.version = 1;
+        his.maxmedia = (uint32_t)prop;
+        his.detnum = (uint32_t)c.sim.detnum;
+        his.colcount = MMC_TRAJ_RECLEN;
+        his.totalphoton = (uint32_t)c.sim.nphoton;
+        his.detected = 0;
+        his.savedphoton = (uint32_t)kept;
+        his.unitinmm = c.unitinmm;
+        his.normalizer = (float)normalizer;
+        his.srcnum = 1;
+        his.respin = 1;
+        snprintf(fname, sizeof(fname), "%s.mct", c.session);
+        if (mmc_host_save_mch(fname, &his, res.traj, kept)) die("can not write %s", fname);
+        printf("saved %lld trajectory positions\n", (long long)kept);
+    }
+    free(res.traj);
     free(field);
     free(res.detdata);
     free(res.detphoton);

@@ -6,13 +6,16 @@
  * 515-607), mcx_savejnii/mcx_savebnii (src/mmc_utils.c:622-905) and the JData
  * array annotation of mcx_jdataencode (src/mmc_utils.c:3071-3185).
  *
- * Mesh output is a [srcnum][maxgate][datalen] array of doubles (datalen =
+ * Mesh output is a [maxgate][datalen][srcnum] array of doubles (datalen =
  * nodes or elements).  NIfTI/Analyze describe it as dim = (srcnum, maxgate,
- * datalen, 1); JNIfTI as Dim = [datalen, maxgate(, srcnum)] with VoxelSize
- * [1, tstep, 1].  JNIfTI data are zlib-compressed (the reference's default
- * zipid) and, in the text form, base64-encoded; `compress = 0` writes the raw
- * array instead.  Binary JSON follows BJData draft 2 (little-endian, with the
- * u/m/M unsigned markers).
+ * datalen, 1); JNIfTI as Dim = [datalen, maxgate, srcnum] with VoxelSize
+ * [1, tstep, 1].  Dual-grid output is [maxgate][nz][ny][nx][srcnum]: NIfTI dim
+ * = (nx, ny, nz, maxgate*srcnum) with the voxel edge as pixdim; JNIfTI Dim =
+ * [nx, ny, nz, maxgate, srcnum], VoxelSize [step, step, step, tstep, 1].
+ * JNIfTI data are zlib-compressed (the reference's default zipid) and, in the
+ * text form, base64-encoded; `compress = 0` writes the raw array instead.
+ * Binary JSON follows BJData draft 2 (little-endian, with the u/m/M unsigned
+ * markers).
  */
 #include <stddef.h>
 #include <stdint.h>
@@ -64,19 +67,28 @@ enum { NII_FLOAT64 = 64, NII_UNITS_MM = 2, NII_UNITS_USEC = 24 };
 
 static int write_all(FILE* fp, const void* p, size_t n) { return fwrite(p, 1, n, fp) == n ? 0 : -1; }
 
-static int save_nifti(const char* session, const double* f, int64_t n, int32_t srcnum, int32_t maxgate,
-                      int32_t datalen, float tstep, int analyze) {
+/* shape of an output array: ndim dims (fastest first) with voxel sizes, and
+ * the 4 NIfTI dims / 3 spatial pixdims */
+typedef struct {
+    int ndim;
+    uint32_t dims[5];
+    float vox[5];
+    int32_t nii[4];
+    float pix;
+} out_shape;
+
+static int16_t nii_dim(int32_t v) { return (int16_t)(v > 32767 ? 32767 : v); }
+
+static int save_nifti(const char* session, const double* f, int64_t n, const out_shape* sh, float tstep,
+                      int analyze) {
     nii1_header h;
     memset(&h, 0, sizeof(h));
     h.sizeof_hdr = 348;
     h.dim[0] = 4;
-    h.dim[1] = (int16_t)srcnum;
-    h.dim[2] = (int16_t)maxgate;
-    h.dim[3] = (int16_t)(datalen > 32767 ? 32767 : datalen);
-    h.dim[4] = 1;
+    for (int k = 0; k < 4; k++) h.dim[1 + k] = nii_dim(sh->nii[k]);
     h.datatype = NII_FLOAT64;
     h.bitpix = 64;
-    h.pixdim[1] = h.pixdim[2] = h.pixdim[3] = 1.f;
+    h.pixdim[1] = h.pixdim[2] = h.pixdim[3] = sh->pix;
     h.pixdim[4] = tstep * 1e6f;
     h.xyzt_units = NII_UNITS_MM | NII_UNITS_USEC;
     memcpy(h.magic, analyze ? "ni1\0" : "n+1\0", 4);
@@ -145,11 +157,9 @@ static const char* output_description(int32_t outputtype) {
 }
 
 /* ------------------------------------------------------- JNIfTI (text) ---- */
-static int save_jnii(const char* session, const double* f, int64_t n, int32_t srcnum, int32_t maxgate,
-                     int32_t datalen, float tstep, int32_t outputtype, int compress) {
-    const int ndim = srcnum > 1 ? 3 : 2;
-    const long dims[3] = {datalen, maxgate, srcnum};
-    const double vox[3] = {1.0, tstep, 1.0};
+static int save_jnii(const char* session, const double* f, int64_t n, const out_shape* sh, int32_t maxgate,
+                     int32_t outputtype, int compress) {
+    const int ndim = sh->ndim;
     size_t rawbytes = sizeof(double) * (size_t)n, zbytes = 0, blen = 0;
     uint8_t* z = NULL;
     if (compress && !(z = zlib_pack(f, rawbytes, &zbytes))) return -1;
@@ -168,11 +178,11 @@ static int save_jnii(const char* session, const double* f, int64_t n, int32_t sr
                 "\t\t\"AnnotationFormat\": \"https://neurojson.org/jnifti/draft1\",\n"
                 "\t\t\"SerialFormat\": \"https://json.org\"\n\t},\n");
     fprintf(fp, "\t\"NIFTIHeader\": {\n\t\t\"NIIHeaderSize\": 348,\n\t\t\"Dim\": [");
-    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%ld", i ? ", " : "", dims[i]);
+    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%u", i ? ", " : "", sh->dims[i]);
     fprintf(fp, "],\n\t\t\"Param1\": 0,\n\t\t\"Param2\": 0,\n\t\t\"Param3\": 0,\n\t\t\"Intent\": 0,\n"
                 "\t\t\"DataType\": \"double\",\n\t\t\"BitDepth\": 64,\n\t\t\"FirstSliceID\": 0,\n"
                 "\t\t\"VoxelSize\": [");
-    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%.9g", i ? ", " : "", vox[i]);
+    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%.9g", i ? ", " : "", (double)sh->vox[i]);
     fprintf(fp, "],\n\t\t\"Orientation\": {\"x\": \"r\", \"y\": \"a\", \"z\": \"s\"},\n"
                 "\t\t\"ScaleSlope\": 1,\n\t\t\"ScaleOffset\": 0,\n\t\t\"LastSliceID\": %d,\n"
                 "\t\t\"SliceType\": 1,\n\t\t\"Unit\": {\"L\": \"mm\", \"T\": \"s\"},\n"
@@ -184,7 +194,7 @@ static int save_jnii(const char* session, const double* f, int64_t n, int32_t sr
                 "\t\t\"Name\": \"%s\",\n\t\t\"NIIFormat\": \"jnifti\"\n\t},\n",
             maxgate, output_description(outputtype), session);
     fprintf(fp, "\t\"NIFTIData\": {\n\t\t\"_ArrayType_\": \"double\",\n\t\t\"_ArraySize_\": [");
-    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%ld", i ? ", " : "", dims[i]);
+    for (int i = 0; i < ndim; i++) fprintf(fp, "%s%u", i ? ", " : "", sh->dims[i]);
     fprintf(fp, "],\n");
     if (compress)
         fprintf(fp, "\t\t\"_ArrayZipType_\": \"zlib\",\n\t\t\"_ArrayZipSize_\": %lld,\n\t\t\"_ArrayZipData_\": \"",
@@ -267,11 +277,11 @@ static void bj_typed(bj_buf* b, char type, const void* data, size_t count, size_
     bj_put(b, data, count * elem);
 }
 
-static int save_bnii(const char* session, const double* f, int64_t n, int32_t srcnum, int32_t maxgate,
-                     int32_t datalen, float tstep, int32_t outputtype, int compress) {
-    const int ndim = srcnum > 1 ? 3 : 2;
-    const uint32_t dims[3] = {(uint32_t)datalen, (uint32_t)maxgate, (uint32_t)srcnum};
-    const float vox[3] = {1.f, tstep, 1.f};
+static int save_bnii(const char* session, const double* f, int64_t n, const out_shape* sh, int32_t maxgate,
+                     int32_t outputtype, int compress) {
+    const int ndim = sh->ndim;
+    const uint32_t* dims = sh->dims;
+    const float* vox = sh->vox;
     const int32_t affine[3][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}};
     bj_buf b = {NULL, 0, 0, 0};
     bj_c(&b, '{');
@@ -374,15 +384,32 @@ static int save_bnii(const char* session, const double* f, int64_t n, int32_t sr
 }
 
 /* ------------------------------------------------------- dispatcher ----- */
-int32_t mmc_host_save_field(const char* session, int32_t format, const double* field, int32_t datalen,
-                            int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype, int32_t compress) {
-    if (!session || !field || datalen < 1 || maxgate < 1 || srcnum < 1) return -1;
+/* the .dat text of mesh_saveweight (src/mmc_mesh.c:1760-1797): "%d\t%e" per
+ * value, or "%d\t%d\t%e" (value, pattern) with several patterns */
+static int save_text(const char* session, const double* f, int32_t datalen, int32_t maxgate, int32_t srcnum) {
+    char fname[4096];
+    snprintf(fname, sizeof(fname), "%s.dat", session);
+    if (srcnum == 1) return mmc_host_save_dat(fname, f, datalen, maxgate);
+    FILE* fp = fopen(fname, "wt");
+    if (!fp) return -1;
+    int rc = 0;
+    for (int32_t g = 0; g < maxgate && !rc; g++)
+        for (int32_t j = 0; j < datalen && !rc; j++)
+            for (int32_t k = 0; k < srcnum && !rc; k++)
+                if (fprintf(fp, "%d\t%d\t%e\n", j + 1, k + 1, f[((size_t)g * datalen + j) * srcnum + k]) <= 0)
+                    rc = -2;
+    if (fclose(fp)) rc = -1;
+    return rc;
+}
+
+static int save_shaped(const char* session, int32_t format, const double* field, int32_t datalen, int32_t maxgate,
+                       int32_t srcnum, float tstep, int32_t outputtype, int32_t compress, const out_shape* sh,
+                       const int32_t tx3dim[3]) {
     const int64_t n = (int64_t)datalen * maxgate * srcnum;
     char fname[4096];
     switch (format) {
         case MMC_FMT_ASCII:
-            snprintf(fname, sizeof(fname), "%s.dat", session);
-            return mmc_host_save_dat(fname, field, datalen, maxgate * srcnum);
+            return save_text(session, field, datalen, maxgate, srcnum);
         case MMC_FMT_BIN:
         case MMC_FMT_MC2: {
             snprintf(fname, sizeof(fname), "%s.%s", session, format == MMC_FMT_BIN ? "bin" : "mc2");
@@ -390,26 +417,76 @@ int32_t mmc_host_save_field(const char* session, int32_t format, const double* f
         }
         case MMC_FMT_NII:
         case MMC_FMT_HDR:
-            return save_nifti(session, field, n, srcnum, maxgate, datalen, tstep, format == MMC_FMT_HDR);
+            return save_nifti(session, field, n, sh, tstep, format == MMC_FMT_HDR);
         case MMC_FMT_TX3: {
             snprintf(fname, sizeof(fname), "%s.tx3", session);
             FILE* fp = fopen(fname, "wb");
             if (!fp) return -1;
             const uint32_t glformat = 0x8814; /* GL_RGBA32F */
-            const int32_t dim[3] = {srcnum, maxgate, datalen};
             int rc = write_all(fp, &glformat, 4);
-            if (!rc) rc = write_all(fp, dim, sizeof(dim));
+            if (!rc) rc = write_all(fp, tx3dim, 3 * sizeof(int32_t));
             if (!rc) rc = write_all(fp, field, sizeof(double) * (size_t)n);
             if (fclose(fp)) rc = -1;
             return rc;
         }
         case MMC_FMT_JNII:
-            return save_jnii(session, field, n, srcnum, maxgate, datalen, tstep, outputtype, compress);
+            return save_jnii(session, field, n, sh, maxgate, outputtype, compress);
         case MMC_FMT_BNII:
-            return save_bnii(session, field, n, srcnum, maxgate, datalen, tstep, outputtype, compress);
+            return save_bnii(session, field, n, sh, maxgate, outputtype, compress);
         default:
             return -1;
     }
+}
+
+/* mesh data: the reference describes it as cfg->dim = (srcnum, maxgate,
+ * datalen) (src/mmc_mesh.c:1746-1756) and JNIfTI dims [datalen, maxgate,
+ * srcnum] (src/mmc_utils.c:941-956) */
+int32_t mmc_host_save_field(const char* session, int32_t format, const double* field, int32_t datalen,
+                            int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype, int32_t compress) {
+    if (!session || !field || datalen < 1 || maxgate < 1 || srcnum < 1) return -1;
+    out_shape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.ndim = 3;
+    sh.dims[0] = (uint32_t)datalen;
+    sh.dims[1] = (uint32_t)maxgate;
+    sh.dims[2] = (uint32_t)srcnum;
+    sh.vox[0] = 1.f;
+    sh.vox[1] = tstep;
+    sh.vox[2] = 1.f;
+    sh.nii[0] = srcnum;
+    sh.nii[1] = maxgate;
+    sh.nii[2] = datalen;
+    sh.nii[3] = 1;
+    sh.pix = 1.f;
+    const int32_t tx3dim[3] = {srcnum, maxgate, datalen};
+    return save_shaped(session, format, field, datalen, maxgate, srcnum, tstep, outputtype, compress, &sh, tx3dim);
+}
+
+/* dual-grid data: cfg->dim voxels of edge cfg->steps (src/mmc_utils.c:
+ * 515-545, 937-963) */
+int32_t mmc_host_save_grid(const char* session, int32_t format, const double* field, const int32_t dim[3],
+                           float step, int32_t maxgate, int32_t srcnum, float tstep, int32_t outputtype,
+                           int32_t compress) {
+    if (!session || !field || !dim || dim[0] < 1 || dim[1] < 1 || dim[2] < 1 || maxgate < 1 || srcnum < 1)
+        return -1;
+    const int64_t ncell = (int64_t)dim[0] * dim[1] * dim[2];
+    if (ncell >= (1ll << 31)) return -1;
+    out_shape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.ndim = 5;
+    for (int k = 0; k < 3; k++) {
+        sh.dims[k] = (uint32_t)dim[k];
+        sh.vox[k] = step;
+        sh.nii[k] = dim[k];
+    }
+    sh.dims[3] = (uint32_t)maxgate;
+    sh.dims[4] = (uint32_t)srcnum;
+    sh.vox[3] = tstep;
+    sh.vox[4] = 1.f;
+    sh.nii[3] = maxgate * srcnum;
+    sh.pix = step;
+    return save_shaped(session, format, field, (int32_t)ncell, maxgate, srcnum, tstep, outputtype, compress, &sh,
+                       dim);
 }
 
 int32_t mmc_host_format_id(const char* name) {

@@ -234,12 +234,53 @@ int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb) {
     return nf;
 }
 
+int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], int32_t dim[3]) {
+    if (!node || nn < 1 || !(step > 0.f) || !nmin || !dim) return -1;
+    for (int k = 0; k < 3; k++) {
+        float lo = 1e30f, hi = -1e30f;              /* VERY_BIG, src/mmc_mesh.h:74 */
+        for (int32_t i = 0; i < nn; i++) {
+            const float v = node[3 * (size_t)i + k];
+            lo = (v < lo) ? v : lo;
+            hi = (v > hi) ? v : hi;
+        }
+        lo -= 1e-6f;                                 /* EPS, src/mmc_mesh.h:70 */
+        hi += 1e-6f;
+        const float q = (hi - lo) / step;
+        if (!(q < 2.0e9f)) return -1;
+        nmin[k] = lo;
+        dim[k] = (int32_t)q + 1;
+    }
+    return 0;
+}
+
+double mmc_host_normalize_grid(const mmc_hip_sim* sim, int64_t ncells, double* field, double absorbweight,
+                               double launchweight, int32_t pair) {
+    (void)absorbweight;
+    float Etotal = (float)launchweight;
+    const int32_t S = (sim->srctype == MMC_SRC_PATTERN && sim->srcnum > 1) ? sim->srcnum : 1;
+    if (pair < 0 || pair >= S || ncells < 1) return 0.0;
+    const float u = sim->unitinmm > 0.f ? sim->unitinmm : 1.f;
+    double k;
+    if (sim->outputtype == MMC_OUT_JACOBIAN || sim->outputtype == MMC_OUT_WL || sim->outputtype == MMC_OUT_WP)
+        k = (sim->outputtype == MMC_OUT_JACOBIAN) ? 1.f / (1e-4f * (float)sim->nphoton) : 1.f / Etotal;
+    else if (sim->outputtype == MMC_OUT_ENERGY)
+        k = 1.f / Etotal;
+    else {
+        k = 1.0 / (Etotal * u * u * u);
+        if (sim->outputtype == MMC_OUT_FLUX) k /= sim->tstep;
+    }
+    const size_t n = (size_t)ncells * (size_t)sim->maxgate;
+    for (size_t i = 0; i < n; i++) field[i * S + pair] *= k;
+    return k;
+}
+
 double mmc_host_normalize_pair(const mmc_hip_sim* sim, int32_t nn, int32_t ne, const int32_t* elem,
                                const int32_t* type, const float* med, const float* evol,
                                const float* nvol, double* field, double absorbweight,
                                double launchweight, int32_t pair) {
     /* mesh_normalize takes the energies as float (src/mmc_mesh.c:2154) */
     float Eabsorb = (float)absorbweight, Etotal = (float)launchweight;
+    if (sim->method == MMC_RT_BLBADOUELGRID) return 0.0;   /* voxel fields: mmc_host_normalize_grid */
     const int32_t datalen = sim->basisorder ? nn : ne;
     const int32_t S = (sim->srctype == MMC_SRC_PATTERN && sim->srcnum > 1) ? sim->srcnum : 1;
     if (pair < 0 || pair >= S) return 0.0;

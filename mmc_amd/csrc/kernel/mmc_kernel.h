@@ -27,6 +27,8 @@ constexpr int kReflShift = 4, kSpecShift = 6;
  * (energy, weighted path, weighted scattering count) rather than fluence */
 constexpr int kReplayShift = 8;
 enum { kReplayNone = 0, kReplayJ = 1, kReplayWL = 2, kReplayWP = 3 };
+/* bit 10: record trajectories */
+constexpr int kFlagTraj = 1 << 10;
 struct SimHot {
     float tstart, tend, rtstep, minenergy;
     float roulettesize;
@@ -49,6 +51,15 @@ struct SimCold {
     int64_t nphoton, photon_first;
     uint64_t seed;
     uint64_t fieldoff;               /* float offset of this simulation's field */
+};
+
+/* Dual-grid output (DMMC) of one simulation: voxel (ix, iy, iz) of edge
+ * step at origin org holds value iz*dxy + iy*dx + ix of each gate. */
+struct SimGrid {
+    float4 org;                      /* grid origin; w = 1/step */
+    float step;
+    int32_t dx, dxy, ncells;         /* nx, nx*ny, nx*ny*nz */
+    int32_t nx, ny, nz, pad;
 };
 
 /* Accumulators per simulation (FP64, final reduction only), followed by the
@@ -90,7 +101,8 @@ struct KArgs {
     double* simacc;                  /* nsim*kAccStride */
     float* detdata;
     int64_t* detphoton;
-    unsigned long long* detcount;    /* nsim+1 counters: [0] global slot counter */
+    unsigned long long* detcount;    /* nsim+2 counters: [0] global slot counter, [1+i] detections of
+                                        simulation i, [nsim+1] trajectory records */
     int64_t maxdet;
     int32_t reclen, savedet;
     mmc_hip_photon* rec;
@@ -106,6 +118,9 @@ struct KArgs {
     const uint64_t* seeds;
     const float* rweight;
     const float* rtime;
+    const SimGrid* grid;             /* nsim, dual-grid tracer only */
+    float* traj;                     /* maxtraj trajectory records, or null */
+    uint64_t maxtraj;
 };
 
 /* Per-photon xorshift128+ state: splitmix64 finalizer over (seed, photon id).
@@ -123,11 +138,16 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
     if ((s0 | s1) == 0) s1 = 1;
 }
 
+/* Tracer and output of a kernel variant: branch-less Badouel with element
+ * output, Plucker with nodal output, branch-less Badouel with dual-grid
+ * (voxel) output. */
+enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2 };
+
 /* OCC: minimum resident workgroups of kBlock threads per CU the register
  * allocation must allow (4 = 128 VGPRs, the default; 5 trades registers for
  * a fifth workgroup).  STAGE: per-simulation tables staged in LDS (small batches) or
  * read through the caches (large batches, whose tables would crowd LDS). */
-template <bool NODAL, bool COMP, int OCC, bool STAGE>
+template <int TR, bool COMP, int OCC, bool STAGE>
 __global__ void transport(KArgs a);
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
                              uint64_t repstride);

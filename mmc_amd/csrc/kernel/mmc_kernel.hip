@@ -135,12 +135,15 @@ __device__ __forceinline__ uint32_t xcd_id() {
 }
 
 /*
- * The transport kernel.  NODAL selects the Plucker tracer with linear nodal
- * deposits (src/mmc_raytrace.c:227-508); otherwise the branch-less Badouel
- * tracer with element deposits (src/mmc_raytrace.c:1414-1709).
+ * The transport kernel.  TR selects the Plucker tracer with linear nodal
+ * deposits (src/mmc_raytrace.c:227-508), the branch-less Badouel tracer with
+ * element deposits (src/mmc_raytrace.c:1414-1709), or the same tracer with
+ * dual-grid voxel deposits (src/mmc_raytrace.c:1615-1673).
  */
-template <bool NODAL, bool COMP, int OCC, bool STAGE>
+template <int TR, bool COMP, int OCC, bool STAGE>
 __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
+    constexpr bool NODAL = TR == kTrNodal;
+    constexpr bool GRID = TR == kTrGrid;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nsim = a.nsim;
     const int nmed = a.prop + 1;
@@ -232,6 +235,22 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         if (na2 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.z - 1), na2);
         if (na3 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.w - 1), na3);
         npend = false;
+    };
+    /* one trajectory record: photon id, position, weight, element
+     * (savedebugdata, src/mmc_raytrace.c:1739-1756) */
+    auto savetraj = [&]() {
+        const unsigned long long k = atomicAdd(a.detcount + nsim + 1, 1ull);
+        if (k < a.maxtraj) {
+            const SimCold& C = a.cold[sim];
+            const uint32_t pid = (uint32_t)(C.photon_first + (gid - s_cum[sim]));
+            float* r = a.traj + (size_t)k * MMC_TRAJ_RECLEN;
+            r[0] = __uint_as_float(pid);
+            r[1] = px;
+            r[2] = py;
+            r[3] = pz;
+            r[4] = w;
+            r[5] = __int_as_float(eid);
+        }
     };
     auto flush_sim = [&](int s) {
         if (s < 0) return;
@@ -412,6 +431,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         kahan_lds(LS(0), LS(1), (rmode == kReplayWL || rmode == kReplayWP) ? rw : w);
                     }
                     if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */
+                    if ((STAGE ? s_hot[sim] : a.hot[sim]).flags & kFlagTraj) savetraj();
                 }
             }
         }
@@ -490,14 +510,16 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         undef = 1;
                         Lmove = (hot.tend - tof) / (nidx * R_C0F) - 1e-4f;
                     }
-                    w *= mmc_expf(-mua * Lmove);
+                    const float att = mmc_expf(-mua * Lmove);
+                    w *= att;
                     slen -= Lmove * mus;
                     const float cw = replay_weight(rmode, currweight, w, Lmove, mus, slen0, rw);
                     float ww = cw - w;
                     tof += Lmove * rc;
                     int g = (rmode == kReplayWL || rmode == kReplayWP) ? (int)(rt * hot.rtstep)
                                                                        : (int)((tof - hot.tstart) * hot.rtstep);
-                    int tshift = max(min(g, hot.maxgate - 1), 0) * a.ne;
+                    const int framelen = GRID ? a.grid[sim].ncells : a.ne;
+                    int tshift = max(min(g, hot.maxgate - 1), 0) * framelen;
                     if (mua > 0.f) {
                         float y = ww - eabs_c, tt = eabs + y;
                         eabs_c = (tt - eabs) - y;
@@ -505,27 +527,73 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     if (!(hot.flags & kFlagEnergy))
                         ww = (mua < EPSF) ? (cw * Lmove) : ((cw - w) / mua);
+                    const float q0x = px, q0y = py, q0z = pz;
                     px = px + vx * Lmove;
                     py = py + vy * Lmove;
                     pz = pz + vz * Lmove;
-                    uint32_t newidx = (uint32_t)(e + tshift);
                     const uint64_t foff = hot.fieldoff;
-                    if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
-                    if (newidx != oldidx) {
-                        /* the first step after a face crossing finds the
-                         * register already flushed (oldw == 0): the reference
-                         * adds that zero (src/mmc_raytrace.c:1562-1612); x + 0
-                         * == x for the non-negative field, so the atomic is
-                         * skipped, which halves the atomics */
-                        if (oldw != 0.f) depo(foff, oldidx, oldw);
-                        oldidx = newidx;
-                        oldw = ww;
+                    if (!GRID) {
+                        uint32_t newidx = (uint32_t)(e + tshift);
+                        if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
+                        if (newidx != oldidx) {
+                            /* the first step after a face crossing finds the
+                             * register already flushed (oldw == 0): the
+                             * reference adds that zero (src/mmc_raytrace.c:
+                             * 1562-1612); x + 0 == x for the non-negative
+                             * field, so the atomic is skipped, which halves
+                             * the atomics */
+                            if (oldw != 0.f) depo(foff, oldidx, oldw);
+                            oldidx = newidx;
+                            oldw = ww;
+                        } else {
+                            oldw += ww;
+                        }
+                        if (faceid == -2 || !isend) {
+                            depo(foff, newidx, oldw);
+                            oldw = 0.f;
+                        }
                     } else {
-                        oldw += ww;
-                    }
-                    if (faceid == -2 || !isend) {
-                        depo(foff, newidx, oldw);
-                        oldw = 0.f;
+                        /* dual grid (src/mmc_raytrace.c:1615-1673): the step
+                         * is cut into an even number of segments no longer
+                         * than half a voxel; segment i deposits the fraction
+                         * segloss^i (1 - segloss) / (1 - att) of the step's
+                         * deposit at the voxel holding its midpoint (clamped
+                         * to the grid), coalesced in the same register */
+                        const SimGrid G = a.grid[sim];
+                        const float tl = 1.f - att;
+                        int seg = (int)(Lmove / G.step) + 1;
+                        seg = seg << 1;
+                        const float dstep = Lmove / (float)seg;
+                        const float segloss = mmc_expf(-mua * dstep);
+                        const float tx = vx * dstep, ty = vy * dstep, tz = vz * dstep;
+                        float sx = (q0x - G.org.x) + tx * 0.5f;
+                        float sy = (q0y - G.org.y) + ty * 0.5f;
+                        float sz = (q0z - G.org.z) + tz * 0.5f;
+                        const float frac = (tl == 0.f) ? 0.f : (1.f - segloss) / tl;
+                        const bool flush = (faceid == -2 || !isend);
+                        float w0 = ww;
+                        for (int i = 0; i < seg; i++) {
+                            const int ix = min(max((int)(sx * G.org.w), 0), G.nx - 1);
+                            const int iy = min(max((int)(sy * G.org.w), 0), G.ny - 1);
+                            const int iz = min(max((int)(sz * G.org.w), 0), G.nz - 1);
+                            const uint32_t newidx = (uint32_t)(iz * G.dxy + iy * G.dx + ix + tshift);
+                            if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
+                            if (newidx != oldidx) {
+                                if (oldw != 0.f) depo(foff, oldidx, oldw);
+                                oldidx = newidx;
+                                oldw = w0 * frac;
+                            } else {
+                                oldw += w0 * frac;
+                            }
+                            if (flush) {
+                                if (oldw != 0.f) depo(foff, newidx, oldw);
+                                oldw = 0.f;
+                            }
+                            w0 *= segloss;
+                            sx = sx + tx;
+                            sy = sy + ty;
+                            sz = sz + tz;
+                        }
                     }
                 }
                 if (faceid == -2) slen = 0.f;
@@ -825,6 +893,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     vx = qx; vy = qy; vz = qz;
                     slen = nextslen;
                     slen0 = nextslen;
+                    if (hot.flags & kFlagTraj) savetraj();
                     if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
                     *LS(slot_ns) += 1.f;
                     ctx = CTX_OUTER;
@@ -838,6 +907,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         /* ============================================ photon ends here ===== */
         {
             if (NODAL && npend) flush_nodal();
+            if (hot.flags & kFlagTraj) savetraj();
             const SimCold& C = a.cold[sim];
             const float w0 = *LS(slot_w0);
             int exitdet = 0;
@@ -924,11 +994,13 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-#define MMC_INSTANTIATE(OCC, STAGE)                                          \
-    template __global__ void transport<false, false, OCC, STAGE>(KArgs);     \
-    template __global__ void transport<false, true, OCC, STAGE>(KArgs);      \
-    template __global__ void transport<true, false, OCC, STAGE>(KArgs);      \
-    template __global__ void transport<true, true, OCC, STAGE>(KArgs);
+#define MMC_INSTANTIATE(OCC, STAGE)                                           \
+    template __global__ void transport<kTrElem, false, OCC, STAGE>(KArgs);    \
+    template __global__ void transport<kTrElem, true, OCC, STAGE>(KArgs);     \
+    template __global__ void transport<kTrNodal, false, OCC, STAGE>(KArgs);   \
+    template __global__ void transport<kTrNodal, true, OCC, STAGE>(KArgs);    \
+    template __global__ void transport<kTrGrid, false, OCC, STAGE>(KArgs);    \
+    template __global__ void transport<kTrGrid, true, OCC, STAGE>(KArgs);
 MMC_INSTANTIATE(4, true)
 MMC_INSTANTIATE(5, true)
 MMC_INSTANTIATE(4, false)

@@ -58,6 +58,7 @@ struct mmc_hip_meshres {
     float4 loc_min, loc_inv;
     int4 loc_dim;
     std::vector<float> med;       /* (prop+1)*4 default media */
+    float gmin[3], gmax[3];       /* node bounding box widened by EPS (dual grid) */
     int64_t bytes = 0;
 };
 
@@ -214,6 +215,18 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
     m->prop = mesh->prop;
     m->nf = mesh->nf;
     m->med.assign(mesh->med, mesh->med + 4 * (mesh->prop + 1));
+    /* the dual grid's extent (mesh_createdualmesh, src/mmc_mesh.c:349-372):
+     * the node bounding box widened by EPS; every grid starts at its corner */
+    for (int k = 0; k < 3; k++) {
+        float lo = 1e30f, hi = -1e30f;
+        for (size_t i = 0; i < nn; i++) {
+            const float v = mesh->node[3 * i + k];
+            lo = (v < lo) ? v : lo;
+            hi = (v > hi) ? v : hi;
+        }
+        m->gmin[k] = lo - 1e-6f;
+        m->gmax[k] = hi + 1e-6f;
+    }
 
     std::vector<int4> fnb(ne);
     std::vector<float4> nd(nn), en(3 * ne);
@@ -283,9 +296,50 @@ extern "C" int64_t mmc_hip_mesh_bytes(mmc_hip_mesh_handle h) { return h ? h->byt
 /* field slots per value: the patterns that share each photon */
 static int slots_of(const mmc_hip_sim* s) { return s->srctype == MMC_SRC_PATTERN ? s->srcnum : 1; }
 
-/* values per gate (nodes or elements) times the pattern slots */
+/* dual grid of a simulation: mesh_createdualmesh (src/mmc_mesh.c:349-381)
+ * over the mesh's widened box with the simulation's voxel edge */
+static SimGrid grid_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
+    SimGrid g;
+    memset(&g, 0, sizeof(g));
+    const float step = s->gridstep;
+    int dim[3];
+    for (int k = 0; k < 3; k++) {
+        float ext = m->gmax[k] - m->gmin[k];
+        float q = ext / step;
+        dim[k] = (q >= 0.f && q < 2.0e9f) ? (int)q + 1 : 0;
+    }
+    g.org = make_float4(m->gmin[0], m->gmin[1], m->gmin[2], 1.f / step);
+    g.step = step;
+    g.nx = dim[0];
+    g.ny = dim[1];
+    g.nz = dim[2];
+    const int64_t dxy = (int64_t)dim[0] * dim[1], nc = dxy * dim[2];
+    g.dx = dim[0];
+    g.dxy = (dxy < (1ll << 31)) ? (int32_t)dxy : 0;
+    g.ncells = (nc < (1ll << 31)) ? (int32_t)nc : 0;
+    return g;
+}
+
+/* values per gate (nodes, elements or voxels) times the pattern slots */
 static int64_t datalen_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
+    if (s->method == MMC_RT_BLBADOUELGRID) return (int64_t)grid_of(m, s).ncells * slots_of(s);
     return (int64_t)(s->basisorder ? m->nn : m->ne) * slots_of(s);
+}
+
+extern "C" int32_t mmc_hip_sim_grid(mmc_hip_mesh_handle h, const mmc_hip_sim* s, float nmin[3], int32_t dim[3]) {
+    if (!h || !s) return set_err(-1, "null argument");
+    if (s->method != MMC_RT_BLBADOUELGRID || !(s->gridstep > 0.f))
+        return set_err(-1, "not a dual-grid simulation (method %d, grid step %g)", s->method, (double)s->gridstep);
+    SimGrid g = grid_of(h, s);
+    if (!g.ncells) return set_err(-1, "grid step %g gives more than 2^31 voxels", (double)s->gridstep);
+    if (nmin)
+        for (int k = 0; k < 3; k++) nmin[k] = h->gmin[k];
+    if (dim) {
+        dim[0] = g.nx;
+        dim[1] = g.ny;
+        dim[2] = g.nz;
+    }
+    return 0;
 }
 
 extern "C" int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* s) {
@@ -298,11 +352,19 @@ extern "C" int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip
  * dereference is validated here. */
 static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0) {
     if (s->nphoton < 0) return set_err(-1, "negative photon count");
-    if (s->method != MMC_RT_PLUCKER && s->method != MMC_RT_BLBADOUEL)
-        return set_err(-1, "unsupported ray-tracer %d (use Plucker or branch-less Badouel)", s->method);
+    if (s->method != MMC_RT_PLUCKER && s->method != MMC_RT_BLBADOUEL && s->method != MMC_RT_BLBADOUELGRID)
+        return set_err(-1, "unsupported ray-tracer %d (use Plucker, branch-less Badouel or the dual grid)", s->method);
     if (s->method != method0) return set_err(-1, "all simulations of a batch must use one ray-tracer");
-    if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1))
+    if (s->method == MMC_RT_BLBADOUELGRID) {
+        /* the dual grid always deposits per voxel (src/mmc_utils.c:3554-3556) */
+        if (!(s->gridstep > 0.f)) return set_err(-1, "dual grid needs a positive voxel edge (gridstep)");
+        const SimGrid g = grid_of(m, s);
+        if (!g.ncells || (uint64_t)g.ncells * (uint64_t)s->maxgate >= (1ull << 32))
+            return set_err(-1, "dual grid of %g-unit voxels too large (%d x %d x %d voxels, %d gates)",
+                           (double)s->gridstep, g.nx, g.ny, g.nz, s->maxgate);
+    } else if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1)) {
         return set_err(-1, "supported pairs: Plucker with nodal output, branch-less Badouel with element output");
+    }
     if (s->srctype != MMC_SRC_PENCIL && s->srctype != MMC_SRC_PLANAR && s->srctype != MMC_SRC_PATTERN)
         return set_err(-1, "unsupported source type %d", s->srctype);
     if (s->e0 < 1 || s->e0 > m->ne) return set_err(-1, "initial element %d outside 1..ne", s->e0);
@@ -389,7 +451,7 @@ struct mmc_hip_slotres {
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     Pooled d_hot, d_cold, d_med, d_cum, d_det, d_pat, d_acc, d_detcnt, d_queue, d_detdata, d_detph, d_rec;
-    Pooled d_seeds, d_rw, d_rt;
+    Pooled d_seeds, d_rw, d_rt, d_grid, d_traj;
     /* launch in flight */
     bool busy = false;
     mmc_hip_mesh_handle m = nullptr;
@@ -397,6 +459,8 @@ struct mmc_hip_slotres {
     int nsim = 0, nrep = 1, reclen = 0, savedet = 0;
     bool comp = false, has_rec = false;
     int64_t maxdet = 0, recfirst = 0, reccount = 0;
+    int trajres = -1;                 /* result that receives the trajectory records */
+    int64_t maxtraj = 0;
     uint64_t fieldlen = 0;
     std::vector<int64_t> cum;
     std::vector<uint64_t> fieldoff;
@@ -410,6 +474,7 @@ struct mmc_hip_slotres {
     std::vector<float> h_pats;
     std::vector<uint64_t> h_seeds;
     std::vector<float> h_rw, h_rt;
+    std::vector<SimGrid> h_grid;
     std::chrono::steady_clock::time_point t_start;
     ~mmc_hip_slotres() {
         if (st) {
@@ -450,7 +515,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         int rc = check_sim(m, sims + i, sims[0].method);
         if (rc) return set_err(rc, "simulation %d: %s", i, std::string(g_err).c_str());
     }
-    const bool nodal = sims[0].method == MMC_RT_PLUCKER;
+    const int tracer = sims[0].method == MMC_RT_PLUCKER ? kTrNodal
+                       : sims[0].method == MMC_RT_BLBADOUELGRID ? kTrGrid : kTrElem;
     const int nmed = m->prop + 1;
     const int block = (tune && tune->block > 0) ? tune->block : kBlock;
     if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
@@ -466,6 +532,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     std::vector<uint64_t>& seeds = sl->h_seeds;
     std::vector<float>& rws = sl->h_rw;
     std::vector<float>& rts = sl->h_rt;
+    std::vector<SimGrid>& grids = sl->h_grid;
+    grids.assign(tracer == kTrGrid ? nsim : 0, SimGrid());
     seeds.clear();
     rws.clear();
     rts.clear();
@@ -501,7 +569,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                   (rmode << kReplayShift) |
                   ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0) |
                   (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
-                  ((s.isspecular & 3) << kSpecShift);
+                  ((s.isspecular & 3) << kSpecShift) | ((s.debuglevel & MMC_DL_TRAJ) ? kFlagTraj : 0);
         h.nout = s.nout;
         h.srcnum = slots_of(&s);
         maxsrc = std::max(maxsrc, h.srcnum);
@@ -559,6 +627,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                 rts.push_back(s.replaytime ? s.replaytime[s.photon_offset + k] : 0.f);
             }
         }
+        if (tracer == kTrGrid) grids[i] = grid_of(m, &s);
         const float* md = s.med ? s.med : m->med.data();
         for (int k = 0; k < nmed; k++)
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
@@ -587,13 +656,14 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         (rc = sl->d_med.ensure(sizeof(float4) * media.size())) || (rc = sl->d_cum.ensure(sizeof(int64_t) * cum.size())) ||
         (rc = sl->d_det.ensure(sizeof(float4) * dets.size())) || (rc = sl->d_pat.ensure(sizeof(float) * pats.size())) ||
         (rc = sl->d_acc.ensure(sizeof(double) * nsim * kAccStride)) ||
-        (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 1))) ||
+        (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 2))) ||
         (rc = sl->d_queue.ensure(sizeof(unsigned long long))))
         return rc;
     if (!seeds.empty() &&
         ((rc = sl->d_seeds.ensure(sizeof(uint64_t) * seeds.size())) || (rc = sl->d_rw.ensure(sizeof(float) * rws.size())) ||
          (rc = sl->d_rt.ensure(sizeof(float) * rts.size()))))
         return rc;
+    if (!grids.empty() && (rc = sl->d_grid.ensure(sizeof(SimGrid) * grids.size()))) return rc;
     if (!arena) {
         if ((rc = dalloc((float**)&sl->field.p, (size_t)fieldlen * nrep, nullptr))) return rc;
         if (comp && (rc = dalloc((float**)&sl->lo.p, (size_t)fieldlen * nrep, nullptr))) return rc;
@@ -612,6 +682,17 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             break;
         }
     if (reccount > 0 && (rc = sl->d_rec.ensure(sizeof(mmc_hip_photon) * (size_t)reccount))) return rc;
+    /* trajectory records: one buffer per launch, returned through the first
+     * result that provides one */
+    int trajres = -1;
+    int64_t maxtraj = 0;
+    for (int i = 0; i < nsim; i++)
+        if ((sims[i].debuglevel & MMC_DL_TRAJ) && res[i].traj && res[i].maxtraj > 0) {
+            trajres = i;
+            maxtraj = res[i].maxtraj;
+            break;
+        }
+    if (maxtraj > 0 && (rc = sl->d_traj.ensure(sizeof(float) * MMC_TRAJ_RECLEN * (size_t)maxtraj))) return rc;
 
     hipStream_t st = sl->st;
     HIPCHK(hipMemcpyAsync(sl->d_hot.p, hot.data(), sizeof(SimHot) * nsim, hipMemcpyHostToDevice, st));
@@ -622,6 +703,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         HIPCHK(hipMemcpyAsync(sl->d_det.p, dets.data(), sizeof(float4) * dets.size(), hipMemcpyHostToDevice, st));
     if (!pats.empty())
         HIPCHK(hipMemcpyAsync(sl->d_pat.p, pats.data(), sizeof(float) * pats.size(), hipMemcpyHostToDevice, st));
+    if (!grids.empty())
+        HIPCHK(hipMemcpyAsync(sl->d_grid.p, grids.data(), sizeof(SimGrid) * grids.size(), hipMemcpyHostToDevice, st));
     if (!seeds.empty()) {
         HIPCHK(hipMemcpyAsync(sl->d_seeds.p, seeds.data(), sizeof(uint64_t) * seeds.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(sl->d_rw.p, rws.data(), sizeof(float) * rws.size(), hipMemcpyHostToDevice, st));
@@ -633,8 +716,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     }
     HIPCHK(hipMemsetAsync(sl->d_acc.p, 0, sizeof(double) * nsim * kAccStride, st));
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)sl->d_queue.p, 1);
-    hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 64) / 64), dim3(64), 0, st,
-                       (unsigned long long*)sl->d_detcnt.p, nsim + 1);
+    hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 65) / 64), dim3(64), 0, st,
+                       (unsigned long long*)sl->d_detcnt.p, nsim + 2);
 
     KArgs a;
     memset(&a, 0, sizeof(a));
@@ -683,6 +766,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.seeds = seeds.empty() ? nullptr : (const uint64_t*)sl->d_seeds.p;
     a.rweight = seeds.empty() ? nullptr : (const float*)sl->d_rw.p;
     a.rtime = seeds.empty() ? nullptr : (const float*)sl->d_rt.p;
+    a.grid = grids.empty() ? nullptr : (const SimGrid*)sl->d_grid.p;
+    a.traj = maxtraj > 0 ? (float*)sl->d_traj.p : nullptr;
+    a.maxtraj = (uint64_t)maxtraj;
 
     const int nslots = kLaneSlots + 2 * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that
@@ -699,9 +785,13 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
      * caches (STAGE=false) */
     const int want = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : 0;
     const void* kfn = nullptr;
-#define MMC_PICK(O, S)                                                                               \
-    kfn = nodal ? (comp ? (const void*)transport<true, true, O, S> : (const void*)transport<true, false, O, S>) \
-                : (comp ? (const void*)transport<false, true, O, S> : (const void*)transport<false, false, O, S>);
+#define MMC_PICK(O, S)                                                                                     \
+    kfn = tracer == kTrNodal  ? (comp ? (const void*)transport<kTrNodal, true, O, S>                           \
+                                      : (const void*)transport<kTrNodal, false, O, S>)                         \
+          : tracer == kTrGrid ? (comp ? (const void*)transport<kTrGrid, true, O, S>                            \
+                                      : (const void*)transport<kTrGrid, false, O, S>)                          \
+                              : (comp ? (const void*)transport<kTrElem, true, O, S>                            \
+                                      : (const void*)transport<kTrElem, false, O, S>);
     if (!stage) {
         MMC_PICK(4, false)
     } else if (want >= 5) {
@@ -737,6 +827,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     sl->maxdet = maxdet;
     sl->recfirst = recfirst;
     sl->reccount = reccount;
+    sl->trajres = trajres;
+    sl->maxtraj = maxtraj;
     sl->fieldlen = fieldlen;
     sl->fieldoff = fieldoff;
     sl->datalen.resize(nsim);
@@ -773,7 +865,7 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
     const int nsim = sl->nsim;
     hipStream_t st = sl->st;
     std::vector<double> acc((size_t)nsim * kAccStride);
-    std::vector<unsigned long long> cnt(nsim + 1);
+    std::vector<unsigned long long> cnt(nsim + 2);
     HIPCHK(hipMemcpyAsync(acc.data(), sl->d_acc.p, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(cnt.data(), sl->d_detcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -851,6 +943,16 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
             if (r.detphoton) r.detphoton[fill[s]] = sl->photon_offset[s] + (dp[k] - sl->cum[s]);
             fill[s]++;
         }
+    }
+    if (sl->trajres >= 0) {
+        mmc_hip_result& r = res[sl->trajres];
+        const int64_t made = (int64_t)cnt[nsim + 1];
+        const int64_t keep = std::min<int64_t>(made, sl->maxtraj);
+        if (keep > 0)
+            HIPCHK(hipMemcpyAsync(r.traj, sl->d_traj.p, sizeof(float) * MMC_TRAJ_RECLEN * (size_t)keep,
+                                  hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        r.ntraj += made;
     }
     if (sl->reccount > 0) {
         for (int i = 0; i < nsim; i++)

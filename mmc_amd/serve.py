@@ -101,16 +101,20 @@ class Scheduler:
     """Admission and chunk planning (host logic, testable without a GPU)."""
 
     def __init__(self, arena_floats: int, datalen: Dict[str, int], slice_photons: int = 2_000_000,
-                 min_chunk: int = 100_000, nodelen: Optional[Dict[str, int]] = None):
+                 min_chunk: int = 100_000, nodelen: Optional[Dict[str, int]] = None,
+                 meshes: Optional[Dict[str, TetMesh]] = None):
         self.arena = FieldArena(arena_floats)
         self.datalen = datalen          # elements per mesh (element fields)
         self.nodelen = nodelen or {}    # nodes per mesh (nodal fields)
+        self.meshes = meshes or {}      # meshes, for the voxel counts of dual-grid requests
         self.slice_photons = slice_photons
         self.min_chunk = min_chunk
         self.waiting: deque = deque()
         self.active: List[Request] = []
 
     def rows(self, r: Request) -> int:
+        if r.cfg.is_grid:
+            return r.cfg.datalen(self.meshes[r.mesh])
         return self.nodelen[r.mesh] if r.cfg.basisorder else self.datalen[r.mesh]
 
     def field_len(self, r: Request) -> int:
@@ -174,7 +178,7 @@ class Server:
                    "mmc_hip_arena_create")
         self.arena = h
         self.sched = Scheduler(nfloat, {k: m.ne for k, m in meshes.items()}, slice_photons,
-                               nodelen={k: m.nn for k, m in meshes.items()})
+                               nodelen={k: m.nn for k, m in meshes.items()}, meshes=meshes)
         self.tune = tune if tune is not None else tune_struct()
         self.launches = 0
         self.kernel_ms = 0.0

@@ -18,10 +18,12 @@ from . import _lib
 from ._lib import PHOTON_DTYPE, ResultC, SimC, TuneC, f32p, f64p, i64p, ptr
 from .mesh import TetMesh
 
-METHODS = {"P": 0, "S": 3, "plucker": 0, "blbadouel": 3}
+METHODS = {"P": 0, "S": 3, "G": 4, "plucker": 0, "blbadouel": 3, "grid": 4}
 OUTPUTS = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5, "flux": 0, "fluence": 1, "energy": 2,
            "jacobian": 3, "wl": 4, "wp": 5}
 SOURCES = {"pencil": 0, "planar": 4, "pattern": 5}
+# one trajectory record (MMC_TRAJ_RECLEN floats)
+TRAJ_DTYPE = np.dtype([("id", "<u4"), ("p", "<f4", 3), ("w", "<f4"), ("eid", "<i4")])
 
 
 @dataclasses.dataclass
@@ -29,7 +31,8 @@ class SimConfig:
     nphoton: int = 1000
     seed: int = 0x623F9A9E
     photon_offset: int = 0
-    method: str = "S"              # 'P' Plucker (nodal), 'S' branch-less Badouel (element)
+    method: str = "S"              # 'P' Plucker (nodal), 'S' branch-less Badouel (element),
+                                   # 'G' dual grid (branch-less Badouel, voxel output)
     basisorder: Optional[int] = None
     outputtype: str = "x"
     isreflect: int = 1
@@ -59,10 +62,35 @@ class SimConfig:
     photonseed: Optional[np.ndarray] = None
     replayweight: Optional[np.ndarray] = None
     replaytime: Optional[np.ndarray] = None
+    gridstep: float = 1.0                  # dual-grid voxel edge (cfg->steps.x), mesh units
+    unitinmm: float = 1.0                  # mesh length unit (normalization of grid fields)
+    trajectory: bool = False               # record launch/scattering/end positions ("-D M")
+    maxjumpdebug: int = 10_000_000         # trajectory records kept
 
     def __post_init__(self):
-        if self.basisorder is None:
+        if self.basisorder is None or METHODS[self.method] == 4:
             self.basisorder = 1 if METHODS[self.method] == 0 else 0
+
+    @property
+    def is_grid(self) -> bool:
+        return METHODS[self.method] == 4
+
+    def grid(self, mesh: TetMesh):
+        """(nmin, dim) of the dual grid over this mesh (mesh_createdualmesh)."""
+        nmin = np.zeros(3, np.float32)
+        dim = np.zeros(3, np.int32)
+        node = np.ascontiguousarray(mesh.node, np.float32)
+        rc = _lib.lib().mmc_host_grid(ptr(node, C.c_float), int(mesh.nn), C.c_float(self.gridstep),
+                                      ptr(nmin, C.c_float), ptr(dim, C.c_int32))
+        if rc != 0:
+            raise ValueError(f"bad dual-grid voxel edge {self.gridstep}")
+        return nmin, dim
+
+    def datalen(self, mesh: TetMesh) -> int:
+        """Values per gate: voxels (dual grid), nodes or elements."""
+        if self.is_grid:
+            return int(np.prod(self.grid(mesh)[1].astype(np.int64)))
+        return mesh.nn if self.basisorder else mesh.ne
 
     @property
     def srcnum(self) -> int:
@@ -147,6 +175,9 @@ class SimConfig:
                 setattr(s, name, ptr(a, C.c_float))
             else:
                 setattr(s, name, C.cast(None, f32p))
+        s.gridstep = float(self.gridstep)
+        s.unitinmm = float(self.unitinmm)
+        s.debuglevel = 8192 if self.trajectory else 0
         s._keep = keep
         return s
 
@@ -180,12 +211,14 @@ class SimResult:
     total_ms: float
     normalizer: float = 1.0
     srcweight: Optional[np.ndarray] = None    # (2, srcnum): launched, absorbed energy per pattern
+    traj: Optional[np.ndarray] = None         # TRAJ_DTYPE records (photon id, position, weight, element)
+    ntraj: int = 0                            # records produced (more than kept if the buffer filled)
 
 
 class _ResultBuf:
     def __init__(self, mesh: TetMesh, cfg: SimConfig, want_field=True, records=None):
         maxgate = cfg.gates()[0]
-        datalen = mesh.nn if cfg.basisorder else mesh.ne
+        datalen = cfg.datalen(mesh)
         S = cfg.srcnum
         shape = (maxgate, datalen, S) if S > 1 else (maxgate, datalen)
         self.field = np.zeros(shape, np.float64) if want_field else None
@@ -197,7 +230,11 @@ class _ResultBuf:
         self.detphoton = np.zeros(max(nd, 1), np.int64) if ndet else None
         self.photons = None
         self.rec = records
+        self.traj = np.zeros(int(cfg.maxjumpdebug), TRAJ_DTYPE) if cfg.trajectory else None
         r = ResultC()
+        if self.traj is not None:
+            r.traj = self.traj.ctypes.data_as(f32p)
+            r.maxtraj = len(self.traj)
         r.field = ptr(self.field, C.c_double) if want_field else C.cast(None, f64p)
         r.detdata = ptr(self.detdata, C.c_float) if ndet else C.cast(None, f32p)
         r.detphoton = ptr(self.detphoton, C.c_int64) if ndet else C.cast(None, i64p)
@@ -219,7 +256,9 @@ class _ResultBuf:
             detected=int(r.detected),
             detdata=self.detdata[:n] if self.detdata is not None else None,
             detphoton=self.detphoton[:n] if self.detphoton is not None else None,
-            photons=self.photons, kernel_ms=r.kernel_ms, total_ms=r.total_ms, srcweight=self.srcweight)
+            photons=self.photons, kernel_ms=r.kernel_ms, total_ms=r.total_ms, srcweight=self.srcweight,
+            traj=self.traj[:min(int(r.ntraj), len(self.traj))] if self.traj is not None else None,
+            ntraj=int(r.ntraj))
 
 
 def tune_struct(block=0, blocks_per_cu=0, compensated=0, chunk=1, replicas=0) -> TuneC:
@@ -250,6 +289,14 @@ def normalize_field(mesh: TetMesh, cfg: SimConfig, res: SimResult):
     normalized with its own energies and the factors are returned as an array."""
     L = _lib.lib()
     sc = cfg.cstruct(mesh)
+    if cfg.is_grid:
+        ncells = cfg.datalen(mesh)
+        if cfg.srcnum == 1:
+            return L.mmc_host_normalize_grid(C.byref(sc), ncells, ptr(res.field, C.c_double), res.absorbweight,
+                                             res.launchweight, 0)
+        return np.array([L.mmc_host_normalize_grid(C.byref(sc), ncells, ptr(res.field, C.c_double),
+                                                   res.srcweight[1, k], res.srcweight[0, k], k)
+                         for k in range(cfg.srcnum)])
     args = (C.byref(sc), mesh.nn, mesh.ne, ptr(mesh.elem, C.c_int32), ptr(mesh.etype, C.c_int32),
             ptr(mesh.med, C.c_float), ptr(mesh.evol, C.c_float), ptr(mesh.nvol, C.c_float),
             ptr(res.field, C.c_double))

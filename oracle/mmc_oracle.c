@@ -334,6 +334,25 @@ void orc_surface_nvol(int nn, int ne, const int* elem, const int* type, const in
         if (nvol[i] < 0.f) nvol[i] = -nvol[i];
 }
 
+/* mesh_createdualmesh, src/mmc_mesh.c:349-381: the node bounding box
+ * widened by EPS, and cfg->dim voxels of edge step covering it */
+void orc_grid(int nn, const float* node, float step, float nmin[3], int dim[3]) {
+    float lo[3] = {VERY_BIG, VERY_BIG, VERY_BIG}, hi[3] = {-VERY_BIG, -VERY_BIG, -VERY_BIG};
+    for (int i = 0; i < nn; i++)
+        for (int k = 0; k < 3; k++) {
+            float v = node[3 * (size_t)i + k];
+            lo[k] = (v < lo[k]) ? v : lo[k];
+            hi[k] = (v > hi[k]) ? v : hi[k];
+        }
+    for (int k = 0; k < 3; k++) {
+        lo[k] -= EPS;
+        hi[k] += EPS;
+        float ext = hi[k] - lo[k];
+        dim[k] = (int)(ext / step) + 1;
+        nmin[k] = lo[k];
+    }
+}
+
 /* tracer_prep exterior numbering, src/mmc_mesh.c:1466-1474 */
 int orc_number_exterior(int ne, int* facenb) {
     int nf = 0;
@@ -470,8 +489,10 @@ static float orc_badouel(oray* r, octx* c) {
         }
         float totalloss = O_EXPF(-mua * r->Lmove);
         r->weight *= totalloss;
+        totalloss = 1.f - totalloss;
         r->slen -= r->Lmove * mus;
         currweight = replay_cw(c, currweight, r, mus);
+        const float q0x = r->p0.x, q0y = r->p0.y, q0z = r->p0.z;
 
         {
             int framelen = c->framelen;
@@ -487,7 +508,47 @@ static float orc_badouel(oray* r, octx* c) {
             r->p0.y = r->p0.y + r->vec.y * r->Lmove;
             r->p0.z = r->p0.z + r->vec.z * r->Lmove;
 
-            if (!cfg->basisorder) {
+            if (cfg->method == ORC_RT_BLBADOUELGRID) {
+                /* dual-grid deposit, src/mmc_raytrace.c:1615-1673: the step
+                 * is cut into an even number of segments no longer than half
+                 * a voxel; each deposits its share of the step's loss at the
+                 * voxel holding its midpoint (voxel index clamped to the grid) */
+                int seg = (int)(r->Lmove / cfg->gridstep) + 1;
+                seg = (seg << 1);
+                float dstep = r->Lmove / seg;
+                float segloss = O_EXPF(-mua * dstep);
+                float tx = r->vec.x * dstep, ty = r->vec.y * dstep, tz = r->vec.z * dstep;
+                float sx = (q0x - cfg->gridmin[0]) + tx * 0.5f;
+                float sy = (q0y - cfg->gridmin[1]) + ty * 0.5f;
+                float sz = (q0z - cfg->gridmin[2]) + tz * 0.5f;
+                float inv = 1.f / cfg->gridstep;
+                float frac = (totalloss == 0.f) ? 0.f : (1.f - segloss) / totalloss;
+                float w0 = ww;
+                const int dx = cfg->griddim[0], dxy = cfg->griddim[0] * cfg->griddim[1];
+                for (int i = 0; i < seg; i++) {
+                    int ix = (int)(sx * inv), iy = (int)(sy * inv), iz = (int)(sz * inv);
+                    ix = ix < 0 ? 0 : (ix >= cfg->griddim[0] ? cfg->griddim[0] - 1 : ix);
+                    iy = iy < 0 ? 0 : (iy >= cfg->griddim[1] ? cfg->griddim[1] - 1 : iy);
+                    iz = iz < 0 ? 0 : (iz >= cfg->griddim[2] ? cfg->griddim[2] - 1 : iz);
+                    unsigned int newidx = (unsigned int)(iz * dxy + iy * dx + ix + tshift);
+                    r->oldidx = (r->oldidx == 0xFFFFFFFFu) ? newidx : r->oldidx;
+                    if (newidx != r->oldidx) {
+                        add_w(c->weight, r->oldidx, r->oldweight);
+                        r->oldidx = newidx;
+                        r->oldweight = w0 * frac;
+                    } else {
+                        r->oldweight += w0 * frac;
+                    }
+                    if (r->faceid == -2 || !r->isend) {
+                        add_w(c->weight, newidx, r->oldweight);
+                        r->oldweight = 0.f;
+                    }
+                    w0 *= segloss;
+                    sx = sx + tx;
+                    sy = sy + ty;
+                    sz = sz + tz;
+                }
+            } else if (!cfg->basisorder) {
                 unsigned int newidx = (unsigned int)(eid + tshift);
                 r->oldidx = (r->oldidx == 0xFFFFFFFFu) ? newidx : r->oldidx;
                 if (newidx != r->oldidx) {
@@ -1017,7 +1078,9 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
     base.m = mesh;
     base.weight = out->weight;
     base.rtstep = 1.f / cfg->tstep;
-    base.framelen = cfg->basisorder ? mesh->nn : mesh->ne;
+    base.framelen = (cfg->method == ORC_RT_BLBADOUELGRID)
+                        ? cfg->griddim[0] * cfg->griddim[1] * cfg->griddim[2]
+                        : (cfg->basisorder ? mesh->nn : mesh->ne);
     base.raytet = 0.0;
     int reclen = (2 + (cfg->ismomentum > 0)) * mesh->prop + (cfg->issaveexit > 0) * 6 + 2;
     odet det = {out->detdata, out->detphoton, 0, out->maxdet, reclen};
@@ -1085,7 +1148,9 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
 /* mesh_normalize, src/mmc_mesh.c:2154-2279 (single source, no RF). */
 double orc_normalize(const orc_cfg* cfg, const orc_mesh* mesh, double* weight, float Eabsorb,
                      float Etotal) {
-    int datalen = cfg->basisorder ? mesh->nn : mesh->ne;
+    const int grid = cfg->method == ORC_RT_BLBADOUELGRID;
+    int datalen = grid ? cfg->griddim[0] * cfg->griddim[1] * cfg->griddim[2]
+                       : (cfg->basisorder ? mesh->nn : mesh->ne);
     double energydeposit = 0.f, normalizor;
     if (cfg->photonseed && (cfg->outputtype == ORC_OT_JACOBIAN || cfg->outputtype == ORC_OT_WL ||
                             cfg->outputtype == ORC_OT_WP)) {
@@ -1099,7 +1164,10 @@ double orc_normalize(const orc_cfg* cfg, const orc_mesh* mesh, double* weight, f
         for (size_t i = 0; i < (size_t)cfg->maxgate * datalen; i++) weight[i] *= normalizor;
         return normalizor;
     }
-    if (cfg->basisorder) {
+    if (grid) {
+        /* voxel fluence per unit launched energy (src/mmc_mesh.c:2205-2206) */
+        normalizor = 1.0 / (Etotal * cfg->unitinmm * cfg->unitinmm * cfg->unitinmm);
+    } else if (cfg->basisorder) {
         for (int i = 0; i < cfg->maxgate; i++)
             for (int j = 0; j < datalen; j++)
                 if (mesh->nvol[j] > 0.f) weight[(size_t)i * datalen + j] /= mesh->nvol[j];

@@ -23,6 +23,7 @@ extern "C" {
 /* method ids = enum TRTMethod (src/mmc_utils.h:97) */
 #define ORC_RT_PLUCKER   0
 #define ORC_RT_BLBADOUEL 3
+#define ORC_RT_BLBADOUELGRID 4      /* dual-grid MMC (DMMC): voxel output */
 /* source types = enum TSrcType (src/mmc_utils.h:101) */
 #define ORC_ST_PENCIL  0
 #define ORC_ST_PLANAR  4
@@ -85,6 +86,11 @@ typedef struct {
     const float* replaytime;          /* replay L/P: detected time of flight per photon, or NULL */
     int nthreads;                     /* <=0: OpenMP default */
     int64_t recfirst, reccount;       /* photon-record window [recfirst, recfirst+reccount) */
+    /* dual grid (ORC_RT_BLBADOUELGRID): voxel edge cfg->steps.x, grid origin
+     * mesh->nmin and cfg->dim from orc_grid; unitinmm for the normalization */
+    float gridstep, unitinmm;
+    float gridmin[3];
+    int griddim[3];
 } orc_cfg;
 
 typedef struct {
@@ -123,6 +129,7 @@ double orc_getreff(double n_in, double n_out);
 void orc_surface_nvol(int nn, int ne, const int* elem, const int* type, const int* facenb_raw,
                       const float* med, int prop, int isreflect, float* nvol);
 int  orc_number_exterior(int ne, int* facenb);
+void orc_grid(int nn, const float* node, float step, float nmin[3], int dim[3]);
 
 /* transport + normalization */
 int    orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out);

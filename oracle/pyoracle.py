@@ -42,7 +42,9 @@ class OCfg(C.Structure):
                 ("nout", C.c_float), ("minenergy", C.c_float), ("roulettesize", C.c_float),
                 ("detnum", C.c_int), ("detpos", f32p), ("seed", C.c_uint32), ("photonseed", u64p),
                 ("replayweight", f32p), ("replaytime", f32p),
-                ("nthreads", C.c_int), ("recfirst", C.c_int64), ("reccount", C.c_int64)]
+                ("nthreads", C.c_int), ("recfirst", C.c_int64), ("reccount", C.c_int64),
+                ("gridstep", C.c_float), ("unitinmm", C.c_float), ("gridmin", C.c_float * 3),
+                ("griddim", C.c_int * 3)]
 
 
 class OOut(C.Structure):
@@ -156,8 +158,12 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     maxgate, t0, t1, dt = sim.gates()
     c = OCfg()
     c.nphoton = int(sim.nphoton)
-    c.method = {"P": 0, "S": 3}[sim.method]
-    c.basisorder = int(sim.basisorder)
+    c.method = {"P": 0, "S": 3, "G": 4}[sim.method]
+    c.basisorder = 0 if sim.method == "G" else int(sim.basisorder)
+    c.unitinmm = float(getattr(sim, "unitinmm", 1.0) or 1.0)
+    if sim.method == "G":
+        c.gridstep = float(sim.gridstep)
+        c.gridmin[:], c.griddim[:] = grid(mesh, c.gridstep)
     c.outputtype = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5}[sim.outputtype]
     c.isreflect, c.isspecular, c.voidtime = int(sim.isreflect), int(sim.isspecular), int(sim.voidtime)
     c.issavedet = int(bool(sim.issavedet and sim.detpos is not None and len(sim.detpos)))
@@ -219,6 +225,15 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     return c
 
 
+def grid(mesh: OracleMesh, step: float):
+    """Grid origin and dimensions of the dual-grid output (orc_grid)."""
+    nmin = np.zeros(3, np.float32)
+    dim = np.zeros(3, np.int32)
+    lib().orc_grid(int(mesh.nn), p(mesh.node, C.c_float), C.c_float(step), p(nmin, C.c_float),
+                   p(dim, C.c_int32))
+    return [float(x) for x in nmin], [int(x) for x in dim]
+
+
 class OracleResult:
     pass
 
@@ -227,7 +242,10 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
         math="glibc"):
     L = lib(math)
     cfg = make_cfg(mesh, sim, photonseed, nthreads, records)
-    datalen = mesh.nn if cfg.basisorder else mesh.ne
+    if cfg.method == 4:
+        datalen = int(np.prod(list(cfg.griddim)))
+    else:
+        datalen = mesh.nn if cfg.basisorder else mesh.ne
     field = np.zeros((cfg.maxgate, datalen), np.float64)
     out = OOut()
     out.weight = p(field, C.c_double)

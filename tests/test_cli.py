@@ -1,5 +1,6 @@
 """The command-line front end (mmc_amd/bin/mmc): the reference's JSON input
 and flags, and its .dat / .bin / .mch outputs."""
+import json
 import os
 import struct
 import subprocess
@@ -45,6 +46,43 @@ def read_mch(path):
     return head, data
 
 
+def test_legacy_inp_input_and_json_dump(tmp_path):
+    # mcx_loadconfig's line format (src/mmc_utils.c:2425-2670), checked
+    # against the values written in tests/golden/onecube/onecube.inp, then
+    # written back by --dumpjson (mcx_savejdata) and re-read unchanged
+    r = run(["-f", os.path.join(GOLDEN, "onecube", "onecube.inp"), "--dumpjson", "dump.json"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    doc = json.load(open(tmp_path / "dump.json"))
+    assert doc["Session"]["Photons"] == 100 and doc["Session"]["RNGSeed"] == 17182818
+    src = doc["Optode"]["Source"]
+    assert src["Type"] == "pencil" and src["Pos"] == [2, 8, 0] and src["Dir"][:3] == [0, 0, 1]
+    f = doc["Forward"]
+    assert f["T0"] == 0 and f["T1"] == pytest.approx(5e-9) and f["Dt"] == pytest.approx(5e-10)
+    assert doc["Mesh"]["MeshID"] == "onecube" and doc["Mesh"]["InitElem"] == 3
+    assert [d["Pos"] + [d["R"]] for d in doc["Optode"]["Detector"]] == [[2, 6, 0, 1], [2, 4, 0, 1], [2, 2, 0, 1]]
+    node = np.loadtxt(os.path.join(GOLDEN, "onecube", "node_onecube.dat"), skiprows=1)[:, 1:]
+    elem = np.loadtxt(os.path.join(GOLDEN, "onecube", "elem_onecube.dat"), skiprows=1, dtype=int)[:, 1:]
+    assert np.array_equal(np.array(doc["Mesh"]["MeshNode"]), node)
+    assert np.array_equal(np.array(doc["Mesh"]["MeshElem"]), elem)
+    med = doc["Domain"]["Media"][1]
+    assert [med["mua"], med["mus"], med["g"], med["n"]] == pytest.approx([0.005, 1.010101, 0.01, 1.0], rel=1e-5)
+    # the dump is a complete input: reading it back gives the same dump
+    r = run(["-f", str(tmp_path / "dump.json"), "--dumpjson", "again.json"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert open(tmp_path / "again.json").read() == open(tmp_path / "dump.json").read()
+
+
+@pytest.mark.gpu
+def test_inp_and_its_json_dump_give_the_same_run(tmp_path):
+    inp = os.path.join(GOLDEN, "onecube", "onecube.inp")
+    assert run(["-f", inp, "--dumpjson", "d.json"], tmp_path).returncode == 0
+    for name, src in (("a", inp), ("b", str(tmp_path / "d.json"))):
+        r = run(["-f", src, "-n", "5000", "-M", "S", "-C", "0", "-F", "bin", "-s", name], tmp_path)
+        assert r.returncode == 0, r.stderr
+    a, b = np.fromfile(tmp_path / "a.bin"), np.fromfile(tmp_path / "b.bin")
+    assert a.sum() > 0 and np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
+
+
 @pytest.mark.gpu
 def test_cube2_json_element_output_and_detectors(tmp_path):
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-n", "20000", "-M", "S", "-C", "0", "-d", "1",
@@ -60,6 +98,25 @@ def test_cube2_json_element_output_and_detectors(tmp_path):
     assert rec.shape[0] == head[7] > 0
     assert set(np.unique(rec[:, 0]).astype(int)) <= {1, 2, 3, 4}
     assert np.all(rec[:, -1] == 1.0)                  # launch weight
+
+
+@pytest.mark.gpu
+def test_cube2_default_is_the_dual_grid(tmp_path):
+    # the GPU builds default to -M G (src/mmc_utils.c:241-243): 60 mm cube,
+    # 2 mm voxels -> int(60.000002 / 2) + 1 = 31 per axis, 50 gates
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-n", "20000", "--gridsize", "2", "-F", "nii",
+             "-s", "g2"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "dual grid: 31 x 31 x 31" in r.stdout
+    raw = open(tmp_path / "g2.nii", "rb").read()
+    assert struct.unpack_from("<8h", raw, 40)[:5] == (4, 31, 31, 31, 50)
+    assert struct.unpack_from("<3f", raw, 80) == (2.0, 2.0, 2.0)
+    f = np.frombuffer(raw[352:], np.float64).reshape(50, 31, 31, 31)
+    assert np.all(f >= 0) and f[0].sum() > 0
+    # the source at (30, 30, 0) points +z: the first gate's fluence peaks in
+    # the voxel column under it, near the surface
+    z, y, x = np.unravel_index(np.argmax(f[0]), f[0].shape)
+    assert (x, y) in {(14, 14), (14, 15), (15, 14), (15, 15)} and z <= 2
 
 
 @pytest.mark.gpu
@@ -94,7 +151,7 @@ def test_every_output_format_holds_the_same_field(tmp_path):
             fields[fmt] = np.frombuffer(p.read_bytes()[16:], np.float64)
         else:
             d = json.load(open(p))["NIFTIData"]
-            assert d["_ArraySize_"] == [8, 10]
+            assert d["_ArraySize_"] == [8, 10, 1]
             fields[fmt] = np.frombuffer(zlib.decompress(base64.b64decode(d["_ArrayZipData_"])), np.float64)
     ref = fields["bin"]
     assert ref.shape == (80,) and ref.sum() > 0

@@ -98,16 +98,18 @@ def test_tx3(tmp_path, field):
     assert np.array_equal(np.frombuffer(raw[16:], np.float64), field)
 
 
-def check_jnifti(doc, field, compressed):
+def check_jnifti(doc, field, compressed, dims=(DATALEN, MAXGATE, 1), vox=(1.0, TSTEP, 1.0)):
+    # mesh data: Dim [datalen, maxgate, srcnum] (lastdim 3 of mcx_savedata,
+    # src/mmc_utils.c:941-956); grid data: [nx, ny, nz, maxgate, srcnum]
     hdr, dat = doc["NIFTIHeader"], doc["NIFTIData"]
-    assert list(hdr["Dim"]) == [DATALEN, MAXGATE]
+    assert list(hdr["Dim"]) == list(dims)
     assert hdr["DataType"] == "double" and hdr["BitDepth"] == 64
     assert hdr["LastSliceID"] == MAXGATE
-    assert np.allclose(hdr["VoxelSize"], [1.0, TSTEP], rtol=1e-6)
+    assert np.allclose(hdr["VoxelSize"], vox, rtol=1e-6)
     assert hdr["Description"] == "MMC volumetric output: Fluence (J/mm^2)"
-    assert dat["_ArrayType_"] == "double" and list(dat["_ArraySize_"]) == [DATALEN, MAXGATE]
+    assert dat["_ArrayType_"] == "double" and list(dat["_ArraySize_"]) == list(dims)
     if compressed:
-        assert dat["_ArrayZipType_"] == "zlib" and dat["_ArrayZipSize_"] == DATALEN * MAXGATE
+        assert dat["_ArrayZipType_"] == "zlib" and dat["_ArrayZipSize_"] == int(np.prod(dims))
         payload = zlib.decompress(dat["_ArrayZipData_"])
     else:
         payload = dat["_ArrayData_"]
@@ -193,3 +195,62 @@ def test_bnii(tmp_path, field, compress):
     if not compress:
         doc["NIFTIData"]["_ArrayData_"] = doc["NIFTIData"]["_ArrayData_"].tobytes()
     check_jnifti(doc, field, compress)
+
+
+# ---- dual-grid data: cfg->dim voxels of edge cfg->steps (mcx_savedata with
+# the grid's dims, src/mmc_utils.c:515-545, 937-963)
+GRID, STEP = (4, 3, 5), 0.5
+
+
+def save_grid(tmp_path, fmt, f, srcnum=1, compress=1):
+    L = _lib.lib()
+    sess = str(tmp_path / "grid")
+    dim = np.array(GRID, np.int32)
+    rc = L.mmc_host_save_grid(sess.encode(), L.mmc_host_format_id(fmt.encode()), _lib.ptr(f, C.c_double),
+                              _lib.ptr(dim, C.c_int32), C.c_float(STEP), MAXGATE, srcnum, C.c_float(TSTEP), 1,
+                              compress)
+    assert rc == 0
+    return sess
+
+
+@pytest.fixture(scope="module")
+def gfield():
+    return np.random.default_rng(4).random(int(np.prod(GRID)) * MAXGATE)
+
+
+def test_grid_nifti_is_a_volume_series(tmp_path, gfield):
+    sess = save_grid(tmp_path, "nii", gfield)
+    raw = open(sess + ".nii", "rb").read()
+    h = nifti_header(raw)
+    assert h["dim"][:5] == (4,) + GRID + (MAXGATE,)
+    assert h["pixdim"][1:4] == (STEP, STEP, STEP)
+    assert np.array_equal(np.frombuffer(raw[352:], np.float64), gfield)
+
+
+def test_grid_tx3_and_jnifti_dims(tmp_path, gfield):
+    sess = save_grid(tmp_path, "tx3", gfield)
+    raw = open(sess + ".tx3", "rb").read()
+    assert struct.unpack_from("<I3i", raw, 0) == (0x8814,) + GRID
+    sess = save_grid(tmp_path, "jnii", gfield)
+    doc = json.load(open(sess + ".jnii"))
+    doc["NIFTIData"]["_ArrayZipData_"] = base64.b64decode(doc["NIFTIData"]["_ArrayZipData_"])
+    check_jnifti(doc, gfield, 1, dims=GRID + (MAXGATE, 1), vox=(STEP, STEP, STEP, TSTEP, 1.0))
+    sess = save_grid(tmp_path, "bnii", gfield, compress=0)
+    doc = bjdata_load(open(sess + ".bnii", "rb").read())
+    doc["NIFTIData"]["_ArrayData_"] = doc["NIFTIData"]["_ArrayData_"].tobytes()
+    check_jnifti(doc, gfield, 0, dims=GRID + (MAXGATE, 1), vox=(STEP, STEP, STEP, TSTEP, 1.0))
+
+
+def test_text_output_of_shared_patterns_names_the_pattern(tmp_path):
+    # mesh_saveweight writes "value<TAB>pattern<TAB>%e" rows when srcnum > 1
+    # (src/mmc_mesh.c:1782-1792), the field laid out [gate][value][pattern]
+    L = _lib.lib()
+    f = np.arange(DATALEN * MAXGATE * 2, dtype=np.float64) + 0.5
+    sess = str(tmp_path / "pat")
+    assert L.mmc_host_save_field(sess.encode(), 0, _lib.ptr(f, C.c_double), DATALEN, MAXGATE, 2, TSTEP, 1, 1) == 0
+    rows = [l.split("\t") for l in open(sess + ".dat").read().splitlines()]
+    assert len(rows) == DATALEN * MAXGATE * 2
+    j = np.tile(np.repeat(np.arange(1, DATALEN + 1), 2), MAXGATE)
+    k = np.tile([1, 2], DATALEN * MAXGATE)
+    assert np.array_equal([int(r[0]) for r in rows], j) and np.array_equal([int(r[1]) for r in rows], k)
+    assert np.allclose([float(r[2]) for r in rows], f, rtol=1e-6)
