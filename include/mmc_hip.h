@@ -77,6 +77,12 @@ typedef struct mmc_hip_mesh {
     const float* med;        /* (prop+1)*4 {mua,mus,g,n}; row 0 = background (mesh->med) */
     const int32_t* srcelem;  /* candidate launch elements for area sources (mesh->srcelem) */
     int32_t srcelemlen;
+    /* 1: elements labelled prop+1 are wide-field detector elements (mesh label -2,
+     * mesh_srcdetelem, src/mmc_mesh.c:390-427); they take the background's optical
+     * properties, photons leaving the tissue keep travelling through the background,
+     * and one that leaves the mesh through a detector element is detected with that
+     * element's id as detector id (point detectors are then ignored) */
+    int32_t isextdet;
 } mmc_hip_mesh;
 
 /* One simulation: the mcconfig fields (src/mmc_utils.h:210-345) the

@@ -55,6 +55,10 @@ double mmc_host_reff(double n_in, double n_out);
 void mmc_host_surface_nvol(int32_t nn, int32_t ne, const int32_t* elem, const int32_t* type,
                            const int32_t* facenb, const float* med, int32_t prop,
                            int32_t isreflect, float* nvol);
+/* Relabels wide-field detector elements (label -2) as prop+1, as
+ * mesh_srcdetelem + mesh_loadmedia do (src/mmc_mesh.c:390-427, 511-527);
+ * returns their count (mmc_hip_mesh.isextdet = count > 0). */
+int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop);
 /* Numbers exterior faces -1,-2,... in element-major order; returns the count. */
 int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb);
 /* Dual-grid extent (mesh_createdualmesh, src/mmc_mesh.c:349-381): the

@@ -27,6 +27,7 @@ class MeshC(C.Structure):
         ("nn", C.c_int32), ("ne", C.c_int32), ("prop", C.c_int32), ("nf", C.c_int32),
         ("node", f32p), ("elem", i32p), ("type", i32p), ("facenb", i32p),
         ("planes", f32p), ("med", f32p), ("srcelem", i32p), ("srcelemlen", C.c_int32),
+        ("isextdet", C.c_int32),
     ]
 
 
@@ -118,6 +119,7 @@ SIGNATURES = {
     "mmc_host_surface_nvol": (None, [C.c_int32, C.c_int32, i32p, i32p, i32p, f32p, C.c_int32,
                                      C.c_int32, f32p]),
     "mmc_host_number_exterior": (C.c_int32, [C.c_int32, i32p]),
+    "mmc_host_detelem": (C.c_int32, [C.c_int32, i32p, C.c_int32]),
     "mmc_host_grid": (C.c_int32, [f32p, C.c_int32, C.c_float, f32p, i32p]),
     "mmc_host_normalize_grid": (C.c_double, [C.POINTER(SimC), C.c_int64, f64p, C.c_double, C.c_double,
                                              C.c_int32]),

@@ -644,8 +644,6 @@ int main(int argc, char** argv) {
     }
     c.sim.gridstep = c.gridstep;
     c.sim.unitinmm = c.unitinmm;
-    if (c.sim.issavedet && c.sim.detnum == 0) c.sim.issavedet = 0;
-    if (!c.sim.issavedet) c.sim.issaveexit = 0;
     c.sim.detpos = c.detpos;
 
     if (!c.node) load_mesh_files(&c);
@@ -673,8 +671,20 @@ int main(int argc, char** argv) {
     for (int i = 0; i < c.ne; i++) {
         for (int k = 0; k < 4; k++) elem[4 * i + k] = c.elem5[5 * i + k];
         type[i] = c.elem5[5 * i + 4];
-        if (type[i] < 0 || type[i] > prop) die("%s", "element labels outside the media list");
     }
+    /* wide-field detector elements (label -2) become label prop+1 and
+     * replace the point detectors (mesh_srcdetelem, src/mmc_mesh.c:390-427) */
+    const int isextdet = mmc_host_detelem(c.ne, type, prop) > 0;
+    if (isextdet) {
+        /* the detector label's medium is a copy of the background (src/mmc_mesh.c:518-520) */
+        c.sim.detnum = 0;
+        c.med = (float*)realloc(c.med, sizeof(float) * 4 * (size_t)(c.nmed + 1));
+        memcpy(c.med + 4 * c.nmed, c.med, 4 * sizeof(float));
+    }
+    if (c.sim.issavedet && c.sim.detnum == 0 && !isextdet) c.sim.issavedet = 0;
+    if (!c.sim.issavedet) c.sim.issaveexit = 0;
+    for (int i = 0; i < c.ne; i++)
+        if (type[i] < 0 || type[i] > prop + isextdet) die("%s", "element labels outside the media list");
     float* evol = (float*)malloc(sizeof(float) * (size_t)c.ne);
     float* nvol = (float*)malloc(sizeof(float) * (size_t)c.nn);
     int32_t* facenb = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)c.ne);
@@ -692,7 +702,7 @@ int main(int argc, char** argv) {
             if (c.sim.e0 <= 0) die("%s", "initial element does not enclose the source!");
         }
     }
-    mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med, NULL, 0};
+    mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med, NULL, 0, isextdet};
     int datalen = c.sim.basisorder ? c.nn : c.ne;
     float gridmin[3] = {0.f, 0.f, 0.f};
     int32_t griddim[3] = {0, 0, 0};

@@ -234,6 +234,16 @@ int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb) {
     return nf;
 }
 
+int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop) {
+    int32_t n = 0;
+    for (int32_t i = 0; i < ne; i++)
+        if (type[i] == -2) {
+            type[i] = prop + 1;
+            n++;
+        }
+    return n;
+}
+
 int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], int32_t dim[3]) {
     if (!node || nn < 1 || !(step > 0.f) || !nmin || !dim) return -1;
     for (int k = 0; k < 3; k++) {

@@ -85,6 +85,7 @@ struct KArgs {
     float4 loc_min, loc_inv;
     int4 loc_dim;
     int32_t nn, ne, prop, nsim;
+    int32_t isextdet, pad_ext;       /* wide-field detector elements labelled prop+1 */
     /* simulations */
     const SimHot* hot;               /* nsim */
     const SimCold* cold;             /* nsim */
@@ -142,12 +143,16 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
  * output, Plucker with nodal output, branch-less Badouel with dual-grid
  * (voxel) output. */
 enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2 };
+/* where a launch keeps its per-simulation tables: read through the caches
+ * or staged in LDS */
+enum { kTabGlobal = 0, kTabLds = 1 };
 
 /* OCC: minimum resident workgroups of kBlock threads per CU the register
  * allocation must allow (4 = 128 VGPRs, the default; 5 trades registers for
- * a fifth workgroup).  STAGE: per-simulation tables staged in LDS (small batches) or
- * read through the caches (large batches, whose tables would crowd LDS). */
-template <int TR, bool COMP, int OCC, bool STAGE>
+ * a fifth workgroup).  TAB: per-simulation tables staged in LDS (small
+ * batches) or read through the caches (large batches, whose tables would
+ * crowd LDS). */
+template <int TR, bool COMP, int OCC, int TAB>
 __global__ void transport(KArgs a);
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
                              uint64_t repstride);

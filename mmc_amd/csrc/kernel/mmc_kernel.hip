@@ -140,13 +140,18 @@ __device__ __forceinline__ uint32_t xcd_id() {
  * element deposits (src/mmc_raytrace.c:1414-1709), or the same tracer with
  * dual-grid voxel deposits (src/mmc_raytrace.c:1615-1673).
  */
-template <int TR, bool COMP, int OCC, bool STAGE>
+template <int TR, bool COMP, int OCC, int TAB>
 __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     constexpr bool NODAL = TR == kTrNodal;
     constexpr bool GRID = TR == kTrGrid;
+    /* per-simulation tables: in LDS (small batches) or through the caches
+     * (large batches).  (Reading a single simulation's tables as wave-uniform
+     * values was measured 16 % slower on the slab: the scalar register file
+     * overflows into lanes of vector registers.) */
+    constexpr bool STAGE = TAB == kTabLds;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nsim = a.nsim;
-    const int nmed = a.prop + 1;
+    const int nmed = a.prop + 1 + a.isextdet;
     /* per-simulation tables: staged in LDS for batches whose tables are
      * small, read through L1/L2 otherwise (lanes of a wave mostly share a
      * simulation, so those reads broadcast) */
@@ -214,6 +219,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     float rw = 1.f, rt = 0.f;           /* replay: detected weight and time of the photon */
     float slen0 = 0.f;                  /* scattering length at the last scattering event */
     int posidx = 0;                     /* launch pixel of a pattern source */
+    /* the lane's simulation index into the tables, and its hot parameters */
+    auto sidx = [&]() -> int { return sim; };
+    auto hot_of = [&]() -> SimHot { return STAGE ? s_hot[sim] : a.hot[sim]; };
 
     /* deposit v at value loc of the simulation's field (offset foff); with
      * shared patterns the value has nsrc slots and slot k receives v times
@@ -241,8 +249,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     auto savetraj = [&]() {
         const unsigned long long k = atomicAdd(a.detcount + nsim + 1, 1ull);
         if (k < a.maxtraj) {
-            const SimCold& C = a.cold[sim];
-            const uint32_t pid = (uint32_t)(C.photon_first + (gid - s_cum[sim]));
+            const SimCold& C = a.cold[sidx()];
+            const uint32_t pid = (uint32_t)(C.photon_first + (gid - s_cum[sidx()]));
             float* r = a.traj + (size_t)k * MMC_TRAJ_RECLEN;
             r[0] = __uint_as_float(pid);
             r[1] = px;
@@ -310,11 +318,11 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     gid = g;
                     alive = true;
-                    const SimCold& C = a.cold[sim];
-                    uint64_t pid = (uint64_t)(C.photon_first + (g - s_cum[sim]));
+                    const SimCold& C = a.cold[sidx()];
+                    uint64_t pid = (uint64_t)(C.photon_first + (g - s_cum[sidx()]));
                     if (C.replaybase >= 0) {
                         /* replay: the saved initial state of this photon */
-                        const int64_t k = C.replaybase + (g - s_cum[sim]);
+                        const int64_t k = C.replaybase + (g - s_cum[sidx()]);
                         r0 = a.seeds[2 * k];
                         r1 = a.seeds[2 * k + 1];
                         rw = a.rweight ? a.rweight[k] : 1.f;
@@ -427,20 +435,20 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     /* replay with weighted path / scattering outputs counts the
                      * detected weight as launched energy (src/mmc_raytrace.c:1815) */
                     {
-                        const int rmode = (a.hot[sim].flags >> kReplayShift) & 3;
+                        const int rmode = (hot_of().flags >> kReplayShift) & 3;
                         kahan_lds(LS(0), LS(1), (rmode == kReplayWL || rmode == kReplayWP) ? rw : w);
                     }
                     if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */
-                    if ((STAGE ? s_hot[sim] : a.hot[sim]).flags & kFlagTraj) savetraj();
+                    if (hot_of().flags & kFlagTraj) savetraj();
                 }
             }
         }
         if (__ballot(alive) == 0) break;
         if (!alive) continue;
 
-        const SimHot hot = STAGE ? s_hot[sim] : a.hot[sim];
+        const SimHot hot = hot_of();
         const int rmode = (hot.flags >> kReplayShift) & 3;
-        const size_t mbase = (size_t)sim * nmed;
+        const size_t mbase = (size_t)sidx() * nmed;
         auto med = [&](int k) -> float4 { return STAGE ? s_med[mbase + k] : a.media[mbase + k]; };
         int reason = 0;
         int faceid = -1, isend = 0, undef = 1, nexteid = 0, faceidx = 4;
@@ -518,7 +526,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     tof += Lmove * rc;
                     int g = (rmode == kReplayWL || rmode == kReplayWP) ? (int)(rt * hot.rtstep)
                                                                        : (int)((tof - hot.tstart) * hot.rtstep);
-                    const int framelen = GRID ? a.grid[sim].ncells : a.ne;
+                    const int framelen = GRID ? a.grid[sidx()].ncells : a.ne;
                     int tshift = max(min(g, hot.maxgate - 1), 0) * framelen;
                     if (mua > 0.f) {
                         float y = ww - eabs_c, tt = eabs + y;
@@ -559,7 +567,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                          * segloss^i (1 - segloss) / (1 - att) of the step's
                          * deposit at the voxel holding its midpoint (clamped
                          * to the grid), coalesced in the same register */
-                        const SimGrid G = a.grid[sim];
+                        const SimGrid G = a.grid[sidx()];
                         const float tl = 1.f - att;
                         int seg = (int)(Lmove / G.step) + 1;
                         seg = seg << 1;
@@ -751,10 +759,13 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         faceid = -1;
                     }
                 }
-                if (!next_iter && savedet && Lmove > 0.f && faceid >= 0 && ctype > 0)
+                /* (paths in wide-field detector elements, label prop+1, are
+                 * not partial paths of a tissue) */
+                if (!next_iter && savedet && Lmove > 0.f && faceid >= 0 && ctype > 0 && ctype <= a.prop)
                     *LS(ppbase + a.prop - 1 + ctype) += Lmove;
             } else {
-                if (savedet && Lmove > 0.f && ctype > 0) *LS(ppbase + a.prop - 1 + ctype) += Lmove;
+                if (savedet && Lmove > 0.f && ctype > 0 && ctype <= a.prop)
+                    *LS(ppbase + a.prop - 1 + ctype) += Lmove;
                 if (ctx == CTX_INNER) fixcount = 0;
                 if (undef) {
                     if (fixcount++ < MAX_TRIAL) {
@@ -835,7 +846,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 eid = neid;
                 if (eid > 0) {
                     if (oldtype == 0 && ntype != 0 && !(hot.flags & kFlagVoidTime)) tof = 0.f;
-                    if (oldtype != 0 && ntype == 0) eid = 0;
+                    /* leaving the tissue into the background ends the photon
+                     * unless wide-field detectors wait outside */
+                    if (oldtype != 0 && ntype == 0 && !a.isextdet) eid = 0;
                 }
                 if (eid > 0) {
                     ctype = ntype;
@@ -908,11 +921,17 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
         {
             if (NODAL && npend) flush_nodal();
             if (hot.flags & kFlagTraj) savetraj();
-            const SimCold& C = a.cold[sim];
+            const SimCold& C = a.cold[sidx()];
             const float w0 = *LS(slot_w0);
             int exitdet = 0;
+            if (reason == MMC_END_EXIT && a.savedet && C.issavedet && eid <= 0 && a.isextdet &&
+                C.detnum == 0 && ctype == a.prop + 1) {
+                /* left the mesh through a wide-field detector element: its
+                 * id is the detector id (src/mmc_raytrace.c:2013-2014) */
+                exitdet = cache_e + 1;
+            }
             if (reason == MMC_END_EXIT && a.savedet && C.issavedet && eid <= 0) {
-                for (int i = 0; i < C.detnum; i++) {
+                for (int i = 0; i < C.detnum && !exitdet; i++) {
                     float4 d = a.detpos[C.detbase + i];
                     if ((d.x - px) * (d.x - px) + (d.y - py) * (d.y - py) + (d.z - pz) * (d.z - pz) <
                         d.w * d.w) {
@@ -994,15 +1013,15 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-#define MMC_INSTANTIATE(OCC, STAGE)                                           \
-    template __global__ void transport<kTrElem, false, OCC, STAGE>(KArgs);    \
-    template __global__ void transport<kTrElem, true, OCC, STAGE>(KArgs);     \
-    template __global__ void transport<kTrNodal, false, OCC, STAGE>(KArgs);   \
-    template __global__ void transport<kTrNodal, true, OCC, STAGE>(KArgs);    \
-    template __global__ void transport<kTrGrid, false, OCC, STAGE>(KArgs);    \
-    template __global__ void transport<kTrGrid, true, OCC, STAGE>(KArgs);
-MMC_INSTANTIATE(4, true)
-MMC_INSTANTIATE(5, true)
-MMC_INSTANTIATE(4, false)
+#define MMC_INSTANTIATE(OCC, TAB)                                           \
+    template __global__ void transport<kTrElem, false, OCC, TAB>(KArgs);    \
+    template __global__ void transport<kTrElem, true, OCC, TAB>(KArgs);     \
+    template __global__ void transport<kTrNodal, false, OCC, TAB>(KArgs);   \
+    template __global__ void transport<kTrNodal, true, OCC, TAB>(KArgs);    \
+    template __global__ void transport<kTrGrid, false, OCC, TAB>(KArgs);    \
+    template __global__ void transport<kTrGrid, true, OCC, TAB>(KArgs);
+MMC_INSTANTIATE(4, kTabLds)
+MMC_INSTANTIATE(5, kTabLds)
+MMC_INSTANTIATE(4, kTabGlobal)
 
 }  // namespace mmc

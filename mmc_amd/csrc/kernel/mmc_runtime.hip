@@ -45,6 +45,7 @@ static int set_err(int code, const char* fmt, ...) {
 
 struct mmc_hip_meshres {
     int nn, ne, prop, nf;
+    int isextdet = 0;             /* elements labelled prop+1 are wide-field detectors */
     float4* planes = nullptr;
     int4* facenb = nullptr;
     int4* elem = nullptr;
@@ -193,8 +194,9 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
     if (mesh->ne <= 0 || mesh->nn <= 0 || !mesh->node || !mesh->elem || !mesh->type ||
         !mesh->facenb || !mesh->planes || !mesh->med)
         return set_err(-1, "incomplete mesh");
-    if (mesh->prop < 1 || mesh->prop > kMaxProp)
-        return set_err(-1, "media count %d outside 1..%d", mesh->prop, kMaxProp);
+    if (mesh->isextdet != 0 && mesh->isextdet != 1) return set_err(-1, "isextdet must be 0 or 1");
+    if (mesh->prop < 1 || mesh->prop + mesh->isextdet > kMaxProp)
+        return set_err(-1, "media count %d (+%d detector label) outside 1..%d", mesh->prop, mesh->isextdet, kMaxProp);
     if ((int64_t)mesh->ne >= (1ll << 27) || (int64_t)mesh->nf >= (1ll << 27))
         return set_err(-1, "mesh too large for packed face neighbours (ne=%d)", mesh->ne);
     const size_t ne = (size_t)mesh->ne, nn = (size_t)mesh->nn;
@@ -206,14 +208,15 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
             return set_err(-1, "facenb[%zu]=%d outside -nf..ne", i, mesh->facenb[i]);
     }
     for (size_t i = 0; i < ne; i++)
-        if (mesh->type[i] < 0 || mesh->type[i] > mesh->prop)
-            return set_err(-1, "type[%zu]=%d outside 0..prop", i, mesh->type[i]);
+        if (mesh->type[i] < 0 || mesh->type[i] > mesh->prop + mesh->isextdet)
+            return set_err(-1, "type[%zu]=%d outside 0..prop%s", i, mesh->type[i], mesh->isextdet ? "+1" : "");
 
     mmc_hip_meshres* m = new mmc_hip_meshres();
     m->nn = mesh->nn;
     m->ne = mesh->ne;
     m->prop = mesh->prop;
     m->nf = mesh->nf;
+    m->isextdet = mesh->isextdet;
     m->med.assign(mesh->med, mesh->med + 4 * (mesh->prop + 1));
     /* the dual grid's extent (mesh_createdualmesh, src/mmc_mesh.c:349-372):
      * the node bounding box widened by EPS; every grid starts at its corner */
@@ -517,7 +520,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     }
     const int tracer = sims[0].method == MMC_RT_PLUCKER ? kTrNodal
                        : sims[0].method == MMC_RT_BLBADOUELGRID ? kTrGrid : kTrElem;
-    const int nmed = m->prop + 1;
+    /* media rows 0..prop, then the detector label's row: a copy of the
+     * background (src/mmc_mesh.c:518-520) */
+    const int nmed = m->prop + 1 + m->isextdet;
     const int block = (tune && tune->block > 0) ? tune->block : kBlock;
     if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
     const bool comp = arena ? (arena->lo != nullptr) : (tune && tune->compensated);
@@ -567,7 +572,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         h.flags = ((s.minenergy > 0.f && span <= 1.f && rmode == kReplayNone) ? kFlagRoulette : 0) |
                   ((s.outputtype == MMC_OUT_ENERGY || rmode == kReplayWL || rmode == kReplayWP) ? kFlagEnergy : 0) |
                   (rmode << kReplayShift) |
-                  ((s.issavedet && s.detnum > 0) ? kFlagSaveDet : 0) |
+                  ((s.issavedet && (s.detnum > 0 || m->isextdet)) ? kFlagSaveDet : 0) |
                   (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
                   ((s.isspecular & 3) << kSpecShift) | ((s.debuglevel & MMC_DL_TRAJ) ? kFlagTraj : 0);
         h.nout = s.nout;
@@ -600,9 +605,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         c.isspecular = s.isspecular;
         c.voidtime = s.voidtime;
         c.outputtype = s.outputtype;
-        c.issavedet = (s.issavedet && s.detnum > 0);
+        /* wide-field detectors replace the point detectors (src/mmc_mesh.c:402-406) */
+        c.issavedet = (s.issavedet && (s.detnum > 0 || m->isextdet));
         c.issaveexit = c.issavedet ? s.issaveexit : 0;
-        c.detnum = c.issavedet ? s.detnum : 0;
+        c.detnum = (c.issavedet && !m->isextdet) ? s.detnum : 0;
         c.detbase = (int)dets.size();
         for (int d = 0; d < c.detnum; d++)
             dets.push_back(make_float4(s.detpos[4 * d], s.detpos[4 * d + 1], s.detpos[4 * d + 2], s.detpos[4 * d + 3]));
@@ -629,8 +635,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         }
         if (tracer == kTrGrid) grids[i] = grid_of(m, &s);
         const float* md = s.med ? s.med : m->med.data();
-        for (int k = 0; k < nmed; k++)
+        for (int k = 0; k <= m->prop; k++)
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
+        if (m->isextdet) media[(size_t)i * nmed + m->prop + 1] = media[(size_t)i * nmed];
         cum[i + 1] = cum[i] + s.nphoton;
         if (c.issavedet) {
             savedet = 1;
@@ -737,6 +744,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.nn = m->nn;
     a.ne = m->ne;
     a.prop = m->prop;
+    a.isextdet = m->isextdet;
     a.nsim = nsim;
     a.hot = (const SimHot*)sl->d_hot.p;
     a.cold = (const SimCold*)sl->d_cold.p;
@@ -793,11 +801,11 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                               : (comp ? (const void*)transport<kTrElem, true, O, S>                            \
                                       : (const void*)transport<kTrElem, false, O, S>);
     if (!stage) {
-        MMC_PICK(4, false)
+        MMC_PICK(4, kTabGlobal)
     } else if (want >= 5) {
-        MMC_PICK(5, true)
+        MMC_PICK(5, kTabLds)
     } else {
-        MMC_PICK(4, true)
+        MMC_PICK(4, kTabLds)
     }
 #undef MMC_PICK
     int dev = 0, ncu = 0, maxbpc = 0;

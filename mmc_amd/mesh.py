@@ -27,7 +27,13 @@ class TetMesh:
         self.prop = len(self.med) - 1
         if self.elem.min() < 1 or self.elem.max() > self.nn:
             raise ValueError("element node ids outside 1..nn")
-        if self.etype.min() < 0 or self.etype.max() > self.prop:
+        # wide-field detector elements: label -2 becomes prop+1 with a copy of
+        # the background medium (mmc_host_detelem; src/mmc_mesh.c:390-427, 511-527)
+        self.etype = self.etype.copy()
+        self.isextdet = int(_lib.lib().mmc_host_detelem(self.ne, ptr(self.etype, _lib.C.c_int32), self.prop) > 0)
+        if self.isextdet:
+            self.med = np.ascontiguousarray(np.vstack([self.med, self.med[:1]]), np.float32)
+        if self.etype.min() < 0 or self.etype.max() > self.prop + self.isextdet:
             raise ValueError("element labels outside 0..prop")
         self.prepared = False
 
@@ -79,6 +85,7 @@ class TetMesh:
             raise RuntimeError("mesh not prepared")
         m = MeshC()
         m.nn, m.ne, m.prop, m.nf = self.nn, self.ne, self.prop, int(self.nf)
+        m.isextdet = self.isextdet
         m.node = ptr(self.node, _lib.C.c_float)
         m.elem = ptr(self.elem, _lib.C.c_int32)
         m.type = ptr(self.etype, _lib.C.c_int32)

@@ -926,7 +926,8 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
 
 #define PPATH_ADD(cond)                                                            \
     do {                                                                          \
-        if (cfg->issavedet && (cond)) ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove; \
+        if (cfg->issavedet && (cond) && m->type[r.eid - 1] <= prop)                 \
+            ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove;                        \
     } while (0)
 
     while (1) {
@@ -945,7 +946,10 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
         }
         /* the reference reads type[eid-1] before testing faceid>=0; with
          * eid=ID_UNDEFINED faceid is -1, so no path is added */
-        if (cfg->issavedet && r.Lmove > 0.f && r.faceid >= 0 && m->type[r.eid - 1] > 0)
+        /* paths in wide-field detector elements (label prop+1) are not
+         * partial paths of a tissue */
+        if (cfg->issavedet && r.Lmove > 0.f && r.faceid >= 0 && m->type[r.eid - 1] > 0 &&
+            m->type[r.eid - 1] <= prop)
             ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove;
         if (r.faceid == -2) break;
 
@@ -967,8 +971,10 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
             if (m->type[oldeid - 1] == 0 && m->type[r.eid - 1]) {
                 if (!cfg->voidtime) r.photontimer = 0.f;
             }
-            if (m->type[oldeid - 1] && m->type[r.eid - 1] == 0) {
-                r.eid = 0;   /* no external detectors in this restatement */
+            /* leaving the tissue into the background ends the photon unless
+             * wide-field detectors wait outside (src/mmc_raytrace.c:1930-1942) */
+            if (m->type[oldeid - 1] && m->type[r.eid - 1] == 0 && !m->isextdet) {
+                r.eid = 0;
                 break;
             }
             r.slen = orc_trace(&r, c);
@@ -994,7 +1000,12 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
                 ppath[reclen - 2 - 2] = r.vec.y;
                 ppath[reclen - 2 - 1] = r.vec.z;
             }
-            if (cfg->issavedet && r.eid <= 0) {
+            if (cfg->issavedet && r.eid <= 0 && m->isextdet && cfg->detnum == 0 &&
+                m->type[oldeid - 1] == prop + 1) {
+                /* left the mesh through a detector element: its id is the
+                 * detector id (src/mmc_raytrace.c:2013-2014) */
+                exitdet = oldeid;
+            } else if (cfg->issavedet && r.eid <= 0) {
                 for (int i = 0; i < cfg->detnum; i++) {
                     const float* d = cfg->detpos + 4 * i;
                     if ((d[0] - r.p0.x) * (d[0] - r.p0.x) + (d[1] - r.p0.y) * (d[1] - r.p0.y) +

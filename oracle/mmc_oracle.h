@@ -64,6 +64,9 @@ typedef struct {
     const float* nvol;     /* nn */
     const int* srcelem;    /* srcelemlen candidate launch elements */
     int srcelemlen;
+    /* wide-field detector elements (mesh label -2, relabelled prop+1 with
+     * the background's optical properties in med[prop+1]) */
+    int isextdet;
 } orc_mesh;
 
 typedef struct {

@@ -29,7 +29,7 @@ class OMesh(C.Structure):
     _fields_ = [("nn", C.c_int), ("ne", C.c_int), ("prop", C.c_int), ("nf", C.c_int),
                 ("node", f32p), ("elem", i32p), ("type", i32p), ("facenb", i32p), ("med", f32p),
                 ("planes", f32p), ("pd", f32p), ("pm", f32p), ("evol", f32p), ("nvol", f32p),
-                ("srcelem", i32p), ("srcelemlen", C.c_int)]
+                ("srcelem", i32p), ("srcelemlen", C.c_int), ("isextdet", C.c_int)]
 
 
 class OCfg(C.Structure):
@@ -116,6 +116,12 @@ class OracleMesh:
         self.etype = np.ascontiguousarray(etype, np.int32).copy()
         self.med = np.ascontiguousarray(med, np.float32).reshape(-1, 4)
         self.nn, self.ne, self.prop = len(self.node), len(self.elem), len(self.med) - 1
+        # wide-field detector elements: label -2 -> prop+1, medium = the
+        # background's (mesh_srcdetelem / mesh_loadmedia, src/mmc_mesh.c:390-427, 511-527)
+        self.isextdet = int(np.any(self.etype == -2))
+        if self.isextdet:
+            self.etype[self.etype == -2] = self.prop + 1
+            self.med = np.ascontiguousarray(np.vstack([self.med, self.med[:1]]), np.float32)
         self.evol = np.zeros(self.ne, np.float32)
         self.nvol = np.zeros(self.nn, np.float32)
         L.orc_getvolume(self.nn, p(self.node, C.c_float), self.ne, p(self.elem, C.c_int32),
@@ -140,6 +146,7 @@ class OracleMesh:
         m.facenb, m.med = p(self.facenb, C.c_int32), p(self.med, C.c_float)
         m.planes, m.pd, m.pm = p(self.planes, C.c_float), p(self.pd, C.c_float), p(self.pm, C.c_float)
         m.evol, m.nvol = p(self.evol, C.c_float), p(self.nvol, C.c_float)
+        m.isextdet = self.isextdet
         if self.srcelem is not None and len(self.srcelem):
             m.srcelem, m.srcelemlen = p(self.srcelem, C.c_int32), len(self.srcelem)
         else:
@@ -166,7 +173,9 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
         c.gridmin[:], c.griddim[:] = grid(mesh, c.gridstep)
     c.outputtype = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5}[sim.outputtype]
     c.isreflect, c.isspecular, c.voidtime = int(sim.isreflect), int(sim.isspecular), int(sim.voidtime)
-    c.issavedet = int(bool(sim.issavedet and sim.detpos is not None and len(sim.detpos)))
+    # wide-field detector elements replace the point detectors (src/mmc_mesh.c:402-406)
+    extdet = bool(getattr(mesh, "isextdet", 0))
+    c.issavedet = int(bool(sim.issavedet and (extdet or (sim.detpos is not None and len(sim.detpos)))))
     c.issaveexit = int(sim.issaveexit) if c.issavedet else 0
     c.ismomentum = 0
     c.srctype = {"pencil": 0, "planar": 4, "pattern": 5}[sim.srctype]
@@ -198,7 +207,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
         c.srcpattern = C.cast(None, f32p)
     c.tstart, c.tend, c.tstep, c.maxgate = t0, t1, dt, maxgate
     c.nout, c.minenergy, c.roulettesize = sim.nout, sim.minenergy, sim.roulettesize
-    if c.issavedet:
+    if c.issavedet and not extdet:
         det = np.ascontiguousarray(sim.detpos, np.float32).reshape(-1, 4)
         keep.append(det)
         c.detnum, c.detpos = len(det), p(det, C.c_float)
