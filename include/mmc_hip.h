@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 6
+#define MMC_HIP_ABI_VERSION 7
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -126,6 +126,12 @@ typedef struct mmc_hip_sim {
     float unitinmm;          /* mesh length unit in mm (normalization of grid fields); 0 = 1 */
     int32_t debuglevel;      /* MMC_DL_TRAJ: record each photon's launch, scattering and end
                                 positions (savedebugdata, src/mmc_raytrace.c:1739-1756) */
+    /* RF forward (cfg->omega, rad/s; mmc_core.cl IS_RF): with omega > 0 the photon carries a
+     * complex weight turning by omega*n/c0 per unit path and deposits the complex fluence;
+     * the imaginary part goes to result.field_im.  Branch-less Badouel (element or grid
+     * output) with srcnum 1; ignored for replayed photons, as in the reference.  One
+     * omega per batch. */
+    float omega;
 } mmc_hip_sim;
 
 /* Per-photon end state, written for photon ids in [recfirst, recfirst+reccount). */
@@ -162,6 +168,8 @@ typedef struct mmc_hip_result {
                                 order) */
     int64_t maxtraj;         /* capacity in records (cfg->maxjumpdebug) */
     int64_t ntraj;           /* records produced (may exceed maxtraj: the rest were dropped) */
+    double* field_im;        /* RF: imaginary part of the field, same layout as field (+=), or NULL
+                                (cfg->exportadjoint of the reference's RF forward mode) */
 } mmc_hip_result;
 
 /* Launch shape for the persistent kernel; zeros pick the defaults. */

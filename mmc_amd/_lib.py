@@ -48,6 +48,7 @@ class SimC(C.Structure):
         ("med", f32p),
         ("photonseed", C.POINTER(C.c_uint64)), ("replayweight", f32p), ("replaytime", f32p),
         ("gridstep", C.c_float), ("unitinmm", C.c_float), ("debuglevel", C.c_int32),
+        ("omega", C.c_float),
     ]
 
 
@@ -74,7 +75,7 @@ class ResultC(C.Structure):
         ("detdata", f32p), ("detphoton", i64p), ("detected", C.c_int64), ("reclen", C.c_int32),
         ("photons", C.c_void_p), ("recfirst", C.c_int64), ("reccount", C.c_int64),
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("srcweight", f64p),
-        ("traj", f32p), ("maxtraj", C.c_int64), ("ntraj", C.c_int64),
+        ("traj", f32p), ("maxtraj", C.c_int64), ("ntraj", C.c_int64), ("field_im", f64p),
     ]
 
 
@@ -138,7 +139,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 6        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 7        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

@@ -250,6 +250,8 @@ static void load_json(cli_cfg* c, const char* path) {
         c->sim.tend = (float)mj_num(mj_get(Forward, "T1"), 0);
         c->sim.tstep = (float)mj_num(mj_get(Forward, "Dt"), 0);
         c->sim.nout = (float)mj_num(mj_get(Forward, "N0"), c->sim.nout);
+        /* RF modulation, rad/s (src/mmc_utils.c:2342) */
+        c->sim.omega = (float)mj_num(mj_get(Forward, "Omega"), 0);
     }
     mj_free(root);
 }
@@ -361,8 +363,9 @@ static void dump_json(const cli_cfg* c) {
     const char* fmts[] = {"ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii", "bnii"};
     fprintf(fp, "\t\t\"RayTracer\": \"%c\",\n\t\t\"OutputFormat\": \"%s\",\n\t\t\"OutputType\": \"%c\"\n\t},\n",
             c->method, fmts[c->outputformat], otype[s->outputtype >= 0 && s->outputtype <= 5 ? s->outputtype : 0]);
-    fprintf(fp, "\t\"Forward\": {\n\t\t\"T0\": %.9g,\n\t\t\"T1\": %.9g,\n\t\t\"Dt\": %.9g,\n\t\t\"N0\": %.9g\n\t},\n",
-            (double)s->tstart, (double)s->tend, (double)s->tstep, (double)s->nout);
+    fprintf(fp, "\t\"Forward\": {\n\t\t\"T0\": %.9g,\n\t\t\"T1\": %.9g,\n\t\t\"Dt\": %.9g,\n\t\t\"N0\": %.9g,\n"
+                "\t\t\"Omega\": %.9g\n\t},\n",
+            (double)s->tstart, (double)s->tend, (double)s->tstep, (double)s->nout, (double)s->omega);
     fprintf(fp, "\t\"Domain\": {\n\t\t\"LengthUnit\": %.9g,\n\t\t\"Step\": [%.9g, %.9g, %.9g],\n\t\t\"Media\": [",
             (double)c->unitinmm, (double)c->gridstep, (double)c->gridstep, (double)c->gridstep);
     for (int i = 0; i < c->nmed; i++)
@@ -714,9 +717,13 @@ int main(int argc, char** argv) {
         datalen = (int)nc;
     }
     double* field = (double*)calloc((size_t)datalen * c.sim.maxgate, sizeof(double));
+    /* RF forward: the imaginary part of the complex fluence */
+    const int isrf = c.sim.omega > 0.f && !c.rseeds;
+    double* field_im = isrf ? (double*)calloc((size_t)datalen * c.sim.maxgate, sizeof(double)) : NULL;
     mmc_hip_result res;
     memset(&res, 0, sizeof(res));
     res.field = field;
+    res.field_im = field_im;
     if (c.savetraj) {
         if (c.maxjumpdebug < 1) die("%s", "--maxjumpdebug must be positive");
         c.sim.debuglevel |= MMC_DL_TRAJ;
@@ -746,6 +753,9 @@ int main(int argc, char** argv) {
         normalizer = isgrid ? mmc_host_normalize_grid(&s, datalen, field, res.absorbweight, res.launchweight, 0)
                             : mmc_host_normalize(&s, c.nn, c.ne, elem, type, c.med, evol, nvol, field,
                                                  res.absorbweight, res.launchweight);
+        /* the imaginary part takes the same factor (src/mmc_mesh.c:2262-2274) */
+        if (field_im)
+            for (size_t k = 0; k < (size_t)datalen * c.sim.maxgate; k++) field_im[k] *= normalizer;
     }
     printf("total simulated energy: %.0f\tabsorbed: %5.5f%%\tnormalizor=%g\n", res.launchweight,
            100.0 * res.absorbweight / res.launchweight, normalizer);
@@ -755,6 +765,16 @@ int main(int argc, char** argv) {
                                              1, c.sim.tstep, c.sim.outputtype, c.zip)
                         : mmc_host_save_field(c.session, c.outputformat, field, datalen, c.sim.maxgate, 1,
                                               c.sim.tstep, c.sim.outputtype, c.zip);
+        if (!rc && field_im) {
+            /* the reference returns it to its MATLAB interface only; written
+             * here as <session>_imag.<ext> */
+            char sess_im[300];
+            snprintf(sess_im, sizeof(sess_im), "%s_imag", c.session);
+            rc = isgrid ? mmc_host_save_grid(sess_im, c.outputformat, field_im, griddim, c.gridstep, c.sim.maxgate, 1,
+                                             c.sim.tstep, c.sim.outputtype, c.zip)
+                        : mmc_host_save_field(sess_im, c.outputformat, field_im, datalen, c.sim.maxgate, 1,
+                                              c.sim.tstep, c.sim.outputtype, c.zip);
+        }
         if (rc)
             die("can not write the output of session %s", c.session);
     }
@@ -825,6 +845,7 @@ int main(int argc, char** argv) {
         printf("saved %lld trajectory positions\n", (long long)kept);
     }
     free(res.traj);
+    free(field_im);
     free(field);
     free(res.detdata);
     free(res.detphoton);

@@ -122,6 +122,8 @@ struct KArgs {
     const SimGrid* grid;             /* nsim, dual-grid tracer only */
     float* traj;                     /* maxtraj trajectory records, or null */
     uint64_t maxtraj;
+    float omega;                     /* RF modulation (rad/s), RF kernels only */
+    int32_t pad_rf;
 };
 
 /* Per-photon xorshift128+ state: splitmix64 finalizer over (seed, photon id).
@@ -141,8 +143,8 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
 
 /* Tracer and output of a kernel variant: branch-less Badouel with element
  * output, Plucker with nodal output, branch-less Badouel with dual-grid
- * (voxel) output. */
-enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2 };
+ * (voxel) output, and the two Badouel outputs with RF (complex) weights. */
+enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2, kTrElemRF = 3, kTrGridRF = 4 };
 /* where a launch keeps its per-simulation tables: read through the caches
  * or staged in LDS */
 enum { kTabGlobal = 0, kTabLds = 1 };

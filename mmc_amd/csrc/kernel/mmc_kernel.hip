@@ -143,7 +143,10 @@ __device__ __forceinline__ uint32_t xcd_id() {
 template <int TR, bool COMP, int OCC, int TAB>
 __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     constexpr bool NODAL = TR == kTrNodal;
-    constexpr bool GRID = TR == kTrGrid;
+    constexpr bool GRID = TR == kTrGrid || TR == kTrGridRF;
+    /* RF forward (omega > 0, mmc_core.cl IS_RF): complex photon weight and
+     * complex fluence, imaginary part in a second block of the field */
+    constexpr bool RF = TR == kTrElemRF || TR == kTrGridRF;
     /* per-simulation tables: in LDS (small batches) or through the caches
      * (large batches).  (Reading a single simulation's tables as wave-uniform
      * values was measured 16 % slower on the slab: the scalar register file
@@ -217,6 +220,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     bool npend = false;
     int nsrc = 1;                       /* patterns sharing the photon (field slots) */
     float rw = 1.f, rt = 0.f;           /* replay: detected weight and time of the photon */
+    float wi = 0.f, oldwi = 0.f;        /* RF: imaginary weight and its coalescing register */
+    float wm = 0.f;                     /* RF: weight magnitude (absorption ledger, roulette) */
     float slen0 = 0.f;                  /* scattering length at the last scattering event */
     int posidx = 0;                     /* launch pixel of a pattern source */
     /* the lane's simulation index into the tables, and its hot parameters */
@@ -342,6 +347,10 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     eid = C.e0;
                     w = 1.f; tof = 0.f; Lmove = 0.f;
                     oldidx = 0xFFFFFFFFu; oldw = 0.f;
+                    if (RF) {
+                        wi = 0.f;
+                        oldwi = 0.f;
+                    }
                     ctx = CTX_OUTER; fixcount = 0;
                     eabs = 0.f; eabs_c = 0.f;
                     raytet_start = raytet;
@@ -428,6 +437,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     ctype = (eid > 0 && eid != ID_UNDEF) ? a.type[eid - 1] : 0;
                     *LS(slot_w0) = w;
+                    if (RF) wm = w;
                     *LS(slot_rg) = 0.f;
                     *LS(slot_ns) = 0.f;
                     if (a.savedet)
@@ -528,13 +538,43 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                                                                        : (int)((tof - hot.tstart) * hot.rtstep);
                     const int framelen = GRID ? a.grid[sidx()].ncells : a.ne;
                     int tshift = max(min(g, hot.maxgate - 1), 0) * framelen;
+                    float dabs = ww;
+                    if (RF) {
+                        /* the magnitude of the complex weight decays as the
+                         * real weight does without modulation */
+                        const float wb = wm;
+                        wm *= att;
+                        dabs = wb - wm;
+                    }
                     if (mua > 0.f) {
-                        float y = ww - eabs_c, tt = eabs + y;
+                        float y = dabs - eabs_c, tt = eabs + y;
                         eabs_c = (tt - eabs) - y;
                         eabs = tt;
                     }
                     if (!(hot.flags & kFlagEnergy))
                         ww = (mua < EPSF) ? (cw * Lmove) : ((cw - w) / mua);
+                    /* RF: complex Beer-Lambert over the step, w <- w e^{-(mua + i k) L}
+                     * with k = omega n / c0, and the complex deposit
+                     * (w_before - w_after) / (mua + i k) (src/mmc_core.cl:868-897) */
+                    float ww_im = 0.f;
+                    const float kap = RF ? a.omega * nidx * R_C0F : 0.f;
+                    const float amag2 = mua * mua + kap * kap;
+                    if (RF && !GRID) {
+                        const float tl = 1.f - att;
+                        const float attr = (tl < 1.f) ? (1.f - tl) : 1.f;
+                        const float phase = a.omega * nidx * R_C0F * Lmove;
+                        float sphi, cphi;
+                        mmc_sincosf(phase, &sphi, &cphi);
+                        const float w0r = currweight, w0i = wi;
+                        const float nre = attr * (w0r * cphi + w0i * sphi);
+                        const float nim = attr * (-w0r * sphi + w0i * cphi);
+                        const float dwr = w0r - nre, dwi = w0i - nim;
+                        ww = (amag2 > 0.f) ? (dwr * mua + dwi * kap) / amag2 : (w0r * Lmove);
+                        ww_im = (amag2 > 0.f) ? (dwi * mua - dwr * kap) / amag2 : (w0i * Lmove);
+                        w = nre;
+                        wi = nim;
+                    }
+                    const uint64_t imoff = RF ? (uint64_t)framelen * (uint64_t)hot.maxgate : 0;
                     const float q0x = px, q0y = py, q0z = pz;
                     px = px + vx * Lmove;
                     py = py + vy * Lmove;
@@ -549,15 +589,32 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                              * reference adds that zero (src/mmc_raytrace.c:
                              * 1562-1612); x + 0 == x for the non-negative
                              * field, so the atomic is skipped, which halves
-                             * the atomics */
-                            if (oldw != 0.f) depo(foff, oldidx, oldw);
+                             * the atomics.  RF: the GPU reference flushes the
+                             * register only when its real part is positive
+                             * (src/mmc_core.cl:905) */
+                            if (RF) {
+                                if (oldw > 0.f) {
+                                    depo(foff, oldidx, oldw);
+                                    if (oldwi != 0.f) depo(foff + imoff, oldidx, oldwi);
+                                }
+                                oldwi = ww_im;
+                            } else if (oldw != 0.f) {
+                                depo(foff, oldidx, oldw);
+                            }
                             oldidx = newidx;
                             oldw = ww;
                         } else {
                             oldw += ww;
+                            if (RF) oldwi += ww_im;
                         }
                         if (faceid == -2 || !isend) {
-                            depo(foff, newidx, oldw);
+                            if (RF) {
+                                if (oldw != 0.f) depo(foff, newidx, oldw);
+                                if (oldwi != 0.f) depo(foff + imoff, newidx, oldwi);
+                                oldwi = 0.f;
+                            } else {
+                                depo(foff, newidx, oldw);
+                            }
                             oldw = 0.f;
                         }
                     } else {
@@ -580,27 +637,55 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         const float frac = (tl == 0.f) ? 0.f : (1.f - segloss) / tl;
                         const bool flush = (faceid == -2 || !isend);
                         float w0 = ww;
+                        /* RF: the complex weight turns by the same phase in
+                         * every segment (src/mmc_core.cl:1043-1081) */
+                        float sr = currweight, si = wi, scos = 1.f, ssin = 0.f;
+                        if (RF) mmc_sincosf(a.omega * nidx * R_C0F * dstep, &ssin, &scos);
                         for (int i = 0; i < seg; i++) {
                             const int ix = min(max((int)(sx * G.org.w), 0), G.nx - 1);
                             const int iy = min(max((int)(sy * G.org.w), 0), G.ny - 1);
                             const int iz = min(max((int)(sz * G.org.w), 0), G.nz - 1);
                             const uint32_t newidx = (uint32_t)(iz * G.dxy + iy * G.dx + ix + tshift);
                             if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
+                            float d = w0 * frac, dim = 0.f;
+                            if (RF) {
+                                const float w0r = sr, w0i = si;
+                                const float nre = segloss * (w0r * scos + w0i * ssin);
+                                const float nim = segloss * (-w0r * ssin + w0i * scos);
+                                sr = nre;
+                                si = nim;
+                                const float dwr = w0r - nre, dwi = w0i - nim;
+                                d = (amag2 < EPSF) ? (w0r * w0) : (dwr * mua + dwi * kap) / amag2;
+                                dim = (amag2 < EPSF) ? (w0i * w0) : (dwi * mua - dwr * kap) / amag2;
+                            }
                             if (newidx != oldidx) {
                                 if (oldw != 0.f) depo(foff, oldidx, oldw);
+                                if (RF) {
+                                    if (oldwi != 0.f) depo(foff + imoff, oldidx, oldwi);
+                                    oldwi = dim;
+                                }
                                 oldidx = newidx;
-                                oldw = w0 * frac;
+                                oldw = d;
                             } else {
-                                oldw += w0 * frac;
+                                oldw += d;
+                                if (RF) oldwi += dim;
                             }
                             if (flush) {
                                 if (oldw != 0.f) depo(foff, newidx, oldw);
                                 oldw = 0.f;
+                                if (RF) {
+                                    if (oldwi != 0.f) depo(foff + imoff, newidx, oldwi);
+                                    oldwi = 0.f;
+                                }
                             }
                             w0 *= segloss;
                             sx = sx + tx;
                             sy = sy + ty;
                             sz = sz + tz;
+                        }
+                        if (RF) {
+                            w = sr;
+                            wi = si;
                         }
                     }
                 }
@@ -861,12 +946,18 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
             if (eid <= 0 || eid == ID_UNDEF || undef) {
                 reason = (eid == ID_UNDEF) ? MMC_END_LOST : MMC_END_EXIT;
             } else {
-                if ((hot.flags & kFlagRoulette) && w < hot.minenergy) {
+                /* RF: the roulette tests the magnitude of the complex weight
+                 * and scales both parts (src/mmc_core.cl:2090-2110) */
+                if ((hot.flags & kFlagRoulette) && (RF ? wm : w) < hot.minenergy) {
                     const float rsize = hot.roulettesize;
                     if (rng_next(r0, r1) * rsize <= 1.f) {
-                        float wb = w;
+                        float wb = RF ? wm : w;
                         w *= rsize;
-                        *LS(slot_rg) += w - wb;
+                        if (RF) {
+                            wi *= rsize;
+                            wm *= rsize;
+                        }
+                        *LS(slot_rg) += (RF ? wm : w) - wb;
                     } else {
                         reason = MMC_END_ROULETTE;
                     }
@@ -958,9 +1049,11 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                 }
             }
-            const float wend = (reason == MMC_END_ROULETTE) ? 0.f : w;
+            /* RF: the escaping energy is the weight's magnitude
+             * (src/mmc_core.cl:2145-2153) */
+            const float wend = (reason == MMC_END_ROULETTE) ? 0.f : (RF ? wm : w);
             float rg = *LS(slot_rg);
-            if (reason == MMC_END_ROULETTE) rg -= w;
+            if (reason == MMC_END_ROULETTE) rg -= (RF ? wm : w);
             kahan_lds(LS(2), LS(3), eabs);
             if (nsrc > 1)
                 for (int k = 0; k < nsrc; k++) *LS(slot_pa + k) += eabs * *LS(slot_pw + k);
@@ -1019,7 +1112,11 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     template __global__ void transport<kTrNodal, false, OCC, TAB>(KArgs);   \
     template __global__ void transport<kTrNodal, true, OCC, TAB>(KArgs);    \
     template __global__ void transport<kTrGrid, false, OCC, TAB>(KArgs);    \
-    template __global__ void transport<kTrGrid, true, OCC, TAB>(KArgs);
+    template __global__ void transport<kTrGrid, true, OCC, TAB>(KArgs);      \
+    template __global__ void transport<kTrElemRF, false, OCC, TAB>(KArgs);  \
+    template __global__ void transport<kTrElemRF, true, OCC, TAB>(KArgs);   \
+    template __global__ void transport<kTrGridRF, false, OCC, TAB>(KArgs);  \
+    template __global__ void transport<kTrGridRF, true, OCC, TAB>(KArgs);
 MMC_INSTANTIATE(4, kTabLds)
 MMC_INSTANTIATE(5, kTabLds)
 MMC_INSTANTIATE(4, kTabGlobal)

@@ -323,6 +323,9 @@ static SimGrid grid_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
     return g;
 }
 
+/* RF forward: complex weights, imaginary field block after the real one */
+static bool rf_of(const mmc_hip_sim* s) { return s->omega > 0.f && !s->photonseed; }
+
 /* values per gate (nodes, elements or voxels) times the pattern slots */
 static int64_t datalen_of(const mmc_hip_meshres* m, const mmc_hip_sim* s) {
     if (s->method == MMC_RT_BLBADOUELGRID) return (int64_t)grid_of(m, s).ncells * slots_of(s);
@@ -348,7 +351,7 @@ extern "C" int32_t mmc_hip_sim_grid(mmc_hip_mesh_handle h, const mmc_hip_sim* s,
 extern "C" int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* s) {
     if (!h || !s) return -1;
     /* FP32 working field + FP64 reduction target */
-    return datalen_of(h, s) * (int64_t)s->maxgate * 12;
+    return datalen_of(h, s) * (int64_t)s->maxgate * 12 * (rf_of(s) ? 2 : 1);
 }
 
 /* Host checks of a simulation against the mesh; every id the kernel will
@@ -388,6 +391,10 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
         if (s->outputtype != MMC_OUT_JACOBIAN && !s->replaytime)
             return set_err(-1, "weighted path/scattering outputs need the detected times");
     }
+    if (!(s->omega >= 0.f)) return set_err(-1, "omega must be >= 0");
+    if (rf_of(s) && (s->method == MMC_RT_PLUCKER || slots_of(s) != 1))
+        return set_err(-1, "RF (omega > 0) needs the branch-less Badouel tracer (element or grid output) and one "
+                           "pattern");
     if (s->isreflect < MMC_BC_NOREFLECT || s->isreflect > MMC_BC_MIRROR)
         return set_err(-1, "unsupported boundary condition %d", s->isreflect);
     if (s->isspecular < 0 || s->isspecular > 2) return set_err(-1, "isspecular must be 0, 1 or 2");
@@ -468,7 +475,7 @@ struct mmc_hip_slotres {
     std::vector<int64_t> cum;
     std::vector<uint64_t> fieldoff;
     std::vector<int64_t> datalen, maxdetph, photon_offset, nphoton;
-    std::vector<int32_t> maxgate, nslot;
+    std::vector<int32_t> maxgate, nslot, rfmul;
     DevBuf field, lo;                 /* own fields of a launch without an arena */
     /* host sources of the launch's asynchronous copies, alive until it ends */
     std::vector<SimHot> h_hot;
@@ -518,8 +525,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         int rc = check_sim(m, sims + i, sims[0].method);
         if (rc) return set_err(rc, "simulation %d: %s", i, std::string(g_err).c_str());
     }
+    const bool rf = rf_of(&sims[0]);
     const int tracer = sims[0].method == MMC_RT_PLUCKER ? kTrNodal
-                       : sims[0].method == MMC_RT_BLBADOUELGRID ? kTrGrid : kTrElem;
+                       : sims[0].method == MMC_RT_BLBADOUELGRID ? (rf ? kTrGridRF : kTrGrid)
+                                                                : (rf ? kTrElemRF : kTrElem);
     /* media rows 0..prop, then the detector label's row: a copy of the
      * background (src/mmc_mesh.c:518-520) */
     const int nmed = m->prop + 1 + m->isextdet;
@@ -538,7 +547,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     std::vector<float>& rws = sl->h_rw;
     std::vector<float>& rts = sl->h_rt;
     std::vector<SimGrid>& grids = sl->h_grid;
-    grids.assign(tracer == kTrGrid ? nsim : 0, SimGrid());
+    const bool gridout = tracer == kTrGrid || tracer == kTrGridRF;
+    grids.assign(gridout ? nsim : 0, SimGrid());
     seeds.clear();
     rws.clear();
     rts.clear();
@@ -578,7 +588,9 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         h.nout = s.nout;
         h.srcnum = slots_of(&s);
         maxsrc = std::max(maxsrc, h.srcnum);
-        const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate;
+        if (rf_of(&s) != rf_of(&sims[0]) || (rf_of(&s) && s.omega != sims[0].omega))
+            return set_err(-1, "simulation %d: all simulations of a batch must share omega", i);
+        const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate * (rf_of(&s) ? 2 : 1);
         if (arena) {
             if (arena_off[i] + flen > arena->n)
                 return set_err(-1, "simulation %d: field [%llu, +%llu) outside the arena", i,
@@ -633,7 +645,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                 rts.push_back(s.replaytime ? s.replaytime[s.photon_offset + k] : 0.f);
             }
         }
-        if (tracer == kTrGrid) grids[i] = grid_of(m, &s);
+        if (gridout) grids[i] = grid_of(m, &s);
         const float* md = s.med ? s.med : m->med.data();
         for (int k = 0; k <= m->prop; k++)
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
@@ -775,8 +787,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.rweight = seeds.empty() ? nullptr : (const float*)sl->d_rw.p;
     a.rtime = seeds.empty() ? nullptr : (const float*)sl->d_rt.p;
     a.grid = grids.empty() ? nullptr : (const SimGrid*)sl->d_grid.p;
+    if (gridout && !a.grid) return set_err(-1, "internal: dual-grid launch without its grid table");
     a.traj = maxtraj > 0 ? (float*)sl->d_traj.p : nullptr;
     a.maxtraj = (uint64_t)maxtraj;
+    a.omega = rf ? sims[0].omega : 0.f;
 
     const int nslots = kLaneSlots + 2 * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that
@@ -793,13 +807,13 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
      * caches (STAGE=false) */
     const int want = (tune && tune->blocks_per_cu > 0) ? tune->blocks_per_cu : 0;
     const void* kfn = nullptr;
-#define MMC_PICK(O, S)                                                                                     \
-    kfn = tracer == kTrNodal  ? (comp ? (const void*)transport<kTrNodal, true, O, S>                           \
-                                      : (const void*)transport<kTrNodal, false, O, S>)                         \
-          : tracer == kTrGrid ? (comp ? (const void*)transport<kTrGrid, true, O, S>                            \
-                                      : (const void*)transport<kTrGrid, false, O, S>)                          \
-                              : (comp ? (const void*)transport<kTrElem, true, O, S>                            \
-                                      : (const void*)transport<kTrElem, false, O, S>);
+#define MMC_KFN(T, O, S) (comp ? (const void*)transport<T, true, O, S> : (const void*)transport<T, false, O, S>)
+#define MMC_PICK(O, S)                                                                 \
+    kfn = tracer == kTrNodal    ? MMC_KFN(kTrNodal, O, S)                               \
+          : tracer == kTrGrid   ? MMC_KFN(kTrGrid, O, S)                                \
+          : tracer == kTrGridRF ? MMC_KFN(kTrGridRF, O, S)                              \
+          : tracer == kTrElemRF ? MMC_KFN(kTrElemRF, O, S)                              \
+                                : MMC_KFN(kTrElem, O, S);
     if (!stage) {
         MMC_PICK(4, kTabGlobal)
     } else if (want >= 5) {
@@ -808,6 +822,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         MMC_PICK(4, kTabLds)
     }
 #undef MMC_PICK
+#undef MMC_KFN
     int dev = 0, ncu = 0, maxbpc = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -845,6 +860,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     sl->nphoton.resize(nsim);
     sl->maxgate.resize(nsim);
     sl->nslot.resize(nsim);
+    sl->rfmul.resize(nsim);
     for (int i = 0; i < nsim; i++) {
         sl->datalen[i] = datalen_of(m, sims + i);
         sl->maxdetph[i] = sims[i].maxdetphoton;
@@ -852,6 +868,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         sl->nphoton[i] = sims[i].nphoton;
         sl->maxgate[i] = sims[i].maxgate;
         sl->nslot[i] = slots_of(sims + i);
+        sl->rfmul[i] = rf_of(sims + i) ? 2 : 1;
     }
     return 0;
 }
@@ -903,27 +920,34 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         uint64_t maxn = 1;
-        for (int i = 0; i < nsim; i++) maxn = std::max<uint64_t>(maxn, (uint64_t)sl->datalen[i] * sl->maxgate[i]);
+        for (int i = 0; i < nsim; i++)
+            maxn = std::max<uint64_t>(maxn, (uint64_t)sl->datalen[i] * sl->maxgate[i] * sl->rfmul[i]);
         DevBuf d64, dsum;
         int rc;
-        if ((rc = dalloc((double**)&d64.p, (size_t)maxn, nullptr)) || (rc = dalloc((double**)&dsum.p, (size_t)nsim, nullptr)))
+        if ((rc = dalloc((double**)&d64.p, (size_t)maxn, nullptr)) ||
+            (rc = dalloc((double**)&dsum.p, (size_t)nsim + 1, nullptr)))
             return rc;
-        HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * nsim, st));
+        HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * (nsim + 1), st));
         for (int i = 0; i < nsim; i++) {
             mmc_hip_result& r = res[i];
-            uint64_t n = (uint64_t)sl->datalen[i] * sl->maxgate[i];
-            if (n == 0 || sl->nphoton[i] == 0) continue;
-            int rb = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ncu * 16);
-            hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st,
-                               (float*)sl->field.p + sl->fieldoff[i],
-                               sl->lo.p ? (float*)sl->lo.p + sl->fieldoff[i] : nullptr, (double*)d64.p, n,
-                               (double*)dsum.p + i, sl->nrep, (uint64_t)sl->fieldlen);
-            HIPCHK(hipGetLastError());
-            if (r.field) {
-                std::vector<double> hbuf(n);
-                HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                for (uint64_t k = 0; k < n; k++) r.field[k] += hbuf[k];
+            /* RF: the imaginary block follows the real one; both are
+             * reduced and the sum covers the real part only */
+            const uint64_t nre = (uint64_t)sl->datalen[i] * sl->maxgate[i];
+            if (nre == 0 || sl->nphoton[i] == 0) continue;
+            int rb = (int)std::min<uint64_t>((nre + 255) / 256, (uint64_t)ncu * 16);
+            for (int part = 0; part < sl->rfmul[i]; part++) {
+                const uint64_t off = sl->fieldoff[i] + (uint64_t)part * nre;
+                hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, st, (float*)sl->field.p + off,
+                                   sl->lo.p ? (float*)sl->lo.p + off : nullptr, (double*)d64.p, nre,
+                                   (double*)dsum.p + (part ? nsim : i), sl->nrep, (uint64_t)sl->fieldlen);
+                HIPCHK(hipGetLastError());
+                double* dst = part ? r.field_im : r.field;
+                if (dst) {
+                    std::vector<double> hbuf(nre);
+                    HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * nre, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipStreamSynchronize(st));
+                    for (uint64_t k = 0; k < nre; k++) dst[k] += hbuf[k];
+                }
             }
         }
         std::vector<double> sums(nsim, 0.0);

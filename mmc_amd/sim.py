@@ -64,12 +64,18 @@ class SimConfig:
     replaytime: Optional[np.ndarray] = None
     gridstep: float = 1.0                  # dual-grid voxel edge (cfg->steps.x), mesh units
     unitinmm: float = 1.0                  # mesh length unit (normalization of grid fields)
+    omega: float = 0.0                     # RF modulation (rad/s): complex fluence, field_im
     trajectory: bool = False               # record launch/scattering/end positions ("-D M")
     maxjumpdebug: int = 10_000_000         # trajectory records kept
 
     def __post_init__(self):
         if self.basisorder is None or METHODS[self.method] == 4:
             self.basisorder = 1 if METHODS[self.method] == 0 else 0
+
+    @property
+    def is_rf(self) -> bool:
+        """RF forward: omega > 0 and not a replay (the reference's isrfforward)."""
+        return self.omega > 0 and self.photonseed is None
 
     @property
     def is_grid(self) -> bool:
@@ -178,6 +184,7 @@ class SimConfig:
         s.gridstep = float(self.gridstep)
         s.unitinmm = float(self.unitinmm)
         s.debuglevel = 8192 if self.trajectory else 0
+        s.omega = float(self.omega)
         s._keep = keep
         return s
 
@@ -212,6 +219,7 @@ class SimResult:
     normalizer: float = 1.0
     srcweight: Optional[np.ndarray] = None    # (2, srcnum): launched, absorbed energy per pattern
     traj: Optional[np.ndarray] = None         # TRAJ_DTYPE records (photon id, position, weight, element)
+    field_im: Optional[np.ndarray] = None     # RF: imaginary part of the field
     ntraj: int = 0                            # records produced (more than kept if the buffer filled)
 
 
@@ -222,6 +230,7 @@ class _ResultBuf:
         S = cfg.srcnum
         shape = (maxgate, datalen, S) if S > 1 else (maxgate, datalen)
         self.field = np.zeros(shape, np.float64) if want_field else None
+        self.field_im = np.zeros(shape, np.float64) if (want_field and cfg.is_rf) else None
         self.srcweight = np.zeros((2, S), np.float64) if S > 1 else None
         ndet = len(cfg.detpos) if (cfg.issavedet and cfg.detpos is not None) else 0
         self.reclen = (1 + 2 * mesh.prop + 6 * int(cfg.issaveexit) + 1) if ndet else 0
@@ -236,6 +245,7 @@ class _ResultBuf:
             r.traj = self.traj.ctypes.data_as(f32p)
             r.maxtraj = len(self.traj)
         r.field = ptr(self.field, C.c_double) if want_field else C.cast(None, f64p)
+        r.field_im = ptr(self.field_im, C.c_double) if self.field_im is not None else C.cast(None, f64p)
         r.detdata = ptr(self.detdata, C.c_float) if ndet else C.cast(None, f32p)
         r.detphoton = ptr(self.detphoton, C.c_int64) if ndet else C.cast(None, i64p)
         r.srcweight = ptr(self.srcweight, C.c_double) if self.srcweight is not None else C.cast(None, f64p)
@@ -258,7 +268,7 @@ class _ResultBuf:
             detphoton=self.detphoton[:n] if self.detphoton is not None else None,
             photons=self.photons, kernel_ms=r.kernel_ms, total_ms=r.total_ms, srcweight=self.srcweight,
             traj=self.traj[:min(int(r.ntraj), len(self.traj))] if self.traj is not None else None,
-            ntraj=int(r.ntraj))
+            ntraj=int(r.ntraj), field_im=self.field_im)
 
 
 def tune_struct(block=0, blocks_per_cu=0, compensated=0, chunk=1, replicas=0) -> TuneC:
@@ -286,7 +296,16 @@ def run(mesh: TetMesh, cfg: SimConfig, records=None, normalize=False, want_field
 
 def normalize_field(mesh: TetMesh, cfg: SimConfig, res: SimResult):
     """mesh_normalize in place; with shared patterns each pattern slot is
-    normalized with its own energies and the factors are returned as an array."""
+    normalized with its own energies and the factors are returned as an array.
+    An RF imaginary part is scaled by the same factor (the reference's
+    imag_slot *= normalizor, src/mmc_mesh.c:2262-2274)."""
+    k = _normalize_real(mesh, cfg, res)
+    if res.field_im is not None:
+        res.field_im *= k
+    return k
+
+
+def _normalize_real(mesh: TetMesh, cfg: SimConfig, res: SimResult):
     L = _lib.lib()
     sc = cfg.cstruct(mesh)
     if cfg.is_grid:

@@ -375,12 +375,16 @@ typedef struct {
     double oldweight;
     float* ppath;
     int pout_undef;
+    /* RF: imaginary weight, its coalescing register, and the magnitude */
+    float weight_im, wmag;
+    double oldweight_im;
 } oray;
 
 typedef struct {
     const orc_cfg* cfg;
     const orc_mesh* m;
     double* weight;
+    double* weight_im;
     float rtstep;
     int framelen;
     double raytet;
@@ -489,7 +493,9 @@ static float orc_badouel(oray* r, octx* c) {
         }
         float totalloss = O_EXPF(-mua * r->Lmove);
         r->weight *= totalloss;
+        const float att = totalloss;
         totalloss = 1.f - totalloss;
+        const int rf = cfg->omega > 0.f && !cfg->replayweight;
         r->slen -= r->Lmove * mus;
         currweight = replay_cw(c, currweight, r, mus);
         const float q0x = r->p0.x, q0y = r->p0.y, q0z = r->p0.z;
@@ -500,9 +506,37 @@ static float orc_badouel(oray* r, octx* c) {
             r->photontimer += r->Lmove * rc;
             int g = replay_gate(c, r->photontimer);
             int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * framelen;
-            if (mua > 0.f) r->Eabsorb += ww;
+            if (rf) {
+                /* the magnitude of the complex weight decays as the real
+                 * weight does without modulation */
+                float wb = r->wmag;
+                r->wmag *= att;
+                if (mua > 0.f) r->Eabsorb += wb - r->wmag;
+            } else if (mua > 0.f) {
+                r->Eabsorb += ww;
+            }
             if (fluence_like(cfg->outputtype))
                 ww = (mua < EPS) ? (currweight * r->Lmove) : ((currweight - r->weight) / mua);
+            /* RF: complex Beer-Lambert over the step and the complex deposit
+             * (w_before - w_after) / (mua + i omega n / c0)
+             * (src/mmc_core.cl:868-897) */
+            float ww_im = 0.f;
+            const float kap = rf ? cfg->omega * n * R_C0 : 0.f;
+            const float amag2 = mua * mua + kap * kap;
+            if (rf && cfg->method != ORC_RT_BLBADOUELGRID) {
+                float attr = totalloss < 1.f ? (1.f - totalloss) : 1.f;
+                float phase = cfg->omega * n * R_C0 * r->Lmove;
+                float sphi, cphi;
+                O_SINCOSF(phase, &sphi, &cphi);
+                float w0r = currweight, w0i = r->weight_im;
+                float nre = attr * (w0r * cphi + w0i * sphi);
+                float nim = attr * (-w0r * sphi + w0i * cphi);
+                float dwr = w0r - nre, dwi = w0i - nim;
+                ww = (amag2 > 0.f) ? (dwr * mua + dwi * kap) / amag2 : (w0r * r->Lmove);
+                ww_im = (amag2 > 0.f) ? (dwi * mua - dwr * kap) / amag2 : (w0i * r->Lmove);
+                r->weight = nre;
+                r->weight_im = nim;
+            }
 
             r->p0.x = r->p0.x + r->vec.x * r->Lmove;
             r->p0.y = r->p0.y + r->vec.y * r->Lmove;
@@ -524,6 +558,10 @@ static float orc_badouel(oray* r, octx* c) {
                 float inv = 1.f / cfg->gridstep;
                 float frac = (totalloss == 0.f) ? 0.f : (1.f - segloss) / totalloss;
                 float w0 = ww;
+                /* RF: the complex weight turns by the same phase in every
+                 * segment (src/mmc_core.cl:1043-1081) */
+                float sr = currweight, si = r->weight_im, scos = 1.f, ssin = 0.f;
+                if (rf) O_SINCOSF(cfg->omega * n * R_C0 * dstep, &ssin, &scos);
                 const int dx = cfg->griddim[0], dxy = cfg->griddim[0] * cfg->griddim[1];
                 for (int i = 0; i < seg; i++) {
                     int ix = (int)(sx * inv), iy = (int)(sy * inv), iz = (int)(sz * inv);
@@ -532,35 +570,66 @@ static float orc_badouel(oray* r, octx* c) {
                     iz = iz < 0 ? 0 : (iz >= cfg->griddim[2] ? cfg->griddim[2] - 1 : iz);
                     unsigned int newidx = (unsigned int)(iz * dxy + iy * dx + ix + tshift);
                     r->oldidx = (r->oldidx == 0xFFFFFFFFu) ? newidx : r->oldidx;
+                    float d = w0 * frac, dim = 0.f;
+                    if (rf) {
+                        float w0r = sr, w0i = si;
+                        float nre = segloss * (w0r * scos + w0i * ssin);
+                        float nim = segloss * (-w0r * ssin + w0i * scos);
+                        sr = nre;
+                        si = nim;
+                        float dwr = w0r - nre, dwi = w0i - nim;
+                        d = (amag2 < EPS) ? (w0r * w0) : (dwr * mua + dwi * kap) / amag2;
+                        dim = (amag2 < EPS) ? (w0i * w0) : (dwi * mua - dwr * kap) / amag2;
+                    }
                     if (newidx != r->oldidx) {
                         add_w(c->weight, r->oldidx, r->oldweight);
+                        if (rf) add_w(c->weight_im, r->oldidx, r->oldweight_im);
                         r->oldidx = newidx;
-                        r->oldweight = w0 * frac;
+                        r->oldweight = d;
+                        r->oldweight_im = dim;
                     } else {
-                        r->oldweight += w0 * frac;
+                        r->oldweight += d;
+                        r->oldweight_im += dim;
                     }
                     if (r->faceid == -2 || !r->isend) {
                         add_w(c->weight, newidx, r->oldweight);
+                        if (rf) add_w(c->weight_im, newidx, r->oldweight_im);
                         r->oldweight = 0.f;
+                        r->oldweight_im = 0.f;
                     }
                     w0 *= segloss;
                     sx = sx + tx;
                     sy = sy + ty;
                     sz = sz + tz;
                 }
+                if (rf) {
+                    r->weight = sr;
+                    r->weight_im = si;
+                }
             } else if (!cfg->basisorder) {
                 unsigned int newidx = (unsigned int)(eid + tshift);
                 r->oldidx = (r->oldidx == 0xFFFFFFFFu) ? newidx : r->oldidx;
                 if (newidx != r->oldidx) {
-                    add_w(c->weight, r->oldidx, r->oldweight);
+                    /* RF: the GPU reference flushes only a register whose real
+                     * part is positive (src/mmc_core.cl:905) */
+                    if (!rf) {
+                        add_w(c->weight, r->oldidx, r->oldweight);
+                    } else if (r->oldweight > 0.0) {
+                        add_w(c->weight, r->oldidx, r->oldweight);
+                        add_w(c->weight_im, r->oldidx, r->oldweight_im);
+                    }
                     r->oldidx = newidx;
                     r->oldweight = ww;
+                    r->oldweight_im = ww_im;
                 } else {
                     r->oldweight += ww;
+                    r->oldweight_im += ww_im;
                 }
                 if (r->faceid == -2 || !r->isend) {
                     add_w(c->weight, newidx, r->oldweight);
+                    if (rf) add_w(c->weight_im, newidx, r->oldweight_im);
                     r->oldweight = 0.f;
+                    r->oldweight_im = 0.f;
                 }
             } else {
                 /* basisorder=1 under BLBadouel: w/3 to the exit-face nodes
@@ -711,7 +780,7 @@ void orc_probe_raytet(const orc_mesh* m, int method, int eid, const float p[3], 
     cfg.maxgate = 1;
     size_t n = (size_t)(cfg.basisorder ? m->nn : m->ne);
     double* scratch = (double*)calloc(n, sizeof(double));
-    octx c = {&cfg, m, scratch, 1.f, (int)n, 0.0};
+    octx c = {&cfg, m, scratch, NULL, 1.f, (int)n, 0.0};
     oray r;
     memset(&r, 0, sizeof(r));
     r.p0.x = p[0]; r.p0.y = p[1]; r.p0.z = p[2];
@@ -920,6 +989,8 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
     if (orc_launch(&r, c, ran)) return -1;
     float w0 = r.weight;
     ppath[reclen - 2] = r.weight;
+    const int rf = cfg->omega > 0.f && !cfg->replayweight;
+    r.wmag = r.weight;
     /* replay with weighted path / scattering outputs: the detected weight
      * is the launched energy (src/mmc_raytrace.c:1815-1817) */
     *launchw += (cfg->outputtype == ORC_OT_WL || cfg->outputtype == ORC_OT_WP) ? c->rw : r.weight;
@@ -1020,12 +1091,18 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
             break;
         }
 
-        if (cfg->minenergy > 0.f && r.weight < cfg->minenergy &&
+        /* RF: the roulette tests the magnitude of the complex weight and
+         * scales both parts (src/mmc_core.cl:2090-2110) */
+        if (cfg->minenergy > 0.f && (rf ? r.wmag : r.weight) < cfg->minenergy &&
             (cfg->tend - cfg->tstart) * c->rtstep <= 1.f) {
             if (orc_rand_next(ran) * cfg->roulettesize <= 1.f) {
-                float wb = r.weight;
+                float wb = rf ? r.wmag : r.weight;
                 r.weight *= cfg->roulettesize;
-                rgain += r.weight - wb;
+                if (rf) {
+                    r.weight_im *= cfg->roulettesize;
+                    r.wmag *= cfg->roulettesize;
+                }
+                rgain += (rf ? r.wmag : r.weight) - wb;
             } else {
                 reason = ORC_END_ROULETTE;
                 break;
@@ -1059,14 +1136,14 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
     if (rec) {
         rec->p[0] = r.p0.x; rec->p[1] = r.p0.y; rec->p[2] = r.p0.z;
         rec->v[0] = r.vec.x; rec->v[1] = r.vec.y; rec->v[2] = r.vec.z;
-        rec->w = (reason == ORC_END_ROULETTE) ? 0.f : r.weight;
+        rec->w = (reason == ORC_END_ROULETTE) ? 0.f : (rf ? r.wmag : r.weight);
         rec->t = r.photontimer;
         rec->eid = r.eid;
         rec->reason = reason;
         rec->nraytet = (int32_t)(c->raytet - raytet0);
         rec->detid = exitdet;
         rec->eabs = (float)r.Eabsorb;
-        rec->rgain = (reason == ORC_END_ROULETTE) ? rgain - r.weight : rgain;
+        rec->rgain = (reason == ORC_END_ROULETTE) ? rgain - (rf ? r.wmag : r.weight) : rgain;
         rec->w0 = w0;
         rec->nscat = nscat;
     }
@@ -1088,6 +1165,7 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
     base.cfg = cfg;
     base.m = mesh;
     base.weight = out->weight;
+    base.weight_im = out->weight_im;
     base.rtstep = 1.f / cfg->tstep;
     base.framelen = (cfg->method == ORC_RT_BLBADOUELGRID)
                         ? cfg->griddim[0] * cfg->griddim[1] * cfg->griddim[2]

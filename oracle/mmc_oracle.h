@@ -94,6 +94,10 @@ typedef struct {
     float gridstep, unitinmm;
     float gridmin[3];
     int griddim[3];
+    /* RF forward (src/mmc_core.cl, IS_RF; GPU-only in the reference):
+     * modulation in rad/s, > 0 for complex weights (Badouel outputs); off
+     * for replays (replayweight given), as in the reference */
+    float omega;
 } orc_cfg;
 
 typedef struct {
@@ -113,6 +117,7 @@ typedef struct {
     orc_photon* rec;         /* reccount records, optional */
     double runtime_ms;
     int error;
+    double* weight_im;       /* RF: imaginary part of the field, or NULL */
 } orc_out;
 
 /* RNG (src/mmc_rand_xorshift128p.c:55-83, src/mmc_rand_common.h:50-51) */

@@ -44,14 +44,14 @@ class OCfg(C.Structure):
                 ("replayweight", f32p), ("replaytime", f32p),
                 ("nthreads", C.c_int), ("recfirst", C.c_int64), ("reccount", C.c_int64),
                 ("gridstep", C.c_float), ("unitinmm", C.c_float), ("gridmin", C.c_float * 3),
-                ("griddim", C.c_int * 3)]
+                ("griddim", C.c_int * 3), ("omega", C.c_float)]
 
 
 class OOut(C.Structure):
     _fields_ = [("weight", f64p), ("launchweight", C.c_double), ("absorbweight", C.c_double),
                 ("raytet", C.c_double), ("detdata", f32p), ("detphoton", i64p), ("maxdet", C.c_int),
                 ("detected", C.c_int), ("reclen", C.c_int), ("rec", C.c_void_p), ("runtime_ms", C.c_double),
-                ("error", C.c_int)]
+                ("error", C.c_int), ("weight_im", f64p)]
 
 
 PHOTON_DTYPE = np.dtype([
@@ -168,6 +168,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     c.method = {"P": 0, "S": 3, "G": 4}[sim.method]
     c.basisorder = 0 if sim.method == "G" else int(sim.basisorder)
     c.unitinmm = float(getattr(sim, "unitinmm", 1.0) or 1.0)
+    c.omega = float(getattr(sim, "omega", 0.0) or 0.0)
     if sim.method == "G":
         c.gridstep = float(sim.gridstep)
         c.gridmin[:], c.griddim[:] = grid(mesh, c.gridstep)
@@ -256,8 +257,11 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     else:
         datalen = mesh.nn if cfg.basisorder else mesh.ne
     field = np.zeros((cfg.maxgate, datalen), np.float64)
+    rf = cfg.omega > 0 and getattr(sim, "replayweight", None) is None   # RF is off for replays
+    field_im = np.zeros_like(field) if rf else None
     out = OOut()
     out.weight = p(field, C.c_double)
+    out.weight_im = p(field_im, C.c_double) if rf else C.cast(None, f64p)
     reclen = 2 * mesh.prop + 6 * cfg.issaveexit + 2
     det = np.zeros((max(maxdet, 1), reclen), np.float32)
     detph = np.zeros(max(maxdet, 1), np.int64)
@@ -276,8 +280,11 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     n = min(out.detected, maxdet)
     r.detected, r.detdata, r.detphoton = out.detected, det[:n], detph[:n]
     r.photons, r.runtime_ms, r.reclen = recs, out.runtime_ms, out.reclen
+    r.field_im = field_im
     r.normalizer = None
     if normalize:
         r.normalizer = L.orc_normalize(C.byref(cfg), C.byref(mesh.c), p(field, C.c_double),
                                        C.c_float(r.absorbweight), C.c_float(r.launchweight))
+        if field_im is not None:
+            field_im *= r.normalizer       # imag_slot *= normalizor (src/mmc_mesh.c:2262-2274)
     return r
