@@ -107,7 +107,7 @@ def bench_plain(args):
                    "steps_per_photon": raytet / photons, "ray_tet_steps_per_ms": raytet / sum(kern_ms)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "mmc::transport<false,false>", "kernel_ms": k_avg,
+                     "kernel": "mmc::transport<0, false, 4, 1>", "kernel_ms": k_avg,
                      "bytes_per_step": BYTES_PER_STEP_ELEM, "steps_per_launch": steps_per_launch},
     }
     res.release()

@@ -305,3 +305,35 @@ def test_replay_outputs_match_the_oracle(slab, out):
     assert gr.launchweight == pytest.approx(o.launchweight, rel=1e-6)
     assert np.linalg.norm(gr.field - o.field) <= 1e-5 * np.linalg.norm(o.field)
     assert abs(gr.normalizer / o.normalizer - 1) < 1e-6
+
+
+def test_invalid_settings_fail_with_a_message(slab):
+    # settings the transport cannot honour are refused before any launch,
+    # with the reason in mmc_hip_last_error (no silent fallback)
+    _, tm, _ = slab
+    cube = M.TetMesh(*M.meshgen.cube60(6.0)).prepare(nodal=True)
+    cases = [
+        (tm, dict(method="G", gridstep=0.0), "voxel edge"),
+        (tm, dict(method="G", gridstep=1e-4), "too large"),
+        (cube, dict(method="P", omega=1e9), "RF"),
+        (tm, dict(omega=-1.0), "omega"),
+        (tm, dict(srctype="pattern", srcparam1=(10, 0, 0, 4), srcparam2=(0, 10, 0, 4),
+                  srcpattern=np.ones((9, 4, 4), np.float32)), "srcnum"),
+        (tm, dict(isreflect=7), "boundary"),
+    ]
+    for mesh, kw, msg in cases:
+        base = dict(nphoton=100, srcpos=(50.0, 50.0, 0.0) if mesh is tm else (30.0, 30.0, 0.0),
+                    srcdir=(0.0, 0.0, 1.0))
+        base.update(kw)
+        with pytest.raises((RuntimeError, ValueError), match=msg):
+            M.run(mesh, M.SimConfig(**base), want_field=False)
+    # a batch cannot mix tracers or modulation frequencies
+    rm = M.ResidentMesh(tm)
+    try:
+        with pytest.raises(RuntimeError, match="ray-tracer"):
+            rm.run_batch([W.slab_sim(100), M.SimConfig(nphoton=100, method="G", srcpos=(50.0, 50.0, 0.0))])
+        with pytest.raises(RuntimeError, match="omega"):
+            rm.run_batch([M.SimConfig(nphoton=100, omega=1e9, srcpos=(50.0, 50.0, 0.0)),
+                          M.SimConfig(nphoton=100, omega=2e9, srcpos=(50.0, 50.0, 0.0))])
+    finally:
+        rm.release()
