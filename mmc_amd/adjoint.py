@@ -1,0 +1,93 @@
+"""Adjoint Jacobians on the dual grid (the reference's -O A / D / W outputs,
+src/mmc_core.cl:2228-2400, src/mmc_cl_host.c:1331-1462).
+
+The adjoint method needs one forward field per source and one per detector,
+the detector run as a source pointing into the medium; ``forward_fields``
+runs them as one batch on a resident mesh and normalizes each, and
+``jacobian_grid`` forms, per voxel,
+
+    J_mua[s][d] = -unitinmm^3 * cw_s * cw_d
+    J_D[s][d]   = -unitinmm * grad(cw_s) . grad(cw_d)
+
+(cw: the field summed over gates; complex products for RF fields) on the
+device through mmc_hip_adjoint_grid.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import f32p, i32p, ptr
+from .mesh import TetMesh
+from .sim import ResidentMesh, SimConfig, normalize_field
+
+
+@dataclasses.dataclass
+class Jacobians:
+    mua: Optional[np.ndarray]          # [s][d][z][y][x]
+    mua_im: Optional[np.ndarray]
+    d: Optional[np.ndarray]
+    d_im: Optional[np.ndarray]
+
+
+def jacobian_grid(field: np.ndarray, field_im: Optional[np.ndarray], ns: int, nd: int, unitinmm: float = 1.0,
+                  mua: bool = True, dcoeff: bool = True) -> Jacobians:
+    """field: [slot][gate][z][y][x] float32 (ns sources, then nd detectors)."""
+    f = np.ascontiguousarray(field, np.float32)
+    if f.ndim != 5 or f.shape[0] != ns + nd:
+        raise ValueError("field must be [ns + nd][gate][z][y][x]")
+    fi = None if field_im is None else np.ascontiguousarray(field_im, np.float32)
+    if fi is not None and fi.shape != f.shape:
+        raise ValueError("field_im must match field")
+    nz, ny, nx = f.shape[2:]
+    shp = (ns, nd, nz, ny, nx)
+    mult = 2 if fi is not None else 1
+    jm = np.zeros((mult,) + shp, np.float32) if mua else None
+    jd = np.zeros((mult,) + shp, np.float32) if dcoeff else None
+    dim = np.array([nx, ny, nz], np.int32)
+    rc = _lib.lib().mmc_hip_adjoint_grid(ptr(f, C.c_float), ptr(fi, C.c_float) if fi is not None else C.cast(None, f32p),
+                                         ptr(dim, C.c_int32), int(f.shape[1]), int(ns), int(nd), C.c_float(unitinmm),
+                                         ptr(jm, C.c_float) if mua else C.cast(None, f32p),
+                                         ptr(jd, C.c_float) if dcoeff else C.cast(None, f32p))
+    _lib.check(rc, "mmc_hip_adjoint_grid")
+    pick = lambda a, k: None if a is None or k >= len(a) else a[k]
+    return Jacobians(pick(jm, 0), pick(jm, 1), pick(jd, 0), pick(jd, 1))
+
+
+def detector_sources(cfg: SimConfig, detpos: Sequence[Sequence[float]],
+                     detdir: Sequence[Sequence[float]]) -> List[SimConfig]:
+    """One pencil simulation per detector, launched from its position along
+    its direction (cfg->detdir), with the settings of cfg."""
+    out = []
+    for p, v in zip(detpos, detdir):
+        out.append(dataclasses.replace(cfg, srctype="pencil", srcpos=tuple(p[:3]), srcdir=tuple(v[:3]), e0=0,
+                                       issavedet=0, detpos=None))
+    return out
+
+
+def forward_fields(mesh: TetMesh, sources: List[SimConfig], detectors: List[SimConfig], tune=None):
+    """Normalized grid fields of the sources then the detectors, as
+    [slot][gate][z][y][x] float32 (and the imaginary parts for RF)."""
+    cfgs = list(sources) + list(detectors)
+    if any(not c.is_grid for c in cfgs):
+        raise ValueError("adjoint Jacobians need dual-grid simulations (method 'G')")
+    rm = ResidentMesh(mesh)
+    try:
+        res = rm.run_batch(cfgs, tune=tune)
+    finally:
+        rm.release()
+    nmin, dim = cfgs[0].grid(mesh)
+    re, im = [], []
+    for c, r in zip(cfgs, res):
+        normalize_field(mesh, c, r)
+        g = c.gates()[0]
+        re.append(r.field.reshape(g, dim[2], dim[1], dim[0]))
+        if r.field_im is not None:
+            im.append(r.field_im.reshape(g, dim[2], dim[1], dim[0]))
+    f = np.ascontiguousarray(np.stack(re), np.float32)
+    fi = np.ascontiguousarray(np.stack(im), np.float32) if im else None
+    return f, fi
