@@ -210,6 +210,16 @@ int32_t mmc_hip_run_batch(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_
                           const mmc_hip_tune* tune, mmc_hip_result* res);
 /* Device bytes the outputs of a simulation occupy (admission accounting). */
 int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
+/* Adjoint Jacobians on the dual grid (the -O A / D / W outputs; mmc_adjoint_kernel and
+ * mmc_adjoint_dcoeff_kernel, src/mmc_core.cl:2228-2400, scaled as src/mmc_cl_host.c:1436-1461).
+ * field: ns source fields then nd detector fields (each detector run as a source), laid out
+ * [slot][gate][voxel] with dim[0]*dim[1]*dim[2] voxels, x fastest (normalized grid fields, as
+ * float); field_im: their imaginary parts (RF) or NULL.  jmua / jd (either may be NULL) receive
+ * [s][d][voxel], followed by the imaginary block for RF:
+ *     J_mua = -unitinmm^3 * cw_s * cw_d,  J_D = -unitinmm * grad(cw_s) . grad(cw_d),
+ * cw = the field summed over gates, gradients by second-order finite differences.  0 on success. */
+int32_t mmc_hip_adjoint_grid(const float* field, const float* field_im, const int32_t dim[3], int32_t maxgate,
+                             int32_t ns, int32_t nd, float unitinmm, float* jmua, float* jd);
 /* Origin and voxel counts of a dual-grid simulation's output on a resident
  * mesh (the grid cfg->dim / mesh->nmin of mesh_createdualmesh,
  * src/mmc_mesh.c:349-381); 0 on success. */

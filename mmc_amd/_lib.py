@@ -100,6 +100,8 @@ SIGNATURES = {
                                       C.POINTER(ResultC)]),
     "mmc_hip_sim_output_bytes": (C.c_int64, [C.c_void_p, C.POINTER(SimC)]),
     "mmc_hip_sim_grid": (C.c_int32, [C.c_void_p, C.POINTER(SimC), f32p, i32p]),
+    "mmc_hip_adjoint_grid": (C.c_int32, [f32p, f32p, i32p, C.c_int32, C.c_int32, C.c_int32, C.c_float, f32p,
+                                         f32p]),
     "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "mmc_hip_arena_release": (C.c_int32, [C.c_void_p]),
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
