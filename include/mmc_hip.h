@@ -220,6 +220,19 @@ int64_t mmc_hip_sim_output_bytes(mmc_hip_mesh_handle h, const mmc_hip_sim* sim);
  * cw = the field summed over gates, gradients by second-order finite differences.  0 on success. */
 int32_t mmc_hip_adjoint_grid(const float* field, const float* field_im, const int32_t dim[3], int32_t maxgate,
                              int32_t ns, int32_t nd, float unitinmm, float* jmua, float* jd);
+/* Adjoint Jacobians on the tetrahedral mesh from nodal fields (basisorder 1;
+ * mmc_adjoint_mesh_full_kernel / mmc_adjoint_mesh_nodal_kernel, src/mmc_core.cl:2403-2649, host
+ * src/mmc_cl_host.c:1117-1320).  field / field_im: [slot][gate][node] (ns sources, then nd
+ * detectors); elem: 1-based node ids, 4 per element; evol: element volumes; deldotdel: per
+ * element the packed <grad phi_i . grad phi_j> * evol of mmc_host_deldotdel, as float (needed for
+ * jd); nvol: node volumes (needed for mode 1).  mode 0: linear-basis integrals per element,
+ * a quarter to each of its nodes (the reference's default); mode 1: J_mua = -nvol * cw_s * cw_d
+ * per node (the reference's adjointmode 1; J_D as in mode 0); mode 2: the per-element values of
+ * mode 0 ([s][d][elem]).  jmua / jd (either may be NULL) receive [s][d][node] (or elem), then the
+ * imaginary block for RF.  0 on success, -1 for bad arguments. */
+int32_t mmc_hip_adjoint_mesh(const float* field, const float* field_im, int32_t nn, int32_t ne,
+                             const int32_t* elem, const float* evol, const float* deldotdel, const float* nvol,
+                             int32_t maxgate, int32_t ns, int32_t nd, int32_t mode, float* jmua, float* jd);
 /* Origin and voxel counts of a dual-grid simulation's output on a resident
  * mesh (the grid cfg->dim / mesh->nmin of mesh_createdualmesh,
  * src/mmc_mesh.c:349-381); 0 on success. */

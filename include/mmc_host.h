@@ -66,6 +66,12 @@ int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb);
  * (int)(extent / step) + 1 voxels per axis.  Returns 0, or -1 for a bad
  * step. */
 int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], int32_t dim[3]);
+/* Per element the products of the linear basis gradients times the element
+ * volume, <grad phi_i . grad phi_j> * evol, upper triangle packed as
+ * [00,01,02,03,11,12,13,22,23,33] into ddd[ne*10] (mesh_deldotdel,
+ * src/mmc_mesh.c:990-1046).  0, or -1 for a node id outside 1..nn. */
+int32_t mmc_host_deldotdel(const float* node, int32_t nn, const int32_t* elem, int32_t ne, const float* evol,
+                           double* ddd);
 /* Normalizes a field in place and returns the factor (mesh_normalize). */
 double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
                           const int32_t* elem, const int32_t* type, const float* med,

@@ -102,6 +102,8 @@ SIGNATURES = {
     "mmc_hip_sim_grid": (C.c_int32, [C.c_void_p, C.POINTER(SimC), f32p, i32p]),
     "mmc_hip_adjoint_grid": (C.c_int32, [f32p, f32p, i32p, C.c_int32, C.c_int32, C.c_int32, C.c_float, f32p,
                                          f32p]),
+    "mmc_hip_adjoint_mesh": (C.c_int32, [f32p, f32p, C.c_int32, C.c_int32, i32p, f32p, f32p, f32p, C.c_int32,
+                                         C.c_int32, C.c_int32, C.c_int32, f32p, f32p]),
     "mmc_hip_arena_create": (C.c_int32, [C.c_uint64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "mmc_hip_arena_release": (C.c_int32, [C.c_void_p]),
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
@@ -124,6 +126,7 @@ SIGNATURES = {
     "mmc_host_number_exterior": (C.c_int32, [C.c_int32, i32p]),
     "mmc_host_detelem": (C.c_int32, [C.c_int32, i32p, C.c_int32]),
     "mmc_host_grid": (C.c_int32, [f32p, C.c_int32, C.c_float, f32p, i32p]),
+    "mmc_host_deldotdel": (C.c_int32, [f32p, C.c_int32, i32p, C.c_int32, f32p, f64p]),
     "mmc_host_normalize_grid": (C.c_double, [C.POINTER(SimC), C.c_int64, f64p, C.c_double, C.c_double,
                                              C.c_int32]),
     "mmc_host_normalize": (C.c_double, [C.POINTER(SimC), C.c_int32, C.c_int32, i32p, i32p, f32p,

@@ -244,6 +244,42 @@ int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop) {
     return n;
 }
 
+/* Gradients of the four linear basis functions of a tetrahedron: phi_i is 0
+ * on the face opposite node i and 1 at node i, so grad(phi_i) is the face's
+ * normal c = (b - a) x (c - a) over its projection onto (p_i - a). */
+static void basis_gradients(const float* node, const int32_t* ee, double g[4][3]) {
+    static const int opp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int i = 0; i < 4; i++) {
+        const float* p = node + 3 * (size_t)(ee[i] - 1);
+        const float* a = node + 3 * (size_t)(ee[opp[i][0]] - 1);
+        const float* b = node + 3 * (size_t)(ee[opp[i][1]] - 1);
+        const float* c = node + 3 * (size_t)(ee[opp[i][2]] - 1);
+        const double u[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+        const double v[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double h = n[0] * ((double)p[0] - a[0]) + n[1] * ((double)p[1] - a[1]) + n[2] * ((double)p[2] - a[2]);
+        for (int k = 0; k < 3; k++) g[i][k] = (h != 0.0) ? n[k] / h : 0.0;
+    }
+}
+
+int32_t mmc_host_deldotdel(const float* node, int32_t nn, const int32_t* elem, int32_t ne, const float* evol,
+                           double* ddd) {
+    if (!node || !elem || !evol || !ddd || nn < 1 || ne < 0) return -1;
+    for (int32_t t = 0; t < ne; t++) {
+        const int32_t* ee = elem + 4 * (size_t)t;
+        for (int k = 0; k < 4; k++)
+            if (ee[k] < 1 || ee[k] > nn) return -1;
+        double g[4][3];
+        basis_gradients(node, ee, g);
+        int idx = 0;
+        for (int i = 0; i < 4; i++)
+            for (int j = i; j < 4; j++)
+                ddd[10 * (size_t)t + idx++] =
+                    (g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2]) * (double)evol[t];
+    }
+    return 0;
+}
+
 int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], int32_t dim[3]) {
     if (!node || nn < 1 || !(step > 0.f) || !nmin || !dim) return -1;
     for (int k = 0; k < 3; k++) {

@@ -8,6 +8,14 @@ J_D = -u (a_s . a_d) at every voxel including the faces; for quadratic
 fields the central and one-sided formulas are exact as well; J_mua of fields
 constant over the grid is -u^3 cw_s cw_d; and an RF product obeys complex
 multiplication.  The GPU entry point must equal the restatement.
+
+Mesh mode (mmc_hip_adjoint_mesh; src/mmc_core.cl:2403-2649): the gradient
+products <grad phi_i . grad phi_j> Ve are pinned on the unit right
+tetrahedron and by zero row sums (the basis functions sum to one), the
+host's cross-product form against the restatement's matrix inverse; the
+element values by the mass matrix Ve/20 (1 + delta_ij) and by exact
+stiffness integrals of linear fields, -Ve (a_s . a_d); the nodal output by
+conservation (the nodes receive the elements' sums).
 """
 import numpy as np
 import pytest
@@ -19,6 +27,14 @@ from mmc_amd import adjoint as MA
 from mmc_amd import meshgen
 
 NX, NY, NZ, G = 7, 5, 4, 3
+
+
+def small_mesh(n=4):
+    node, elem = meshgen.t6_lattice(n, n, n, 1.5)
+    node = node + np.random.default_rng(2).uniform(-0.1, 0.1, node.shape).astype(np.float32)
+    etype = np.ones(len(elem), np.int32)
+    med = np.array([[0, 0, 1, 1], [0.01, 1.0, 0.0, 1.37]], np.float32)
+    return M.TetMesh(node, elem, etype, med).prepare(nodal=True)
 
 
 def linear_fields(slopes, offsets):
@@ -85,4 +101,87 @@ def test_gpu_adjoint_run_on_a_slab():
     assert np.allclose(j.mua, jm, rtol=1e-6, atol=0)
     assert np.all(j.mua <= 0) and j.mua.min() < 0
     # the sensitivity to the nearer detector is larger near the source
+    assert j.mua[0, 0].sum() < j.mua[0, 1].sum()
+
+
+def test_deldotdel_closed_forms():
+    ref = np.array([0.5, -1, -1, -1, 1, 0, 0, 1, 0, 1], np.float64) / np.array([1, 6, 6, 6, 6, 1, 1, 6, 1, 6])
+    node = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    assert np.allclose(A.deldotdel(node, [[1, 2, 3, 4]], [1 / 6]), ref, rtol=1e-12, atol=1e-15)
+    tm = small_mesh()
+    host = MA.deldotdel(tm)
+    assert np.allclose(host, A.deldotdel(tm.node, tm.elem, tm.evol), rtol=1e-9, atol=1e-12)
+    full = np.zeros((tm.ne, 4, 4))
+    k = 0
+    for i in range(4):
+        for j in range(i, 4):
+            full[:, i, j] = full[:, j, i] = host[:, k]
+            k += 1
+    assert np.abs(full.sum(2)).max() <= 1e-9 * np.abs(full).max()
+
+
+def test_mesh_restatement_closed_forms():
+    tm = small_mesh()
+    ddd = A.deldotdel(tm.node, tm.elem, tm.evol).astype(np.float32)
+    x = tm.node.astype(np.float64)
+    a, b = np.array([0.3, -0.2, 0.5]), np.array([-0.1, 0.4, 0.2])
+    f = np.stack([(x @ a + 0.5) / G, (x @ b + 0.25) / G]).astype(np.float32)   # 1 source, 1 detector
+    f = np.repeat(f[:, None], G, axis=1)
+    jm, _, jd, _ = A.mesh_element_values(f, None, tm.elem, tm.evol, ddd, 1, 1)
+    # float32 sums of terms of both signs: cancellation costs a few digits
+    assert np.allclose(jd[0, 0], -tm.evol * np.dot(a, b), rtol=1e-3, atol=1e-7)
+    # mass matrix Ve/20 (1 + delta_ij)
+    rng = np.random.default_rng(3)
+    r = rng.random((2, G, tm.nn)).astype(np.float32)
+    jm, _, _, _ = A.mesh_element_values(r, None, tm.elem, tm.evol, ddd, 1, 1)
+    s, d = A.cw(r)[0][tm.elem - 1].astype(np.float64), A.cw(r)[1][tm.elem - 1].astype(np.float64)
+    want = -tm.evol / 20 * ((s * d).sum(1) + s.sum(1) * d.sum(1))
+    assert np.allclose(jm[0, 0], want, rtol=1e-5)
+    # nodes receive the elements' sums; nodal approximation
+    nod = A.to_nodes(jm, tm.elem, tm.nn)
+    assert nod.sum() == pytest.approx(jm.sum(), rel=1e-5)
+    e0 = tm.elem[0] - 1
+    assert nod[0, 0, e0[0]] == pytest.approx(0.25 * jm[0, 0][(tm.elem - 1 == e0[0]).any(1)].sum(), rel=1e-5)
+    ja, _ = A.mesh_nodal_approx(r, None, tm.nvol, 1, 1)
+    assert np.allclose(ja[0, 0], -tm.nvol * A.cw(r)[0] * A.cw(r)[1], rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rf", [False, True])
+@pytest.mark.parametrize("mode", ["fem", "nodal", "elem"])
+def test_gpu_mesh_adjoint_equals_the_restatement(mode, rf):
+    tm = small_mesh()
+    ns, nd = 2, 2
+    rng = np.random.default_rng(11)
+    f = rng.random((ns + nd, G, tm.nn), np.float32)
+    fi = rng.random(f.shape, np.float32) if rf else None
+    j = MA.jacobian_mesh(tm, f, fi, ns, nd, mode=mode)
+    ddd = MA.deldotdel(tm).astype(np.float32)
+    em, emi, ed, edi = A.mesh_element_values(f, fi, tm.elem, tm.evol, ddd, ns, nd)
+    if mode == "elem":
+        want = (em, emi, ed, edi)
+    else:
+        nodes = lambda v: None if v is None else A.to_nodes(v, tm.elem, tm.nn)
+        want = [nodes(em), nodes(emi), nodes(ed), nodes(edi)]
+        if mode == "nodal":
+            want[0], want[1] = A.mesh_nodal_approx(f, fi, tm.nvol, ns, nd)
+    for got, w in zip((j.mua, j.mua_im, j.d, j.d_im), want):
+        assert (got is None) == (w is None)
+        if w is not None:
+            assert np.allclose(got, w, rtol=1e-6, atol=1e-6 * np.abs(w).max())
+
+
+@pytest.mark.gpu
+def test_gpu_mesh_adjoint_run_on_a_cube():
+    node, elem = meshgen.t6_lattice(20, 20, 20, 3.0)
+    etype = np.ones(len(elem), np.int32)
+    med = np.array([[0, 0, 1, 1], [0.005, 1.0, 0.01, 1.37]], np.float32)
+    tm = M.TetMesh(node, elem, etype, med).prepare(nodal=True)
+    src = M.SimConfig(nphoton=200000, seed=5, method="P", srcpos=(24.5, 30.5, 0.0), srcdir=(0.0, 0.0, 1.0),
+                      tstart=0.0, tend=5e-9, tstep=1e-9, outputtype="f")
+    dets = MA.detector_sources(src, [(30.5, 30.5, 0.0), (36.5, 30.5, 0.0)], [(0, 0, 1), (0, 0, 1)])
+    f, fi = MA.forward_fields(tm, [src], dets)
+    assert fi is None and f.shape == (3, 5, tm.nn)
+    j = MA.jacobian_mesh(tm, f, None, 1, 2)
+    assert np.all(j.mua <= 0) and j.mua.min() < 0
     assert j.mua[0, 0].sum() < j.mua[0, 1].sum()
