@@ -175,5 +175,8 @@ MMC_MATHFN float mmc_acosf(float x) {
  * is exact; the product is rounded to 53 bits and then to 24, both to nearest
  * even, on the CPU and the GPU alike. */
 MMC_MATHFN float mmc_twopi(float u) { return (float)(6.283185307179586 * (double)u); }
+/* (float)(pi*(double)u), the arcsine source's zenith angle
+ * (src/mmc_raytrace.c:2478). */
+MMC_MATHFN float mmc_pi(float u) { return (float)(3.141592653589793 * (double)u); }
 
 #endif

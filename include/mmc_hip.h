@@ -33,9 +33,24 @@ extern "C" {
  * box (src/mmc_raytrace.c:1615-1673, src/mmc_mesh.c:349-381) */
 #define MMC_RT_BLBADOUELGRID 4
 /* source types (enum TSrcType, src/mmc_utils.h:101) */
-#define MMC_SRC_PENCIL   0
-#define MMC_SRC_PLANAR   4
-#define MMC_SRC_PATTERN  5
+#define MMC_SRC_PENCIL      0
+#define MMC_SRC_ISOTROPIC   1
+#define MMC_SRC_CONE        2
+#define MMC_SRC_GAUSSIAN    3
+#define MMC_SRC_PLANAR      4
+#define MMC_SRC_PATTERN     5
+#define MMC_SRC_FOURIER     6
+#define MMC_SRC_ARCSINE     7
+#define MMC_SRC_DISK        8
+#define MMC_SRC_FOURIERX    9
+#define MMC_SRC_FOURIERX2D  10
+#define MMC_SRC_ZGAUSSIAN   11
+#define MMC_SRC_LINE        12
+#define MMC_SRC_SLIT        13
+#define MMC_SRC_PENCILARRAY 14
+#define MMC_SRC_PATTERN3D   15
+#define MMC_SRC_HYPERBOLOID 16
+#define MMC_SRC_RING        17
 /* output types (enum TOutputType, src/mmc_utils.h:106) */
 #define MMC_OUT_FLUX     0
 #define MMC_OUT_FLUENCE  1

@@ -59,6 +59,11 @@ void mmc_host_surface_nvol(int32_t nn, int32_t ne, const int32_t* elem, const in
  * mesh_srcdetelem + mesh_loadmedia do (src/mmc_mesh.c:390-427, 511-527);
  * returns their count (mmc_hip_mesh.isextdet = count > 0). */
 int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop);
+/* Source elements (label -1): writes their 1-based ids in index order to
+ * srcelem (if not NULL; ne entries suffice), relabels them 0 and returns their
+ * count -- the candidate list of area sources' launch search
+ * (mesh_srcdetelem, src/mmc_mesh.c:390-427). */
+int32_t mmc_host_srcelem(int32_t ne, int32_t* type, int32_t* srcelem);
 /* Numbers exterior faces -1,-2,... in element-major order; returns the count. */
 int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb);
 /* Dual-grid extent (mesh_createdualmesh, src/mmc_mesh.c:349-381): the

@@ -46,6 +46,7 @@ typedef struct {
     int nmed;
     float detpos[4 * MAXDET];
     float* pattern;
+    size_t npattern;
     int gpuid;
     /* replay (-E file.mch) and seed saving (-q 1) */
     char seedfile[1024];
@@ -122,7 +123,11 @@ static int keyidx(const char* s, const char* const* table) {
     return -1;
 }
 
-static const char* const kSrcTypes[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern", ""};
+/* enum TSrcType names (src/mmc_utils.c:178-180, then the four ids that
+ * only the MATLAB/Python front ends name) */
+static const char* const kSrcTypes[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier",
+                                        "arcsine", "disk", "fourierx", "fourierx2d", "zgaussian", "line", "slit",
+                                        "pencilarray", "pattern3d", "hyperboloid", "ring", ""};
 
 /* mcx_loadjson subset (src/mmc_utils.c:1566-2392) */
 static void load_json(cli_cfg* c, const char* path) {
@@ -206,6 +211,7 @@ static void load_json(cli_cfg* c, const char* path) {
                 int n = mj_size(data);
                 c->pattern = (float*)calloc((size_t)(n ? n : 1), sizeof(float));
                 vec_of(data, c->pattern, n);
+                c->npattern = (size_t)n;
                 c->sim.srcpattern = c->pattern;
             }
         }
@@ -332,6 +338,7 @@ static void load_inp(cli_cfg* c, const char* path) {
                 }
                 if (!pf) die("%s", "source pattern file can not be opened");
                 if (fread(c->pattern, sizeof(float), n, pf) != n) die("%s", "source pattern file is too short");
+                c->npattern = n;
                 fclose(pf);
                 c->sim.srcpattern = c->pattern;
             }
@@ -348,7 +355,6 @@ static void load_inp(cli_cfg* c, const char* path) {
 /* mcx_savejdata (src/mmc_utils.c:2710-2850): the settings as a JSON input
  * with the mesh inline, readable by -f; '-' writes to stdout */
 static void dump_json(const cli_cfg* c) {
-    static const char* const names[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern"};
     static const char otype[] = "xfejlp";
     FILE* fp = (!c->jsonfile[0] || !strcmp(c->jsonfile, "-")) ? stdout : fopen(c->jsonfile, "wt");
     if (!fp) die("can not write %s", c->jsonfile);
@@ -382,15 +388,15 @@ static void dump_json(const cli_cfg* c) {
         fprintf(fp, "%s\n\t\t\t[%d, %d, %d, %d, %d]", i ? "," : "", e[0], e[1], e[2], e[3], e[4]);
     }
     fprintf(fp, "\n\t\t]\n\t},\n\t\"Optode\": {\n\t\t\"Source\": {\n\t\t\t\"Type\": \"%s\",\n",
-            names[s->srctype >= 0 && s->srctype <= 5 ? s->srctype : 0]);
+            kSrcTypes[s->srctype >= MMC_SRC_PENCIL && s->srctype <= MMC_SRC_RING ? s->srctype : 0]);
     fprintf(fp, "\t\t\t\"Pos\": [%.9g, %.9g, %.9g],\n\t\t\t\"Dir\": [%.9g, %.9g, %.9g, %.9g],\n",
             (double)s->srcpos[0], (double)s->srcpos[1], (double)s->srcpos[2], (double)s->srcdir[0],
             (double)s->srcdir[1], (double)s->srcdir[2], (double)s->srcdir[3]);
     fprintf(fp, "\t\t\t\"Param1\": [%.9g, %.9g, %.9g, %.9g],\n\t\t\t\"Param2\": [%.9g, %.9g, %.9g, %.9g]",
             (double)s->srcparam1[0], (double)s->srcparam1[1], (double)s->srcparam1[2], (double)s->srcparam1[3],
             (double)s->srcparam2[0], (double)s->srcparam2[1], (double)s->srcparam2[2], (double)s->srcparam2[3]);
-    if (s->srctype == MMC_SRC_PATTERN && c->pattern) {
-        size_t n = (size_t)(s->srcparam1[3] * s->srcparam2[3]) * (size_t)(s->srcnum > 0 ? s->srcnum : 1);
+    if ((s->srctype == MMC_SRC_PATTERN || s->srctype == MMC_SRC_PATTERN3D) && c->pattern) {
+        size_t n = c->npattern;
         fprintf(fp, ",\n\t\t\t\"Pattern\": [");
         for (size_t i = 0; i < n; i++) fprintf(fp, "%s%.9g", i ? ", " : "", (double)c->pattern[i]);
         fprintf(fp, "]");
@@ -684,6 +690,23 @@ int main(int argc, char** argv) {
         c.med = (float*)realloc(c.med, sizeof(float) * 4 * (size_t)(c.nmed + 1));
         memcpy(c.med + 4 * c.nmed, c.med, 4 * sizeof(float));
     }
+    /* source elements (label -1): the launch search's candidates, relabelled
+     * 0; the first is the default initial element (src/mmc_mesh.c:390-427) */
+    int32_t* srcelem = (int32_t*)malloc(sizeof(int32_t) * (size_t)(c.ne > 0 ? c.ne : 1));
+    const int nsrcelem = mmc_host_srcelem(c.ne, type, srcelem);
+    if (c.sim.e0 <= 0 && nsrcelem > 0) c.sim.e0 = srcelem[0];
+    /* pattern sizes the kernel will read */
+    if (c.sim.srctype == MMC_SRC_PATTERN || c.sim.srctype == MMC_SRC_PATTERN3D) {
+        const float* p1 = c.sim.srcparam1;
+        const double need = c.sim.srctype == MMC_SRC_PATTERN
+                                ? (double)(int)p1[3] * (int)c.sim.srcparam2[3] * (c.sim.srcnum > 0 ? c.sim.srcnum : 1)
+                                : (double)(int)p1[0] * (int)p1[1] * (int)p1[2];
+        if (!c.pattern || need < 1 || need > (double)c.npattern) {
+            char msg[96];
+            snprintf(msg, sizeof(msg), "%.0f values, %zu given", need, c.npattern);
+            die("the source pattern needs %s", msg);
+        }
+    }
     if (c.sim.issavedet && c.sim.detnum == 0 && !isextdet) c.sim.issavedet = 0;
     if (!c.sim.issavedet) c.sim.issaveexit = 0;
     for (int i = 0; i < c.ne; i++)
@@ -698,14 +721,22 @@ int main(int argc, char** argv) {
         mmc_host_surface_nvol(c.nn, c.ne, elem, type, facenb, c.med, prop, c.sim.isreflect, nvol);
     int nf = mmc_host_number_exterior(c.ne, facenb);
     mmc_host_planes(c.ne, c.node, elem, planes);
-    if (c.sim.srctype == MMC_SRC_PENCIL) {
+    const int st = c.sim.srctype;
+    if (st == MMC_SRC_PENCIL || st == MMC_SRC_ISOTROPIC || st == MMC_SRC_CONE || st == MMC_SRC_ARCSINE) {
+        /* launched at srcpos: the enclosing element and barycentric weights */
         if (c.sim.e0 <= 0 || c.sim.e0 > c.ne ||
             mmc_host_barycentric(c.node, elem, c.sim.e0, c.sim.srcpos, c.sim.bary0)) {
             c.sim.e0 = mmc_host_initelem(c.ne, c.node, elem, c.sim.srcpos, c.sim.bary0);
             if (c.sim.e0 <= 0) die("%s", "initial element does not enclose the source!");
         }
+    } else if (c.sim.e0 <= 0 || c.sim.e0 > c.ne) {
+        /* area sources search e0 first, then the source elements */
+        float b[4];
+        c.sim.e0 = mmc_host_initelem(c.ne, c.node, elem, c.sim.srcpos, b);
+        if (c.sim.e0 <= 0) c.sim.e0 = 1;
     }
-    mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med, NULL, 0, isextdet};
+    mmc_hip_mesh mesh = {c.nn, c.ne, prop, nf, c.node, elem, type, facenb, planes, c.med,
+                         nsrcelem ? srcelem : NULL, nsrcelem, isextdet};
     int datalen = c.sim.basisorder ? c.nn : c.ne;
     float gridmin[3] = {0.f, 0.f, 0.f};
     int32_t griddim[3] = {0, 0, 0};
@@ -854,6 +885,7 @@ int main(int argc, char** argv) {
     free(c.rtime);
     free(elem);
     free(type);
+    free(srcelem);
     free(evol);
     free(nvol);
     free(facenb);

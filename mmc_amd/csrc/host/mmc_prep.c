@@ -234,6 +234,17 @@ int32_t mmc_host_number_exterior(int32_t ne, int32_t* facenb) {
     return nf;
 }
 
+int32_t mmc_host_srcelem(int32_t ne, int32_t* type, int32_t* srcelem) {
+    int32_t n = 0;
+    for (int32_t i = 0; i < ne; i++)
+        if (type[i] == -1) {
+            if (srcelem) srcelem[n] = i + 1;
+            type[i] = 0;
+            n++;
+        }
+    return n;
+}
+
 int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop) {
     int32_t n = 0;
     for (int32_t i = 0; i < ne; i++)

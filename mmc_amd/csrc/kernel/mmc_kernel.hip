@@ -80,6 +80,249 @@ __device__ __forceinline__ float rng_scatlen(uint64_t& a, uint64_t& b) {
     return -mmc_logf(rng_next(a, b) + EPSF);
 }
 
+/* rotatevector (src/mmc_mesh.h:336-357): turn (vx, vy, vz) by the zenith
+ * (st, ct) and azimuth (sp, cp), then renormalize */
+__device__ __forceinline__ void rotate_dir(float& vx, float& vy, float& vz, float st, float ct, float sp, float cp) {
+    float qx, qy, qz;
+    if (vz > -1.f + EPSF && vz < 1.f - EPSF) {
+        const float t0 = 1.f - vz * vz;
+        const float t1 = st * (1.f / sqrtf(t0));
+        qx = t1 * (vx * vz * cp - vy * sp) + vx * ct;
+        qy = t1 * (vy * vz * cp + vx * sp) + vy * ct;
+        qz = -t1 * t0 * cp + vz * ct;
+    } else {
+        qx = st * cp;
+        qy = st * sp;
+        qz = (vz > 0.f) ? ct : -ct;
+    }
+    const float n = 1.f / sqrtf(qx * qx + qy * qy + qz * qz);
+    vx = qx * n;
+    vy = qy * n;
+    vz = qz * n;
+}
+
+constexpr float kJustBelowOne = 0.9998f;   /* pattern index clamp, src/mmc_raytrace.c:54 */
+constexpr int kConeTries = 1 << 16;        /* bound of the cone source's rejection loop */
+
+/* launchphoton (src/mmc_raytrace.c:2346-2592): start position, direction and
+ * weight of a photon for every source type, from the photon's stream after
+ * its first scattering length.  Returns true when the start moved off the
+ * source point, so the enclosing element has to be searched (after the EPS
+ * nudge along the direction); pencil, cone, isotropic and arcsine sources keep
+ * the given element and barycentric weights.  posidx: the pattern pixel. */
+__device__ __forceinline__ bool launch_source(const SimCold& C, const float* pattern, uint64_t& r0, uint64_t& r1,
+                                              float& px, float& py, float& pz, float& vx, float& vy, float& vz,
+                                              float& w, int& posidx) {
+    const int st = C.srctype;
+    const float4 P1 = C.srcparam1, P2 = C.srcparam2, S = C.srcpos, D = C.srcdir;
+    float ox = S.x, oy = S.y, oz = S.z;      /* focus origin */
+    bool canfocus = true;
+    if (st == MMC_SRC_PENCIL) return false;
+    if (st == MMC_SRC_PLANAR || st == MMC_SRC_PATTERN || st == MMC_SRC_PATTERN3D || st == MMC_SRC_FOURIER ||
+        st == MMC_SRC_PENCILARRAY) {
+        const float rx = rng_next(r0, r1);
+        const float ry = rng_next(r0, r1);
+        if (st == MMC_SRC_PATTERN3D) {
+            const float rz = rng_next(r0, r1);
+            px = S.x + rx * P1.x;
+            py = S.y + ry * P1.y;
+            pz = S.z + rz * P1.z;
+            const int nx = (int)P1.x, ny = (int)P1.y, nz = (int)P1.z;
+            const int iz = min((int)(rz * kJustBelowOne * (float)nz), nz - 1);
+            const int iy = min((int)(ry * kJustBelowOne * (float)ny), ny - 1);
+            const int ix = min((int)(rx * kJustBelowOne * (float)nx), nx - 1);
+            w = (C.srcnum > 1) ? 1.f : pattern[C.patbase + (iz * ny + iy) * nx + ix];
+            ox += P1.x * 0.5f;
+            oy += P1.y * 0.5f;
+            oz += P1.z * 0.5f;
+        } else if (st == MMC_SRC_PENCILARRAY) {
+            const float fx = floorf(rx * P1.w), fy = floorf(ry * P2.w);
+            px = S.x + fx * P1.x / (P1.w - 1.f) + fy * P2.x / (P2.w - 1.f);
+            py = S.y + fx * P1.y / (P1.w - 1.f) + fy * P2.y / (P2.w - 1.f);
+            pz = S.z + fx * P1.z / (P1.w - 1.f) + fy * P2.z / (P2.w - 1.f);
+            ox += (P1.x + P2.x) * 0.5f;
+            oy += (P1.y + P2.y) * 0.5f;
+            oz += (P1.z + P2.z) * 0.5f;
+        } else {
+            px = S.x + rx * P1.x + ry * P2.x;
+            py = S.y + rx * P1.y + ry * P2.y;
+            pz = S.z + rx * P1.z + ry * P2.z;
+            if (st == MMC_SRC_PATTERN) {
+                const int xs = C.patnx, ys = C.patny;
+                const int iy = min((int)(ry * ys), ys - 1);
+                const int ix = min((int)(rx * xs), xs - 1);
+                posidx = iy * xs + ix;
+                w = (C.srcnum == 1) ? pattern[C.patbase + posidx] : 1.f;
+            } else if (st == MMC_SRC_FOURIER) {
+                const float kx = floorf(P1.w), ky = floorf(P2.w);
+                float sn, cs;
+                mmc_sincosf(mmc_twopi(kx * rx + ky * ry + P1.w - kx), &sn, &cs);
+                w = (cs * (1.f - P2.w + ky) + 1.f) * 0.5f;
+            }
+            ox += (P1.x + P2.x) * 0.5f;
+            oy += (P1.y + P2.y) * 0.5f;
+            oz += (P1.z + P2.z) * 0.5f;
+        }
+    } else if (st == MMC_SRC_FOURIERX || st == MMC_SRC_FOURIERX2D) {
+        const float rx = rng_next(r0, r1);
+        const float ry = rng_next(r0, r1);
+        const float vw = P1.w * (1.f / sqrtf(P1.x * P1.x + P1.y * P1.y + P1.z * P1.z));
+        const float ux = vw * (D.y * P1.z - D.z * P1.y);
+        const float uy = vw * (D.z * P1.x - D.x * P1.z);
+        const float uz = vw * (D.x * P1.y - D.y * P1.x);
+        px = S.x + rx * P1.x + ry * ux;
+        py = S.y + rx * P1.y + ry * uy;
+        pz = S.z + rx * P1.z + ry * uz;
+        if (st == MMC_SRC_FOURIERX2D) {
+            float sa, ca, sb, cb;
+            mmc_sincosf(mmc_twopi(P2.x * rx + P2.z), &sa, &ca);
+            mmc_sincosf(mmc_twopi(P2.y * ry + P2.w), &sb, &cb);
+            w = (sa * sb + 1.f) * 0.5f;
+        } else {
+            float sn, cs;
+            mmc_sincosf(mmc_twopi(P2.x * rx + P2.y * ry + P2.z), &sn, &cs);
+            w = (cs * (1.f - P2.w) + 1.f) * 0.5f;
+        }
+        ox += (P1.x + ux) * 0.5f;
+        oy += (P1.y + uy) * 0.5f;
+        oz += (P1.z + uz) * 0.5f;
+    } else if (st == MMC_SRC_DISK || st == MMC_SRC_RING || st == MMC_SRC_GAUSSIAN) {
+        float phi;
+        if (st == MMC_SRC_RING && (P1.z > 0.f || P1.w > 0.f))
+            phi = fabsf(P1.z - P1.w) * rng_next(r0, r1) + fminf(P1.z, P1.w);
+        else
+            phi = mmc_twopi(rng_next(r0, r1));
+        float sphi, cphi;
+        mmc_sincosf(phi, &sphi, &cphi);
+        float rr;
+        if (st != MMC_SRC_GAUSSIAN) {
+            rr = sqrtf(rng_next(r0, r1) * fabsf(P1.x * P1.x - P1.y * P1.y) + P1.y * P1.y);
+        } else if (fabsf(D.w) < 1e-5f || fabsf(P1.y) < 1e-5f) {
+            rr = sqrtf(-mmc_logf(rng_next(r0, r1))) * P1.x;
+        } else {
+            const float z0 = (float)((double)(P1.x * P1.x) * 3.141592653589793 / (double)P1.y);  /* Rayleigh range */
+            rr = sqrtf(-mmc_logf(rng_next(r0, r1)) * (1.f + (D.w * D.w / (z0 * z0)))) * P1.x;
+        }
+        if (D.z > -1.f + EPSF && D.z < 1.f - EPSF) {
+            const float t0 = 1.f - D.z * D.z;
+            const float t1 = rr / sqrtf(t0);
+            px = S.x + t1 * (D.x * D.z * cphi - D.y * sphi);
+            py = S.y + t1 * (D.y * D.z * cphi + D.x * sphi);
+            pz = S.z - t1 * t0 * cphi;
+        } else {
+            px = S.x + rr * cphi;
+            py = S.y + rr * sphi;
+        }
+    } else if (st == MMC_SRC_CONE || st == MMC_SRC_ISOTROPIC || st == MMC_SRC_ARCSINE) {
+        float sphi, cphi;
+        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
+        float ang;
+        if (st == MMC_SRC_CONE) {
+            int k = 0;
+            do {
+                ang = (P1.y > 0.f) ? mmc_twopi(rng_next(r0, r1)) : mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
+            } while (ang > P1.x && ++k < kConeTries);
+            ang = (ang > P1.x) ? P1.x : ang;
+        } else if (st == MMC_SRC_ISOTROPIC) {
+            ang = mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
+        } else {
+            ang = mmc_pi(rng_next(r0, r1));
+        }
+        float sth, cth;
+        mmc_sincosf(ang, &sth, &cth);
+        rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
+        return false;
+    } else if (st == MMC_SRC_ZGAUSSIAN) {
+        float sphi, cphi;
+        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
+        const float u1 = rng_next(r0, r1);
+        const float u2 = rng_next(r0, r1);
+        const float ang = sqrtf(-2.f * mmc_logf(u1)) * (1.f - 2.f * u2) * P1.x;
+        float sth, cth;
+        mmc_sincosf(ang, &sth, &cth);
+        rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
+        canfocus = false;
+    } else if (st == MMC_SRC_HYPERBOLOID) {
+        float sphi, cphi;
+        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
+        const float rr = sqrtf(0.5f * rng_scatlen(r0, r1)) * P1.x;   /* waist radius */
+        const float tilt = (P1.z > 0.f) ? (-P1.y / P1.z) : 0.f;
+        float bx = -rr * sphi, by = rr * cphi, bz = P1.z;
+        const float nb = 1.f / sqrtf(bx * bx + by * by + bz * bz);
+        bx *= nb;
+        by *= nb;
+        bz *= nb;
+        const float dx = rr * (cphi - tilt * sphi), dy = rr * (sphi + tilt * cphi);
+        if (D.z > -1.f + EPSF && D.z < 1.f - EPSF) {
+            const float sth = sqrtf(1.f - D.z * D.z);
+            const float ist = 1.f / sth;
+            const float cpb = D.x * ist, spb = D.y * ist;
+            px = S.x + dx * cpb * D.z - dy * spb;
+            py = S.y + dx * spb * D.z + dy * cpb;
+            pz = S.z - dx * sth;
+            vx = bx * cpb * D.z - by * spb + bz * cpb * sth;
+            vy = bx * spb * D.z + by * cpb + bz * spb * sth;
+            vz = -bx * sth + bz * D.z;
+        } else {
+            px = S.x + dx;
+            py = S.y + dy;
+            pz = S.z;
+            vx = bx;
+            vy = by;
+            vz = (D.z > 0.f) ? bz : -bz;
+        }
+        const float nv = 1.f / sqrtf(vx * vx + vy * vy + vz * vz);
+        vx *= nv;
+        vy *= nv;
+        vz *= nv;
+        canfocus = false;
+    } else if (st == MMC_SRC_LINE || st == MMC_SRC_SLIT) {
+        const float t = rng_next(r0, r1);
+        px = S.x + t * P1.x;
+        py = S.y + t * P1.y;
+        pz = S.z + t * P1.z;
+        if (st == MMC_SRC_LINE) {
+            /* a direction across the line; normalized here (the reference
+             * keeps the unnormalized cross product, which scales path lengths) */
+            const float tt = 1.f - 2.f * rng_next(r0, r1);
+            const float ss = 1.f - 2.f * rng_next(r0, r1);
+            const float pp = sqrtf(fmaxf(1.f - vx * vx - vy * vy, 0.f)) * (rng_next(r0, r1) > 0.5f ? 1.f : -1.f);
+            const float qx = vy * pp - vz * ss, qy = vz * tt - vx * pp, qz = vx * ss - vy * tt;
+            const float n2 = qx * qx + qy * qy + qz * qz;
+            if (n2 > 0.f) {
+                const float n = 1.f / sqrtf(n2);
+                vx = qx * n;
+                vy = qy * n;
+                vz = qz * n;
+            }
+        }
+        ox += P1.x * 0.5f;
+        oy += P1.y * 0.5f;
+        oz += P1.z * 0.5f;
+        canfocus = (st == MMC_SRC_SLIT);
+    }
+    if (canfocus && D.w != 0.f) {
+        /* converging (focus > 0) or diverging (focus < 0) beam */
+        ox += D.w * vx;
+        oy += D.w * vy;
+        oz += D.w * vz;
+        if (D.w < 0.f) {
+            vx = px - ox;
+            vy = py - oy;
+            vz = pz - oz;
+        } else {
+            vx = ox - px;
+            vy = oy - py;
+            vz = oz - pz;
+        }
+        const float rn = 1.f / sqrtf(vx * vx + vy * vy + vz * vz);
+        vx = vx * rn;
+        vy = vy * rn;
+        vz = vz * rn;
+    }
+    return true;
+}
+
 /* SSE4.1 dpps over xyz: (ax*bx + ay*by) + (az*bz + 0) */
 __device__ __forceinline__ float dpps(float ax, float ay, float az, float bx, float by, float bz) {
     float t2 = ax * bx + ay * by;
@@ -355,25 +598,14 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     eabs = 0.f; eabs_c = 0.f;
                     raytet_start = raytet;
                     slen = rng_scatlen(r0, r1);
-                    if (C.srctype == MMC_SRC_PLANAR || C.srctype == MMC_SRC_PATTERN) {
-                        float rx = rng_next(r0, r1);
-                        float ry = rng_next(r0, r1);
-                        px = C.srcpos.x + rx * C.srcparam1.x + ry * C.srcparam2.x;
-                        py = C.srcpos.y + rx * C.srcparam1.y + ry * C.srcparam2.y;
-                        pz = C.srcpos.z + rx * C.srcparam1.z + ry * C.srcparam2.z;
-                        if (C.srctype == MMC_SRC_PATTERN) {
-                            int xs = C.patnx, ys = C.patny;
-                            int iy = min((int)(ry * ys), ys - 1);
-                            int ix = min((int)(rx * xs), xs - 1);
-                            posidx = iy * xs + ix;
-                            w = (C.srcnum == 1) ? a.pattern[C.patbase + posidx] : 1.f;
-                            if (nsrc > 1)
-                                for (int k = 0; k < nsrc; k++) {
-                                    const float pk = a.pattern[C.patbase + posidx * nsrc + k];
-                                    *LS(slot_pw + k) = pk;
-                                    *LS(slot_pl + k) += pk;
-                                }
+                    const bool search = launch_source(C, a.pattern, r0, r1, px, py, pz, vx, vy, vz, w, posidx);
+                    if (C.srctype == MMC_SRC_PATTERN && nsrc > 1)
+                        for (int k = 0; k < nsrc; k++) {
+                            const float pk = a.pattern[C.patbase + posidx * nsrc + k];
+                            *LS(slot_pw + k) = pk;
+                            *LS(slot_pl + k) += pk;
                         }
+                    if (search) {
                         px = EPSF * vx + 1.f * px;
                         py = EPSF * vy + 1.f * py;
                         pz = EPSF * vz + 1.f * pz;

@@ -371,8 +371,25 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
     } else if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1)) {
         return set_err(-1, "supported pairs: Plucker with nodal output, branch-less Badouel with element output");
     }
-    if (s->srctype != MMC_SRC_PENCIL && s->srctype != MMC_SRC_PLANAR && s->srctype != MMC_SRC_PATTERN)
+    if (s->srctype < MMC_SRC_PENCIL || s->srctype > MMC_SRC_RING)
         return set_err(-1, "unsupported source type %d", s->srctype);
+    if (s->srctype == MMC_SRC_PATTERN3D) {
+        const float* p1 = s->srcparam1;
+        if (!s->srcpattern || !(p1[0] >= 1.f && p1[1] >= 1.f && p1[2] >= 1.f) ||
+            (double)p1[0] * p1[1] * p1[2] > 2e9)
+            return set_err(-1, "3-D pattern source needs a pattern of srcparam1.x*y*z (>= 1 each) voxels");
+    }
+    if (s->srctype == MMC_SRC_PENCILARRAY && !(s->srcparam1[3] >= 2.f && s->srcparam2[3] >= 2.f))
+        return set_err(-1, "pencil-array source needs at least 2 beams along each axis (srcparam1.w, srcparam2.w)");
+    if (s->srctype == MMC_SRC_CONE) {
+        /* the half angle's rejection loop accepts with probability
+         * (1 - cos a)/2, or a/(2 pi) for srcparam1.y > 0; keep it >= 1e-3 */
+        const double ha = s->srcparam1[0];
+        const double acc = s->srcparam1[1] > 0.f ? ha / 6.283185307179586 : (1.0 - cos(ha)) * 0.5;
+        if (!(acc >= 1e-3))
+            return set_err(-1, "cone source half angle %g too small (needs >= %s)", ha,
+                           s->srcparam1[1] > 0.f ? "0.00629" : "0.0633 rad");
+    }
     if (s->e0 < 1 || s->e0 > m->ne) return set_err(-1, "initial element %d outside 1..ne", s->e0);
     if (s->maxgate < 1 || !(s->tstep > 0.f) || !(s->tend > s->tstart))
         return set_err(-1, "bad time gates");
@@ -630,6 +647,12 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             c.patbase = (int)pats.size();
             h.patbase = c.patbase;
             pats.insert(pats.end(), s.srcpattern, s.srcpattern + (size_t)c.patnx * c.patny * c.srcnum);
+        } else if (s.srctype == MMC_SRC_PATTERN3D) {
+            c.patnx = (int)s.srcparam1[0];
+            c.patny = (int)s.srcparam1[1];
+            c.patbase = (int)pats.size();
+            h.patbase = c.patbase;
+            pats.insert(pats.end(), s.srcpattern, s.srcpattern + (size_t)c.patnx * c.patny * (int)s.srcparam1[2]);
         }
         c.nphoton = s.nphoton;
         c.photon_first = s.photon_offset;

@@ -27,6 +27,14 @@ class TetMesh:
         self.prop = len(self.med) - 1
         if self.elem.min() < 1 or self.elem.max() > self.nn:
             raise ValueError("element node ids outside 1..nn")
+        # source elements (label -1): candidates of the area sources' launch
+        # search, relabelled 0 (mmc_host_srcelem; src/mmc_mesh.c:390-427)
+        if (self.etype == -1).any():
+            self.etype = self.etype.copy()
+            ids = np.zeros(self.ne, np.int32)
+            n = _lib.lib().mmc_host_srcelem(self.ne, ptr(self.etype, _lib.C.c_int32), ptr(ids, _lib.C.c_int32))
+            if self.srcelem is None:
+                self.srcelem = ids[:n].copy()
         # wide-field detector elements: label -2 becomes prop+1 with a copy of
         # the background medium (mmc_host_detelem; src/mmc_mesh.c:390-427, 511-527)
         self.etype = self.etype.copy()

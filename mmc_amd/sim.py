@@ -21,7 +21,11 @@ from .mesh import TetMesh
 METHODS = {"P": 0, "S": 3, "G": 4, "plucker": 0, "blbadouel": 3, "grid": 4}
 OUTPUTS = {"x": 0, "f": 1, "e": 2, "j": 3, "l": 4, "p": 5, "flux": 0, "fluence": 1, "energy": 2,
            "jacobian": 3, "wl": 4, "wp": 5}
-SOURCES = {"pencil": 0, "planar": 4, "pattern": 5}
+SOURCES = {"pencil": 0, "isotropic": 1, "cone": 2, "gaussian": 3, "planar": 4, "pattern": 5, "fourier": 6,
+           "arcsine": 7, "disk": 8, "fourierx": 9, "fourierx2d": 10, "zgaussian": 11, "line": 12, "slit": 13,
+           "pencilarray": 14, "pattern3d": 15, "hyperboloid": 16, "ring": 17}
+# sources launched at srcpos in the given element (no launch search)
+POINT_SOURCES = (0, 1, 2, 7)
 # one trajectory record (MMC_TRAJ_RECLEN floats)
 TRAJ_DTYPE = np.dtype([("id", "<u4"), ("p", "<f4", 3), ("w", "<f4"), ("eid", "<i4")])
 
@@ -125,7 +129,7 @@ class SimConfig:
         s.srcnum = self.srcnum
         pos = list(self.srcpos) + [0.0] * (4 - len(self.srcpos))
         dirv = list(self.srcdir) + [0.0] * (4 - len(self.srcdir))
-        if s.srctype == 0:
+        if s.srctype in POINT_SOURCES:
             e0, bary = mesh.locate(pos, self.e0)
         else:
             e0 = self.e0 if self.e0 > 0 else mesh.locate(pos)[0] if _encloses(mesh, pos) else 1
@@ -139,7 +143,7 @@ class SimConfig:
         keep = []
         if self.srcpattern is not None:
             pat = np.asarray(self.srcpattern, np.float32)
-            if pat.ndim == 3:             # (srcnum, ny, nx) -> pixel-major, pattern fastest
+            if pat.ndim == 3 and self.srctype == "pattern":   # (srcnum, ny, nx) -> pixel-major, pattern fastest
                 pat = np.transpose(pat, (1, 2, 0))
             pat = np.ascontiguousarray(pat).ravel()
             keep.append(pat)

@@ -895,30 +895,249 @@ float orc_reflect(const orc_mesh* m, int method, float nout, int isreflect, int 
     return ret;
 }
 
-/* launchphoton, src/mmc_raytrace.c:2346-2671 for pencil, planar and pattern
- * sources.  Returns nonzero if no candidate element encloses the start. */
+/* rotatevector, src/mmc_mesh.h:336-357 (non-SSE build) */
+static void orc_rotate(v3* d, float stheta, float ctheta, float sphi, float cphi) {
+    v3 p;
+    float tmp0;
+    if (d->z > -1.f + EPS && d->z < 1.f - EPS) {
+        tmp0 = 1.f - d->z * d->z;
+        float tmp1 = stheta * (1.f / sqrtf(tmp0));
+        p.x = tmp1 * (d->x * d->z * cphi - d->y * sphi) + d->x * ctheta;
+        p.y = tmp1 * (d->y * d->z * cphi + d->x * sphi) + d->y * ctheta;
+        p.z = -tmp1 * tmp0 * cphi + d->z * ctheta;
+    } else {
+        p.x = stheta * cphi;
+        p.y = stheta * sphi;
+        p.z = (d->z > 0.f) ? ctheta : -ctheta;
+    }
+    tmp0 = 1.f / sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+    d->x = p.x * tmp0;
+    d->y = p.y * tmp0;
+    d->z = p.z * tmp0;
+}
+
+#define ORC_JUST_BELOW_ONE 0.9998f    /* src/mmc_raytrace.c:54 */
+#define ORC_CONE_TRIES (1 << 16)
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* launchphoton, src/mmc_raytrace.c:2346-2671, every source type.  Where the
+ * reference calls the double-precision log it uses logf here, the second
+ * draw of the z-Gaussian source comes from the photon's own stream (the
+ * reference takes it from a second generator), the cone's rejection loop is
+ * bounded, and the line source's direction is normalized.  Returns nonzero
+ * if no candidate element encloses the start. */
 static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
     const orc_cfg* cfg = c->cfg;
     const orc_mesh* m = c->m;
+    const int st = cfg->srctype;
+    const float *P1 = cfg->srcparam1, *P2 = cfg->srcparam2, *S = cfg->srcpos, *D = cfg->srcdir;
+    const float focus = D[3];
+    v3 origin = {r->p0.x, r->p0.y, r->p0.z};
+    int canfocus = 1;
     r->slen = orc_rand_scatlen(ran);
-    if (cfg->srctype == ORC_ST_PENCIL) {
+    if (st == ORC_ST_PENCIL) {
         if (r->eid > 0) return 0;
-    } else if (cfg->srctype == ORC_ST_PLANAR || cfg->srctype == ORC_ST_PATTERN) {
+    } else if (st == ORC_ST_PLANAR || st == ORC_ST_PATTERN || st == ORC_ST_PATTERN3D || st == ORC_ST_FOURIER ||
+               st == ORC_ST_PENCILARRAY) {
         float rx = orc_rand_next(ran);
         float ry = orc_rand_next(ran);
-        r->p0.x = cfg->srcpos[0] + rx * cfg->srcparam1[0] + ry * cfg->srcparam2[0];
-        r->p0.y = cfg->srcpos[1] + rx * cfg->srcparam1[1] + ry * cfg->srcparam2[1];
-        r->p0.z = cfg->srcpos[2] + rx * cfg->srcparam1[2] + ry * cfg->srcparam2[2];
-        r->weight = 1.f;
-        if (cfg->srctype == ORC_ST_PATTERN) {
-            int xsize = (int)cfg->srcparam1[3];
-            int ysize = (int)cfg->srcparam2[3];
-            int iy = (int)(ry * ysize), ix = (int)(rx * xsize);
-            iy = iy < ysize - 1 ? iy : ysize - 1;
-            ix = ix < xsize - 1 ? ix : xsize - 1;
-            r->posidx = (unsigned int)(iy * xsize + ix);
-            r->weight = (cfg->srcnum == 1) ? cfg->srcpattern[r->posidx] : 1.f;
+        if (st == ORC_ST_PATTERN3D) {
+            float rz = orc_rand_next(ran);
+            int nx = (int)P1[0], ny = (int)P1[1], nz = (int)P1[2];
+            r->p0.x = S[0] + rx * P1[0];
+            r->p0.y = S[1] + ry * P1[1];
+            r->p0.z = S[2] + rz * P1[2];
+            r->weight = (cfg->srcnum > 1) ? 1.f
+                : cfg->srcpattern[imin((int)(rz * ORC_JUST_BELOW_ONE * nz), nz - 1) * ny * nx
+                                  + imin((int)(ry * ORC_JUST_BELOW_ONE * ny), ny - 1) * nx
+                                  + imin((int)(rx * ORC_JUST_BELOW_ONE * nx), nx - 1)];
+            origin.x += P1[0] * 0.5f;
+            origin.y += P1[1] * 0.5f;
+            origin.z += P1[2] * 0.5f;
+        } else if (st == ORC_ST_PENCILARRAY) {
+            r->p0.x = S[0] + floorf(rx * P1[3]) * P1[0] / (P1[3] - 1.f) + floorf(ry * P2[3]) * P2[0] / (P2[3] - 1.f);
+            r->p0.y = S[1] + floorf(rx * P1[3]) * P1[1] / (P1[3] - 1.f) + floorf(ry * P2[3]) * P2[1] / (P2[3] - 1.f);
+            r->p0.z = S[2] + floorf(rx * P1[3]) * P1[2] / (P1[3] - 1.f) + floorf(ry * P2[3]) * P2[2] / (P2[3] - 1.f);
+            r->weight = 1.f;
+            origin.x += (P1[0] + P2[0]) * 0.5f;
+            origin.y += (P1[1] + P2[1]) * 0.5f;
+            origin.z += (P1[2] + P2[2]) * 0.5f;
+        } else {
+            r->p0.x = S[0] + rx * P1[0] + ry * P2[0];
+            r->p0.y = S[1] + rx * P1[1] + ry * P2[1];
+            r->p0.z = S[2] + rx * P1[2] + ry * P2[2];
+            r->weight = 1.f;
+            if (st == ORC_ST_PATTERN) {
+                int xsize = (int)P1[3];
+                int ysize = (int)P2[3];
+                int iy = (int)(ry * ysize), ix = (int)(rx * xsize);
+                iy = iy < ysize - 1 ? iy : ysize - 1;
+                ix = ix < xsize - 1 ? ix : xsize - 1;
+                r->posidx = (unsigned int)(iy * xsize + ix);
+                r->weight = (cfg->srcnum == 1) ? cfg->srcpattern[r->posidx] : 1.f;
+            } else if (st == ORC_ST_FOURIER) {
+                float sn, cs;
+                O_SINCOSF((float)((floorf(P1[3]) * rx + floorf(P2[3]) * ry + P1[3] - floorf(P1[3])) * TWO_PI),
+                          &sn, &cs);
+                r->weight = (cs * (1.f - P2[3] + floorf(P2[3])) + 1.f) * 0.5f;
+            }
+            origin.x += (P1[0] + P2[0]) * 0.5f;
+            origin.y += (P1[1] + P2[1]) * 0.5f;
+            origin.z += (P1[2] + P2[2]) * 0.5f;
         }
+    } else if (st == ORC_ST_FOURIERX || st == ORC_ST_FOURIERX2D) {
+        float rx = orc_rand_next(ran);
+        float ry = orc_rand_next(ran);
+        float v2w = P1[3] * (1.f / sqrtf(P1[0] * P1[0] + P1[1] * P1[1] + P1[2] * P1[2]));
+        v3 v2 = {v2w * (D[1] * P1[2] - D[2] * P1[1]), v2w * (D[2] * P1[0] - D[0] * P1[2]),
+                 v2w * (D[0] * P1[1] - D[1] * P1[0])};
+        r->p0.x = S[0] + rx * P1[0] + ry * v2.x;
+        r->p0.y = S[1] + rx * P1[1] + ry * v2.y;
+        r->p0.z = S[2] + rx * P1[2] + ry * v2.z;
+        if (st == ORC_ST_FOURIERX2D) {
+            float sa, ca, sb, cb;
+            O_SINCOSF((float)((P2[0] * rx + P2[2]) * TWO_PI), &sa, &ca);
+            O_SINCOSF((float)((P2[1] * ry + P2[3]) * TWO_PI), &sb, &cb);
+            r->weight = (sa * sb + 1.f) * 0.5f;
+        } else {
+            float sn, cs;
+            O_SINCOSF((float)((P2[0] * rx + P2[1] * ry + P2[2]) * TWO_PI), &sn, &cs);
+            r->weight = (cs * (1.f - P2[3]) + 1.f) * 0.5f;
+        }
+        origin.x += (P1[0] + v2.x) * 0.5f;
+        origin.y += (P1[1] + v2.y) * 0.5f;
+        origin.z += (P1[2] + v2.z) * 0.5f;
+    } else if (st == ORC_ST_DISK || st == ORC_ST_RING || st == ORC_ST_GAUSSIAN) {
+        float sphi, cphi, phi, r0;
+        if (st == ORC_ST_RING && (P1[2] > 0.f || P1[3] > 0.f))
+            phi = fabsf(P1[2] - P1[3]) * orc_rand_next(ran) + fminf(P1[2], P1[3]);
+        else
+            phi = (float)(TWO_PI * orc_rand_next(ran));
+        O_SINCOSF(phi, &sphi, &cphi);
+        if (st == ORC_ST_DISK || st == ORC_ST_RING) {
+            r0 = sqrtf(orc_rand_next(ran) * fabsf(P1[0] * P1[0] - P1[1] * P1[1]) + P1[1] * P1[1]);
+        } else if (fabsf(focus) < 1e-5f || fabsf(P1[1]) < 1e-5f) {
+            r0 = sqrtf(-O_LOGF(orc_rand_next(ran))) * P1[0];
+        } else {
+            float z0 = (float)(P1[0] * P1[0] * M_PI / P1[1]);     /* Rayleigh range */
+            r0 = sqrtf(-O_LOGF(orc_rand_next(ran)) * (1.f + (focus * focus / (z0 * z0)))) * P1[0];
+        }
+        if (D[2] > -1.f + EPS && D[2] < 1.f - EPS) {
+            float tmp0 = 1.f - D[2] * D[2];
+            float tmp1 = r0 / sqrtf(tmp0);
+            r->p0.x = S[0] + tmp1 * (D[0] * D[2] * cphi - D[1] * sphi);
+            r->p0.y = S[1] + tmp1 * (D[1] * D[2] * cphi + D[0] * sphi);
+            r->p0.z = S[2] - tmp1 * tmp0 * cphi;
+        } else {
+            r->p0.x += r0 * cphi;
+            r->p0.y += r0 * sphi;
+        }
+    } else if (st == ORC_ST_CONE || st == ORC_ST_ISOTROPIC || st == ORC_ST_ARCSINE) {
+        float ang, stheta, ctheta, sphi, cphi;
+        ang = (float)(TWO_PI * orc_rand_next(ran));
+        O_SINCOSF(ang, &sphi, &cphi);
+        if (st == ORC_ST_CONE) {
+            int k = 0;
+            do {
+                ang = (P1[1] > 0) ? (float)(TWO_PI * orc_rand_next(ran)) : O_ACOSF(2.f * orc_rand_next(ran) - 1.f);
+            } while (ang > P1[0] && ++k < ORC_CONE_TRIES);
+            if (ang > P1[0]) ang = P1[0];
+        } else if (st == ORC_ST_ISOTROPIC) {
+            ang = O_ACOSF(2.f * orc_rand_next(ran) - 1.f);
+        } else {
+            ang = (float)(M_PI * orc_rand_next(ran));
+        }
+        O_SINCOSF(ang, &stheta, &ctheta);
+        orc_rotate(&r->vec, stheta, ctheta, sphi, cphi);
+        canfocus = 0;
+        if (r->eid > 0) return 0;
+    } else if (st == ORC_ST_ZGAUSSIAN) {
+        float ang, stheta, ctheta, sphi, cphi;
+        ang = (float)(TWO_PI * orc_rand_next(ran));
+        O_SINCOSF(ang, &sphi, &cphi);
+        float u1 = orc_rand_next(ran);
+        float u2 = orc_rand_next(ran);
+        ang = sqrtf(-2.f * O_LOGF(u1)) * (1.f - 2.f * u2) * P1[0];
+        O_SINCOSF(ang, &stheta, &ctheta);
+        orc_rotate(&r->vec, stheta, ctheta, sphi, cphi);
+        canfocus = 0;
+    } else if (st == ORC_ST_HYPERBOLOID) {
+        float sphi, cphi;
+        float r0 = (float)(TWO_PI * orc_rand_next(ran));
+        O_SINCOSF(r0, &sphi, &cphi);
+        r0 = sqrtf(0.5f * orc_rand_scatlen(ran)) * P1[0];
+        float tilt = (P1[2] > 0.f) ? (-P1[1] / P1[2]) : 0.f;
+        float bx = -r0 * sphi, by = r0 * cphi, bz = P1[2];
+        float nb = 1.f / sqrtf(bx * bx + by * by + bz * bz);
+        bx *= nb;
+        by *= nb;
+        bz *= nb;
+        float dpx = r0 * (cphi - tilt * sphi);
+        float dpy = r0 * (sphi + tilt * cphi);
+        if (D[2] > -1.f + EPS && D[2] < 1.f - EPS) {
+            float sin_t = sqrtf(1.f - D[2] * D[2]);
+            float isin = 1.f / sin_t;
+            float cpb = D[0] * isin, spb = D[1] * isin;
+            r->p0.x = S[0] + dpx * cpb * D[2] - dpy * spb;
+            r->p0.y = S[1] + dpx * spb * D[2] + dpy * cpb;
+            r->p0.z = S[2] - dpx * sin_t;
+            r->vec.x = bx * cpb * D[2] - by * spb + bz * cpb * sin_t;
+            r->vec.y = bx * spb * D[2] + by * cpb + bz * spb * sin_t;
+            r->vec.z = -bx * sin_t + bz * D[2];
+        } else {
+            r->p0.x = S[0] + dpx;
+            r->p0.y = S[1] + dpy;
+            r->p0.z = S[2];
+            r->vec.x = bx;
+            r->vec.y = by;
+            r->vec.z = (D[2] > 0.f) ? bz : -bz;
+        }
+        float nv = 1.f / sqrtf(r->vec.x * r->vec.x + r->vec.y * r->vec.y + r->vec.z * r->vec.z);
+        r->vec.x *= nv;
+        r->vec.y *= nv;
+        r->vec.z *= nv;
+        canfocus = 0;
+    } else if (st == ORC_ST_LINE || st == ORC_ST_SLIT) {
+        float t = orc_rand_next(ran);
+        r->p0.x += t * P1[0];
+        r->p0.y += t * P1[1];
+        r->p0.z += t * P1[2];
+        if (st == ORC_ST_LINE) {
+            float tt = 1.f - 2.f * orc_rand_next(ran);
+            float ss = 1.f - 2.f * orc_rand_next(ran);
+            float q = 1.f - r->vec.x * r->vec.x - r->vec.y * r->vec.y;
+            float pp = sqrtf(q > 0.f ? q : 0.f) * (orc_rand_next(ran) > 0.5f ? 1.f : -1.f);
+            v3 vv = {r->vec.y * pp - r->vec.z * ss, r->vec.z * tt - r->vec.x * pp, r->vec.x * ss - r->vec.y * tt};
+            float n2 = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z;
+            if (n2 > 0.f) {
+                float n = 1.f / sqrtf(n2);
+                r->vec.x = vv.x * n;
+                r->vec.y = vv.y * n;
+                r->vec.z = vv.z * n;
+            }
+        }
+        origin.x += P1[0] * 0.5f;
+        origin.y += P1[1] * 0.5f;
+        origin.z += P1[2] * 0.5f;
+        canfocus = (st == ORC_ST_SLIT);
+    }
+    if (canfocus && focus != 0.f) {     /* converging or diverging beam */
+        origin.x += focus * r->vec.x;
+        origin.y += focus * r->vec.y;
+        origin.z += focus * r->vec.z;
+        if (focus < 0.f) {
+            r->vec.x = r->p0.x - origin.x;
+            r->vec.y = r->p0.y - origin.y;
+            r->vec.z = r->p0.z - origin.z;
+        } else {
+            r->vec.x = origin.x - r->p0.x;
+            r->vec.y = origin.y - r->p0.y;
+            r->vec.z = origin.z - r->p0.z;
+        }
+        float rn2 = 1.f / sqrtf(r->vec.x * r->vec.x + r->vec.y * r->vec.y + r->vec.z * r->vec.z);
+        r->vec.x = r->vec.x * rn2;
+        r->vec.y = r->vec.y * rn2;
+        r->vec.z = r->vec.z * rn2;
     }
     r->p0 = mult_add(r->p0, r->vec, 1.f, EPS);
 

@@ -25,9 +25,24 @@ extern "C" {
 #define ORC_RT_BLBADOUEL 3
 #define ORC_RT_BLBADOUELGRID 4      /* dual-grid MMC (DMMC): voxel output */
 /* source types = enum TSrcType (src/mmc_utils.h:101) */
-#define ORC_ST_PENCIL  0
-#define ORC_ST_PLANAR  4
-#define ORC_ST_PATTERN 5
+#define ORC_ST_PENCIL      0
+#define ORC_ST_ISOTROPIC   1
+#define ORC_ST_CONE        2
+#define ORC_ST_GAUSSIAN    3
+#define ORC_ST_PLANAR      4
+#define ORC_ST_PATTERN     5
+#define ORC_ST_FOURIER     6
+#define ORC_ST_ARCSINE     7
+#define ORC_ST_DISK        8
+#define ORC_ST_FOURIERX    9
+#define ORC_ST_FOURIERX2D  10
+#define ORC_ST_ZGAUSSIAN   11
+#define ORC_ST_LINE        12
+#define ORC_ST_SLIT        13
+#define ORC_ST_PENCILARRAY 14
+#define ORC_ST_PATTERN3D   15
+#define ORC_ST_HYPERBOLOID 16
+#define ORC_ST_RING        17
 /* output types = enum TOutputType (src/mmc_utils.h:106) */
 #define ORC_OT_FLUX    0
 #define ORC_OT_FLUENCE 1

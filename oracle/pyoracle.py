@@ -160,6 +160,11 @@ class OracleMesh:
         return e, bary
 
 
+SOURCES = {"pencil": 0, "isotropic": 1, "cone": 2, "gaussian": 3, "planar": 4, "pattern": 5, "fourier": 6,
+           "arcsine": 7, "disk": 8, "fourierx": 9, "fourierx2d": 10, "zgaussian": 11, "line": 12, "slit": 13,
+           "pencilarray": 14, "pattern3d": 15, "hyperboloid": 16, "ring": 17}     # enum TSrcType
+
+
 def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     """Oracle settings from an mmc_amd.SimConfig-like object (same field names)."""
     maxgate, t0, t1, dt = sim.gates()
@@ -179,14 +184,14 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     c.issavedet = int(bool(sim.issavedet and (extdet or (sim.detpos is not None and len(sim.detpos)))))
     c.issaveexit = int(sim.issaveexit) if c.issavedet else 0
     c.ismomentum = 0
-    c.srctype = {"pencil": 0, "planar": 4, "pattern": 5}[sim.srctype]
+    c.srctype = SOURCES[sim.srctype]
     c.srcnum = 1
     pos = list(sim.srcpos) + [0.0] * (4 - len(sim.srcpos))
     dirv = list(sim.srcdir) + [0.0] * (4 - len(sim.srcdir))
     c.srcpos[:], c.srcdir[:] = pos[:4], dirv[:4]
     c.srcparam1[:], c.srcparam2[:] = list(sim.srcparam1)[:4], list(sim.srcparam2)[:4]
     keep = []
-    if c.srctype == 0:
+    if c.srctype in (0, 1, 2, 7):       # launched at srcpos in the given element
         e0 = 0
         bary = np.zeros(4, np.float32)
         if sim.e0 > 0:
@@ -199,7 +204,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
         c.e0 = e0
         c.bary0[:] = [float(x) for x in bary]
     else:
-        c.e0 = int(sim.e0)
+        c.e0 = int(sim.e0) if sim.e0 > 0 else max(int(mesh.initelem(pos)[0]), 1)
     if sim.srcpattern is not None:
         pat = np.ascontiguousarray(sim.srcpattern, np.float32).ravel()
         keep.append(pat)
