@@ -106,12 +106,17 @@ def jacobian_mesh(mesh: TetMesh, field: np.ndarray, field_im: Optional[np.ndarra
 
 def detector_sources(cfg: SimConfig, detpos: Sequence[Sequence[float]],
                      detdir: Sequence[Sequence[float]]) -> List[SimConfig]:
-    """One pencil simulation per detector, launched from its position along
-    its direction (cfg->detdir), with the settings of cfg."""
+    """One simulation per detector, launched from its position along its
+    direction (cfg->detdir, w = focus) as a disk of the detector's radius
+    (detpos w; a pencil when 0), with the settings of cfg -- the adjoint
+    sources of src/mmc_utils.c:3760-3800."""
     out = []
     for p, v in zip(detpos, detdir):
-        out.append(dataclasses.replace(cfg, srctype="pencil", srcpos=tuple(p[:3]), srcdir=tuple(v[:3]), e0=0,
-                                       issavedet=0, detpos=None))
+        r = float(p[3]) if len(p) > 3 else 0.0
+        v4 = tuple(v[:3]) + ((float(v[3]),) if len(v) > 3 else (0.0,))
+        out.append(dataclasses.replace(cfg, srctype="disk" if r > 0 else "pencil", srcpos=tuple(p[:3]),
+                                       srcdir=v4, srcparam1=(r, 0.0, 0.0, 0.0), srcparam2=(0.0, 0.0, 0.0, 0.0),
+                                       srcpattern=None, e0=0, issavedet=0, detpos=None))
     return out
 
 

@@ -45,6 +45,10 @@ typedef struct {
     float* med;
     int nmed;
     float detpos[4 * MAXDET];
+    float detdir[4 * MAXDET];  /* Optode.Detector[].Dir: adjoint sources' directions (w = focus) */
+    int ndetdir;
+    char adjoint;              /* -O a|d|u|v|w|q: adjoint Jacobian output, 0 if none */
+    int adjointmode;           /* mesh mode: 0 linear-basis integrals, 1 nodal approximation */
     float* pattern;
     size_t npattern;
     int gpuid;
@@ -125,6 +129,15 @@ static int keyidx(const char* s, const char* const* table) {
 
 /* enum TSrcType names (src/mmc_utils.c:178-180, then the four ids that
  * only the MATLAB/Python front ends name) */
+/* -O / Session.OutputType (src/mmc_utils.c:161): x f e j l p, and the
+ * adjoint Jacobians a d u v w q, which run fluence simulations */
+static void set_outputtype(cli_cfg* c, char t) {
+    t |= 0x20;
+    c->adjoint = strchr("aduvwq", t) ? t : 0;
+    c->sim.outputtype = (t == 'f' || c->adjoint) ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY
+                        : t == 'j' ? MMC_OUT_JACOBIAN : t == 'l' ? MMC_OUT_WL : t == 'p' ? MMC_OUT_WP : MMC_OUT_FLUX;
+}
+
 static const char* const kSrcTypes[] = {"pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier",
                                         "arcsine", "disk", "fourierx", "fourierx2d", "zgaussian", "line", "slit",
                                         "pencilarray", "pattern3d", "hyperboloid", "ring", ""};
@@ -221,6 +234,11 @@ static void load_json(cli_cfg* c, const char* path) {
             for (const mj_node* d = dets->child; d && i < MAXDET; d = d->next, i++) {
                 vec_of(mj_get(d, "Pos"), c->detpos + 4 * i, 3);
                 c->detpos[4 * i + 3] = (float)mj_num(mj_get(d, "R"), 1.0);
+                const mj_node* dd = mj_get(d, "Dir");
+                if (dd) {
+                    vec_of(dd, c->detdir + 4 * i, 4);
+                    c->ndetdir = i + 1;
+                }
             }
             c->sim.detnum = i;
         }
@@ -246,10 +264,7 @@ static void load_json(cli_cfg* c, const char* path) {
             if (v->type == MJ_NUMBER) snprintf(num, sizeof(num), "%d", (int)v->num);
             debug_flags(c, v->type == MJ_NUMBER ? num : mj_str(v, ""));
         }
-        if ((v = mj_get(Session, "OutputType"))) {
-            char t = mj_str(v, "x")[0] | 0x20;
-            c->sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : MMC_OUT_FLUX;
-        }
+        if ((v = mj_get(Session, "OutputType"))) set_outputtype(c, mj_str(v, "x")[0]);
     }
     if (Forward) {
         c->sim.tstart = (float)mj_num(mj_get(Forward, "T0"), 0);
@@ -368,7 +383,8 @@ static void dump_json(const cli_cfg* c) {
             c->savetraj ? MMC_DL_TRAJ : 0, s->basisorder);
     const char* fmts[] = {"ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii", "bnii"};
     fprintf(fp, "\t\t\"RayTracer\": \"%c\",\n\t\t\"OutputFormat\": \"%s\",\n\t\t\"OutputType\": \"%c\"\n\t},\n",
-            c->method, fmts[c->outputformat], otype[s->outputtype >= 0 && s->outputtype <= 5 ? s->outputtype : 0]);
+            c->method, fmts[c->outputformat],
+            c->adjoint ? c->adjoint : otype[s->outputtype >= 0 && s->outputtype <= 5 ? s->outputtype : 0]);
     fprintf(fp, "\t\"Forward\": {\n\t\t\"T0\": %.9g,\n\t\t\"T1\": %.9g,\n\t\t\"Dt\": %.9g,\n\t\t\"N0\": %.9g,\n"
                 "\t\t\"Omega\": %.9g\n\t},\n",
             (double)s->tstart, (double)s->tend, (double)s->tstep, (double)s->nout, (double)s->omega);
@@ -402,9 +418,14 @@ static void dump_json(const cli_cfg* c) {
         fprintf(fp, "]");
     }
     fprintf(fp, "\n\t\t},\n\t\t\"Detector\": [");
-    for (int i = 0; i < s->detnum; i++)
-        fprintf(fp, "%s\n\t\t\t{\"Pos\": [%.9g, %.9g, %.9g], \"R\": %.9g}", i ? "," : "", (double)c->detpos[4 * i],
+    for (int i = 0; i < s->detnum; i++) {
+        fprintf(fp, "%s\n\t\t\t{\"Pos\": [%.9g, %.9g, %.9g], \"R\": %.9g", i ? "," : "", (double)c->detpos[4 * i],
                 (double)c->detpos[4 * i + 1], (double)c->detpos[4 * i + 2], (double)c->detpos[4 * i + 3]);
+        if (i < c->ndetdir)
+            fprintf(fp, ", \"Dir\": [%.9g, %.9g, %.9g, %.9g]", (double)c->detdir[4 * i], (double)c->detdir[4 * i + 1],
+                    (double)c->detdir[4 * i + 2], (double)c->detdir[4 * i + 3]);
+        fprintf(fp, "}");
+    }
     fprintf(fp, "\n\t\t]\n\t}\n}\n");
     if (fp != stdout) fclose(fp);
 }
@@ -519,8 +540,11 @@ static void usage(void) {
         "                        branch-less Badouel (element, -C 0)\n"
         "  --gridsize D          dual-grid voxel edge in mesh units (default 1)\n"
         "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
-        "  -O/--outputtype [X|F|E|J|L|P] flux, fluence, energy deposit; replay: Jacobian,\n"
-        "                        weighted path length, weighted scattering count\n"
+        "  -O/--outputtype [X|F|E|J|L|P|A|D|U|V|W|Q] flux, fluence, energy deposit; replay:\n"
+        "                        Jacobian, weighted path length, weighted scattering count;\n"
+        "                        adjoint Jacobians (detectors need Dir): mua, D, mus, musp,\n"
+        "                        mua and D, mua and musp -> <session>_jmua, _jd, _jmus, _jmusp\n"
+        "  --adjointmode [0|1]   mesh-mode J_mua: linear-basis integrals (0) or nodal approx. (1)\n"
         "  -F/--outputformat [ascii|bin|nii|hdr|mc2|tx3|jnii|bnii]\n"
         "  -Z/--zip [zlib|base64]   jnii/bnii payload: zlib-compressed (default) or raw\n"
         "  -b/--reflect [0|1|2|3] boundary: no reflect, Fresnel, absorb exterior, mirror\n"
@@ -554,6 +578,120 @@ static void debug_flags(cli_cfg* c, const char* v) {
 static const char* opt_value(int argc, char** argv, int* i) {
     if (*i + 1 >= argc) die("missing value for %s", argv[*i]);
     return argv[++*i];
+}
+
+/* Adjoint Jacobians (-O a|d|u|v|w|q; src/mmc_utils.c:3755-3800,
+ * src/mmc_cl_host.c:1117-1462, src/mmc_mesh.c:1802-1960): the source, then
+ * each detector launched from its position along Optode.Detector[].Dir (a
+ * disk of its radius), each run as its own fluence simulation with an equal
+ * share of the photons and normalized alone; the Jacobians of the normalized
+ * fields are written as <session>_jmua / _jd (_jmus, _jmusp: the same J_D
+ * product under the reference's names), values [s][d][cell] with the
+ * imaginary block after the real one for RF. */
+static int run_adjoint(cli_cfg* c, const mmc_hip_mesh* mesh, const int32_t* elem, const int32_t* type,
+                       const float* evol, const float* nvol, int isgrid, const int32_t griddim[3], int datalen) {
+    const int nd = c->sim.detnum, ns = 1, nslot = ns + nd;
+    if (nd < 1 || c->ndetdir < nd) die("%s", "adjoint outputs need detectors with directions (Optode.Detector[].Dir)");
+    if (!isgrid && c->sim.basisorder != 1)
+        die("%s", "mesh-mode adjoint Jacobians need nodal fluence (-M P -C 1) or use the dual grid (-M G)");
+    if (c->sim.srctype == MMC_SRC_PATTERN && c->sim.srcnum > 1) die("%s", "adjoint outputs take one source pattern");
+    const int isrf = c->sim.omega > 0.f;
+    const size_t cells = (size_t)datalen, per = cells * (size_t)c->sim.maxgate;
+    float* f = (float*)calloc(per * nslot, sizeof(float));
+    float* fi = isrf ? (float*)calloc(per * nslot, sizeof(float)) : NULL;
+    double* field = (double*)calloc(per, sizeof(double));
+    double* field_im = isrf ? (double*)calloc(per, sizeof(double)) : NULL;
+    if (!f || !field || (isrf && (!fi || !field_im))) die("%s", "can not allocate the adjoint fields");
+    const int64_t nph = c->sim.nphoton / nslot > 0 ? c->sim.nphoton / nslot : 1;
+    for (int k = 0; k < nslot; k++) {
+        mmc_hip_sim s = c->sim;
+        s.nphoton = nph;
+        s.issavedet = 0;
+        s.issaveexit = 0;
+        s.detnum = 0;
+        s.detpos = NULL;
+        if (k >= ns) {
+            const float* dp = c->detpos + 4 * (k - ns);
+            const float* dd = c->detdir + 4 * (k - ns);
+            memcpy(s.srcpos, dp, 3 * sizeof(float));
+            s.srcpos[3] = 1.f;
+            memcpy(s.srcdir, dd, 4 * sizeof(float));
+            memset(s.srcparam1, 0, sizeof(s.srcparam1));
+            memset(s.srcparam2, 0, sizeof(s.srcparam2));
+            s.srcparam1[0] = dp[3];
+            s.srctype = dp[3] > 0.f ? MMC_SRC_DISK : MMC_SRC_PENCIL;
+            s.srcpattern = NULL;
+            s.srcnum = 1;
+            float b[4] = {0.f, 0.f, 0.f, 0.f};
+            s.e0 = mmc_host_initelem(c->ne, c->node, elem, s.srcpos, b);
+            if (s.e0 <= 0) die("%s", "a detector lies outside the mesh (adjoint source)");
+            memcpy(s.bary0, b, sizeof(b));
+        }
+        mmc_hip_result res;
+        memset(&res, 0, sizeof(res));
+        memset(field, 0, per * sizeof(double));
+        if (field_im) memset(field_im, 0, per * sizeof(double));
+        res.field = field;
+        res.field_im = field_im;
+        if (mmc_hip_run(mesh, &s, NULL, &res) != 0) die("%s", mmc_hip_last_error());
+        const double k_norm = isgrid ? mmc_host_normalize_grid(&s, datalen, field, res.absorbweight,
+                                                               res.launchweight, 0)
+                                     : mmc_host_normalize(&s, c->nn, c->ne, elem, type, c->med, evol, nvol, field,
+                                                          res.absorbweight, res.launchweight);
+        for (size_t i = 0; i < per; i++) f[k * per + i] = (float)field[i];
+        if (fi)
+            for (size_t i = 0; i < per; i++) fi[k * per + i] = (float)(field_im[i] * k_norm);
+        printf("adjoint slot %d of %d: %lld photons, %.2f photon/ms\n", k + 1, nslot, (long long)nph,
+               (double)nph / res.total_ms);
+    }
+    const char t = c->adjoint;
+    const int want_mua = (t == 'a' || t == 'w' || t == 'q'), want_d = (t != 'a');
+    const size_t nout = cells * (size_t)(ns * nd) * (isrf ? 2 : 1);
+    float* jm = want_mua ? (float*)calloc(nout, sizeof(float)) : NULL;
+    float* jd = want_d ? (float*)calloc(nout, sizeof(float)) : NULL;
+    int rc;
+    if (isgrid) {
+        rc = mmc_hip_adjoint_grid(f, fi, griddim, c->sim.maxgate, ns, nd, c->unitinmm, jm, jd);
+    } else {
+        float* ddd = NULL;
+        if (want_d) {
+            double* d64 = (double*)malloc(sizeof(double) * 10 * (size_t)c->ne);
+            ddd = (float*)malloc(sizeof(float) * 10 * (size_t)c->ne);
+            if (!d64 || !ddd || mmc_host_deldotdel(c->node, c->nn, elem, c->ne, evol, d64))
+                die("%s", "can not form the element gradient products");
+            for (size_t i = 0; i < 10 * (size_t)c->ne; i++) ddd[i] = (float)d64[i];
+            free(d64);
+        }
+        rc = mmc_hip_adjoint_mesh(f, fi, c->nn, c->ne, elem, evol, ddd, nvol, c->sim.maxgate, ns, nd,
+                                  c->adjointmode == 1 ? 1 : 0, jm, jd);
+        free(ddd);
+    }
+    if (rc) die("adjoint Jacobians failed: %s", mmc_hip_last_error());
+    const char* name_d = (t == 'u') ? "jmus" : (t == 'v' || t == 'q') ? "jmusp" : "jd";
+    for (int w = 0; w < 2; w++) {
+        const float* j = w == 0 ? jm : jd;
+        if (!j) continue;
+        char sess[400];
+        snprintf(sess, sizeof(sess), "%s_%s", c->session, w == 0 ? "jmua" : name_d);
+        double* out = (double*)malloc(sizeof(double) * nout);
+        if (!out) die("%s", "can not allocate the Jacobian output");
+        for (size_t i = 0; i < nout; i++) out[i] = j[i];
+        const int nblk = ns * nd * (isrf ? 2 : 1);
+        rc = isgrid ? mmc_host_save_grid(sess, c->outputformat, out, griddim, c->gridstep, nblk, 1, c->sim.tstep,
+                                         MMC_OUT_FLUENCE, c->zip)
+                    : mmc_host_save_field(sess, c->outputformat, out, datalen, nblk, 1, c->sim.tstep,
+                                          MMC_OUT_FLUENCE, c->zip);
+        free(out);
+        if (rc) die("can not write %s", sess);
+        printf("saved %s (%d source-detector pairs%s)\n", sess, ns * nd, isrf ? ", real then imaginary" : "");
+    }
+    free(jm);
+    free(jd);
+    free(f);
+    free(fi);
+    free(field);
+    free(field_im);
+    return 0;
 }
 
 int main(int argc, char** argv) {
@@ -602,10 +740,9 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "-M") || !strcmp(a, "--method")) c.method = opt_value(argc, argv, &i)[0] | 0x20;
         else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "--gridsize")) c.gridstep = (float)atof(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "--adjointmode")) c.adjointmode = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
-            char t = opt_value(argc, argv, &i)[0] | 0x20;
-            c.sim.outputtype = t == 'f' ? MMC_OUT_FLUENCE : t == 'e' ? MMC_OUT_ENERGY : t == 'j' ? MMC_OUT_JACOBIAN
-                             : t == 'l' ? MMC_OUT_WL : t == 'p' ? MMC_OUT_WP : MMC_OUT_FLUX;
+            set_outputtype(&c, opt_value(argc, argv, &i)[0]);
         } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) {
             const char* f = opt_value(argc, argv, &i);
             if ((c.outputformat = mmc_host_format_id(f)) < 0) die("unsupported output format '%s'", f);
@@ -746,6 +883,17 @@ int main(int argc, char** argv) {
         int64_t nc = (int64_t)griddim[0] * griddim[1] * griddim[2];
         if (nc >= (1ll << 31)) die("%s", "the dual grid has more than 2^31 voxels (raise --gridsize)");
         datalen = (int)nc;
+    }
+    if (c.adjoint) {
+        int rc = run_adjoint(&c, &mesh, elem, type, evol, nvol, isgrid, griddim, datalen);
+        free(elem);
+        free(type);
+        free(srcelem);
+        free(evol);
+        free(nvol);
+        free(facenb);
+        free(planes);
+        return rc;
     }
     double* field = (double*)calloc((size_t)datalen * c.sim.maxgate, sizeof(double));
     /* RF forward: the imaginary part of the complex fluence */

@@ -89,11 +89,15 @@ def test_gpu_adjoint_equals_the_restatement(rf):
 def test_gpu_adjoint_run_on_a_slab():
     # one source and two detectors on a layered slab: the detectors run as
     # sources into the medium; J_mua is negative where both fields reach
+    # (detectors of 1.5 mm radius launch as disks: the bottom layer is the
+    # launch search's candidate list)
     arrays = meshgen.slab_layers(2.0, nx=40, ny=40, nz=20)
-    tm = M.TetMesh(*arrays).prepare(nodal=False)
+    srcelem = (np.where(meshgen.centroids(arrays[0], arrays[1])[:, 2] < 2.0)[0] + 1).astype(np.int32)
+    tm = M.TetMesh(*arrays, srcelem=srcelem).prepare(nodal=False)
     src = M.SimConfig(nphoton=200000, seed=4, method="G", gridstep=2.0, srcpos=(14.0, 20.0, 0.0),
                       srcdir=(0.0, 0.0, 1.0), tstart=0.0, tend=5e-9, tstep=1e-9, outputtype="f")
-    dets = MA.detector_sources(src, [(20.0, 20.0, 0.0), (26.0, 20.0, 0.0)], [(0, 0, 1), (0, 0, 1)])
+    dets = MA.detector_sources(src, [(20.0, 20.0, 0.0, 1.5), (26.0, 20.0, 0.0, 1.5)], [(0, 0, 1), (0, 0, 1)])
+    assert dets[0].srctype == "disk" and dets[0].srcparam1[0] == 1.5
     f, fi = MA.forward_fields(tm, [src], dets)
     assert fi is None and f.shape[0] == 3
     j = MA.jacobian_grid(f, None, 1, 2, unitinmm=1.0)

@@ -183,3 +183,31 @@ def test_saved_seeds_replay_the_detected_photons(tmp_path):
     # one medium: replay weight w0*exp(-mua*ppath), deposit = weight * path
     w = rec[:, -1].astype(np.float64) * np.exp(-0.005 * rec[:, 2].astype(np.float64))
     assert abs(f.sum() / (w * rec[:, 2]).sum() - 1) < 1e-4
+
+
+@pytest.mark.gpu
+def test_adjoint_jacobians_from_the_command_line(tmp_path):
+    # -O w on the dual grid: the source, then each detector (given a Dir) run
+    # as a source; <session>_jmua and _jd hold [s][d][voxel] float values
+    cfg = json.load(open(os.path.join(GOLDEN, "cube2", "cube2.json")))
+    cfg["Optode"]["Detector"] = [{"Pos": [30.0, 20.0, 0.0], "R": 0.0, "Dir": [0, 0, 1]},
+                                 {"Pos": [30.0, 40.0, 0.0], "R": 0.0, "Dir": [0, 0, 1]}]
+    js = tmp_path / "adj.json"
+    js.write_text(json.dumps(cfg))
+    r = run(["-f", str(js), "-n", "60000", "--gridsize", "2", "-O", "w", "-F", "bin", "-s", "adj"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("adjoint slot") == 3
+    jm = np.fromfile(tmp_path / "adj_jmua.bin", np.float64).reshape(2, 31, 31, 31)
+    jd = np.fromfile(tmp_path / "adj_jd.bin", np.float64).reshape(2, 31, 31, 31)
+    assert np.all(jm <= 0) and jm.min() < 0 and np.abs(jd).max() > 0
+    # each pair is most sensitive between its source and detector, near the surface
+    for d, y in ((0, 20.0), (1, 40.0)):
+        z, yy, x = np.unravel_index(np.argmin(jm[d]), jm[d].shape)
+        assert z <= 3 and abs(x * 2 - 30) <= 6 and min(yy * 2, 60) <= max(30, y) + 4
+    r = run(["-f", str(js), "-n", "30000", "--gridsize", "2", "-O", "a", "-F", "bin", "-s", "only"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "only_jmua.bin").exists() and not (tmp_path / "only_jd.bin").exists()
+    del cfg["Optode"]["Detector"][1]["Dir"]
+    js.write_text(json.dumps(cfg))
+    r = run(["-f", str(js), "-n", "1000", "-O", "a", "-s", "bad"], tmp_path)
+    assert r.returncode != 0 and "Dir" in r.stderr
