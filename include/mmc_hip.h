@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MMC_HIP_ABI_VERSION 7
+#define MMC_HIP_ABI_VERSION 8
 
 /* ray-tracing methods (enum TRTMethod, src/mmc_utils.h:97) */
 #define MMC_RT_PLUCKER   0
@@ -147,6 +147,10 @@ typedef struct mmc_hip_sim {
      * output) with srcnum 1; ignored for replayed photons, as in the reference.  One
      * omega per batch. */
     float omega;
+    /* cfg->ismomentum (-m): detected records carry, after the partial paths, the momentum
+     * transfer sum of (1 - cos theta) over the photon's scattering events per medium
+     * (src/mmc_mesh.c:1707-1709, src/mmc_raytrace.c:2073-2075) */
+    int32_t ismomentum;
 } mmc_hip_sim;
 
 /* Per-photon end state, written for photon ids in [recfirst, recfirst+reccount). */

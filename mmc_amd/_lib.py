@@ -48,7 +48,7 @@ class SimC(C.Structure):
         ("med", f32p),
         ("photonseed", C.POINTER(C.c_uint64)), ("replayweight", f32p), ("replaytime", f32p),
         ("gridstep", C.c_float), ("unitinmm", C.c_float), ("debuglevel", C.c_int32),
-        ("omega", C.c_float),
+        ("omega", C.c_float), ("ismomentum", C.c_int32),
     ]
 
 
@@ -145,7 +145,7 @@ SIGNATURES = {
 }
 
 
-ABI_VERSION = 7        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
+ABI_VERSION = 8        # MMC_HIP_ABI_VERSION of include/mmc_hip.h
 
 
 def lib():

@@ -253,6 +253,7 @@ static void load_json(cli_cfg* c, const char* path) {
         c->sim.issavedet = (int)mj_num(mj_get(Session, "DoPartialPath"), c->sim.issavedet);
         c->sim.isspecular = (int)mj_num(mj_get(Session, "DoSpecular"), c->sim.isspecular);
         c->sim.issaveexit = (int)mj_num(mj_get(Session, "DoSaveExit"), c->sim.issaveexit);
+        c->sim.ismomentum = (int)mj_num(mj_get(Session, "DoDCS"), c->sim.ismomentum);
         c->sim.basisorder = (int)mj_num(mj_get(Session, "BasisOrder"), c->sim.basisorder);
         if ((v = mj_get(Session, "OutputFormat"))) {
             c->outputformat = mmc_host_format_id(mj_str(v, "ascii"));
@@ -552,6 +553,7 @@ static void usage(void) {
         "  -S/--save2pt [0|1]    save the fluence field\n"
         "  -d/--savedet [0|1]    save detected photons (.mch)\n"
         "  -x/--saveexit [0|1]   save exit position and direction of detected photons\n"
+        "  -m/--momentum [0|1]   save the momentum transfer per medium of detected photons\n"
         "  -H/--maxdetphoton N   detected-photon buffer size\n"
         "  -e/--minenergy W      roulette threshold\n"
         "  -V/--specular [0|1]   specular reflection at launch\n"
@@ -741,6 +743,7 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "--gridsize")) c.gridstep = (float)atof(opt_value(argc, argv, &i));
         else if (!strcmp(a, "--adjointmode")) c.adjointmode = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "-m") || !strcmp(a, "--momentum")) c.sim.ismomentum = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
             set_outputtype(&c, opt_value(argc, argv, &i)[0]);
         } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) {
@@ -845,7 +848,10 @@ int main(int argc, char** argv) {
         }
     }
     if (c.sim.issavedet && c.sim.detnum == 0 && !isextdet) c.sim.issavedet = 0;
-    if (!c.sim.issavedet) c.sim.issaveexit = 0;
+    if (!c.sim.issavedet) {
+        c.sim.issaveexit = 0;
+        c.sim.ismomentum = 0;
+    }
     for (int i = 0; i < c.ne; i++)
         if (type[i] < 0 || type[i] > prop + isextdet) die("%s", "element labels outside the media list");
     float* evol = (float*)malloc(sizeof(float) * (size_t)c.ne);
@@ -910,7 +916,7 @@ int main(int argc, char** argv) {
         res.traj = (float*)calloc((size_t)c.maxjumpdebug * MMC_TRAJ_RECLEN, sizeof(float));
         if (!res.traj) die("%s", "can not allocate the trajectory buffer");
     }
-    int reclen = 2 * prop + 6 * (c.sim.issaveexit > 0) + 2;
+    int reclen = (2 + (c.sim.ismomentum > 0)) * prop + 6 * (c.sim.issaveexit > 0) + 2;
     if (c.sim.issavedet) {
         res.detdata = (float*)calloc((size_t)c.sim.maxdetphoton * reclen, sizeof(float));
         res.detphoton = (int64_t*)calloc((size_t)c.sim.maxdetphoton, sizeof(int64_t));
@@ -973,7 +979,8 @@ int main(int argc, char** argv) {
         his.normalizer = (float)normalizer;
         his.srcnum = 1;
         his.respin = 1;
-        his.savedetflag = c.sim.issaveexit ? 0x77 : 0x47;
+        /* columns present: D(etid) S(cattering) P(ath) M(omentum) X(exit) V W(eight) bits */
+        his.savedetflag = (c.sim.issaveexit ? 0x77 : 0x47) | (c.sim.ismomentum > 0 ? 0x08 : 0);
         his.seedbyte = c.issaveseed ? 16 : 0;
         snprintf(fname, sizeof(fname), "%s.mch", c.session);
         if (mmc_host_save_mch(fname, &his, res.detdata, saved)) die("can not write %s", fname);

@@ -27,8 +27,10 @@ constexpr int kReflShift = 4, kSpecShift = 6;
  * (energy, weighted path, weighted scattering count) rather than fluence */
 constexpr int kReplayShift = 8;
 enum { kReplayNone = 0, kReplayJ = 1, kReplayWL = 2, kReplayWP = 3 };
-/* bit 10: record trajectories */
+/* bit 10: record trajectories; bit 11: momentum transfer block in the
+ * detected records */
 constexpr int kFlagTraj = 1 << 10;
+constexpr int kFlagMom = 1 << 11;
 struct SimHot {
     float tstart, tend, rtstep, minenergy;
     float roulettesize;
@@ -85,7 +87,8 @@ struct KArgs {
     float4 loc_min, loc_inv;
     int4 loc_dim;
     int32_t nn, ne, prop, nsim;
-    int32_t isextdet, pad_ext;       /* wide-field detector elements labelled prop+1 */
+    int32_t isextdet;                /* wide-field detector elements labelled prop+1 */
+    int32_t mom;                     /* 1: lanes keep a momentum-transfer block of prop values */
     /* simulations */
     const SimHot* hot;               /* nsim */
     const SimCold* cold;             /* nsim */

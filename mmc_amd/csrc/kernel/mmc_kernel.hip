@@ -410,8 +410,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     const int tid = threadIdx.x;
     /* lane-private LDS: [slot][thread] */
     auto LS = [&](int slot) -> float* { return s_lane + (size_t)slot * nthr + tid; };
-    const int ppbase = kLaneSlots;                   /* nscat[prop], ppath[prop] */
-    const int slot_w0 = kLaneSlots + 2 * a.prop;
+    const int ppbase = kLaneSlots;                   /* nscat[prop], ppath[prop], mom[prop] */
+    const int slot_w0 = kLaneSlots + (2 + a.mom) * a.prop;
     const int slot_rg = slot_w0 + 1;
     const int slot_ns = slot_w0 + 2;
     /* shared patterns (srcnum > 1): the photon's weight in each pattern, and
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     *LS(slot_rg) = 0.f;
                     *LS(slot_ns) = 0.f;
                     if (a.savedet)
-                        for (int k = 0; k < 2 * a.prop; k++) *LS(ppbase + k) = 0.f;
+                        for (int k = 0; k < (2 + a.mom) * a.prop; k++) *LS(ppbase + k) = 0.f;
                     /* replay with weighted path / scattering outputs counts the
                      * detected weight as launched energy (src/mmc_raytrace.c:1815) */
                     {
@@ -1231,6 +1231,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     slen0 = nextslen;
                     if (hot.flags & kFlagTraj) savetraj();
                     if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
+                    /* momentum transfer (src/mmc_raytrace.c:2073-2075) */
+                    if ((hot.flags & kFlagMom) && ctype > 0) *LS(ppbase + 2 * a.prop - 1 + ctype) += 1.f - ctheta;
                     *LS(slot_ns) += 1.f;
                     ctx = CTX_OUTER;
                     continue;
@@ -1268,8 +1270,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     if ((int64_t)slot < a.maxdet && a.detdata) {
                         float* row = a.detdata + (size_t)slot * a.reclen;
                         row[0] = (float)exitdet;
-                        for (int k = 0; k < 2 * a.prop; k++) row[1 + k] = *LS(ppbase + k);
-                        int col = 1 + 2 * a.prop;
+                        const int npp = (hot_of().flags & kFlagMom) ? 3 * a.prop : 2 * a.prop;
+                        for (int k = 0; k < npp; k++) row[1 + k] = *LS(ppbase + k);
+                        int col = 1 + npp;
                         if (C.issaveexit) {
                             row[col++] = px; row[col++] = py; row[col++] = pz;
                             row[col++] = vx; row[col++] = vy; row[col++] = vz;

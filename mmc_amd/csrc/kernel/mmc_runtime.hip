@@ -298,6 +298,10 @@ extern "C" int64_t mmc_hip_mesh_bytes(mmc_hip_mesh_handle h) { return h ? h->byt
 
 /* field slots per value: the patterns that share each photon */
 static int slots_of(const mmc_hip_sim* s) { return s->srctype == MMC_SRC_PATTERN ? s->srcnum : 1; }
+/* momentum transfer is recorded with the detected photons only */
+static int mom_of(const mmc_hip_sim* s, const mmc_hip_meshres* m) {
+    return (s->ismomentum > 0 && s->issavedet && (s->detnum > 0 || m->isextdet)) ? 1 : 0;
+}
 
 /* dual grid of a simulation: mesh_createdualmesh (src/mmc_mesh.c:349-381)
  * over the mesh's widened box with the simulation's voxel edge */
@@ -601,7 +605,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                   (rmode << kReplayShift) |
                   ((s.issavedet && (s.detnum > 0 || m->isextdet)) ? kFlagSaveDet : 0) |
                   (s.voidtime ? kFlagVoidTime : 0) | ((s.isreflect & 3) << kReflShift) |
-                  ((s.isspecular & 3) << kSpecShift) | ((s.debuglevel & MMC_DL_TRAJ) ? kFlagTraj : 0);
+                  ((s.isspecular & 3) << kSpecShift) | ((s.debuglevel & MMC_DL_TRAJ) ? kFlagTraj : 0) |
+                  (mom_of(&s, m) ? kFlagMom : 0);
         h.nout = s.nout;
         h.srcnum = slots_of(&s);
         maxsrc = std::max(maxsrc, h.srcnum);
@@ -674,12 +679,15 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             media[(size_t)i * nmed + k] = make_float4(md[4 * k], md[4 * k + 1], md[4 * k + 2], md[4 * k + 3]);
         if (m->isextdet) media[(size_t)i * nmed + m->prop + 1] = media[(size_t)i * nmed];
         cum[i + 1] = cum[i] + s.nphoton;
+        /* record: detector id, scattering counts, partial paths, momentum
+         * transfer (ismomentum), exit position and direction (issaveexit),
+         * launch weight; the device rows use the longest record's stride */
+        res[i].reclen = c.issavedet ? 1 + (2 + mom_of(&sims[i], m)) * m->prop + 6 * c.issaveexit + 1 : 0;
         if (c.issavedet) {
             savedet = 1;
-            maxrec = std::max(maxrec, 1 + 2 * m->prop + 6 * c.issaveexit + 1);
+            maxrec = std::max(maxrec, res[i].reclen);
             maxdet += s.maxdetphoton > 0 ? s.maxdetphoton : 0;
         }
-        res[i].reclen = c.issavedet ? 1 + 2 * m->prop + 6 * c.issaveexit + 1 : 0;
     }
     const int64_t total = cum[nsim];
     const int reclen = maxrec;
@@ -814,8 +822,11 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     a.traj = maxtraj > 0 ? (float*)sl->d_traj.p : nullptr;
     a.maxtraj = (uint64_t)maxtraj;
     a.omega = rf ? sims[0].omega : 0.f;
+    int anymom = 0;
+    for (int i = 0; i < nsim; i++) anymom |= mom_of(&sims[i], m);
+    a.mom = anymom;
 
-    const int nslots = kLaneSlots + 2 * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
+    const int nslots = kLaneSlots + (2 + anymom) * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that
      * large batches keep 4+ workgroups per CU */
     const size_t simtab = (sizeof(SimHot) + sizeof(float4) * (size_t)nmed) * nsim;

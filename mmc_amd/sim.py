@@ -69,6 +69,7 @@ class SimConfig:
     gridstep: float = 1.0                  # dual-grid voxel edge (cfg->steps.x), mesh units
     unitinmm: float = 1.0                  # mesh length unit (normalization of grid fields)
     omega: float = 0.0                     # RF modulation (rad/s): complex fluence, field_im
+    ismomentum: int = 0                    # detected records add momentum transfer per medium (-m)
     trajectory: bool = False               # record launch/scattering/end positions ("-D M")
     maxjumpdebug: int = 10_000_000         # trajectory records kept
 
@@ -189,6 +190,7 @@ class SimConfig:
         s.unitinmm = float(self.unitinmm)
         s.debuglevel = 8192 if self.trajectory else 0
         s.omega = float(self.omega)
+        s.ismomentum = int(self.ismomentum)
         s._keep = keep
         return s
 
@@ -237,7 +239,8 @@ class _ResultBuf:
         self.field_im = np.zeros(shape, np.float64) if (want_field and cfg.is_rf) else None
         self.srcweight = np.zeros((2, S), np.float64) if S > 1 else None
         ndet = len(cfg.detpos) if (cfg.issavedet and cfg.detpos is not None) else 0
-        self.reclen = (1 + 2 * mesh.prop + 6 * int(cfg.issaveexit) + 1) if ndet else 0
+        mom = int(cfg.ismomentum > 0)
+        self.reclen = (1 + (2 + mom) * mesh.prop + 6 * int(cfg.issaveexit) + 1) if ndet else 0
         nd = min(int(cfg.maxdetphoton), int(cfg.nphoton)) if ndet else 0
         self.detdata = np.zeros((max(nd, 1), max(self.reclen, 1)), np.float32) if ndet else None
         self.detphoton = np.zeros(max(nd, 1), np.int64) if ndet else None

@@ -816,8 +816,15 @@ static void orc_fixphoton(oray* r, const orc_mesh* m) {
     r->p0.z += (c0.z * 0.25f - r->p0.z) * FIX_PHOTON;
 }
 
-/* mc_next_scatter, src/mmc_mesh.c:1644-1715 (non-SSE-math build) */
+/* mc_next_scatter, src/mmc_mesh.c:1644-1715 (non-SSE-math build); *mom
+ * receives the momentum transfer 1 - cos(theta) (src/mmc_mesh.c:1707-1709) */
+static float orc_scatter_mom(float g, float dir[3], uint64_t t[2], float* mom);
 float orc_scatter(float g, float dir[3], uint64_t t[2]) {
+    float mom;
+    return orc_scatter_mom(g, dir, t, &mom);
+}
+
+static float orc_scatter_mom(float g, float dir[3], uint64_t t[2], float* mom) {
     float sphi, cphi, tmp0, stheta, ctheta, tmp1;
     float nextslen = orc_rand_scatlen(t);
     tmp0 = (float)(TWO_PI * orc_rand_next(t));
@@ -847,6 +854,7 @@ float orc_scatter(float g, float dir[3], uint64_t t[2]) {
         py = stheta * sphi;
         pz = (dir[2] > 0.f) ? ctheta : -ctheta;
     }
+    *mom = 1.f - ctheta;
     dir[0] = px;
     dir[1] = py;
     dir[2] = pz;
@@ -1330,8 +1338,11 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
 
         {
             float dir[3] = {r.vec.x, r.vec.y, r.vec.z};
-            r.slen0 = orc_scatter(med_of(m, m->type[r.eid - 1])[2], dir, ran);
+            float mom = 0.f;
+            r.slen0 = orc_scatter_mom(med_of(m, m->type[r.eid - 1])[2], dir, ran, &mom);
             r.vec.x = dir[0]; r.vec.y = dir[1]; r.vec.z = dir[2];
+            if (cfg->ismomentum && cfg->issavedet && m->type[r.eid - 1] > 0)
+                ppath[2 * prop - 1 + m->type[r.eid - 1]] += mom;   /* third block: momentum transfer */
         }
         r.slen = r.slen0;
         if (m->type[r.eid - 1] > 0) ppath[m->type[r.eid - 1] - 1]++;

@@ -183,7 +183,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     extdet = bool(getattr(mesh, "isextdet", 0))
     c.issavedet = int(bool(sim.issavedet and (extdet or (sim.detpos is not None and len(sim.detpos)))))
     c.issaveexit = int(sim.issaveexit) if c.issavedet else 0
-    c.ismomentum = 0
+    c.ismomentum = int(getattr(sim, "ismomentum", 0) > 0)
     c.srctype = SOURCES[sim.srctype]
     c.srcnum = 1
     pos = list(sim.srcpos) + [0.0] * (4 - len(sim.srcpos))
@@ -267,7 +267,7 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     out = OOut()
     out.weight = p(field, C.c_double)
     out.weight_im = p(field_im, C.c_double) if rf else C.cast(None, f64p)
-    reclen = 2 * mesh.prop + 6 * cfg.issaveexit + 2
+    reclen = (2 + (cfg.ismomentum > 0)) * mesh.prop + 6 * cfg.issaveexit + 2
     det = np.zeros((max(maxdet, 1), reclen), np.float32)
     detph = np.zeros(max(maxdet, 1), np.int64)
     out.detdata = p(det, C.c_float) if maxdet else C.cast(None, f32p)

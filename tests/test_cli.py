@@ -27,7 +27,7 @@ def test_help_and_errors(tmp_path):
     assert r.returncode != 0 and "unsupported compression" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-F", "nifti"], tmp_path)
     assert r.returncode != 0 and "unsupported output format" in r.stderr
-    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--momentum", "1"], tmp_path)
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "--implicit", "1"], tmp_path)
     assert r.returncode != 0 and "unsupported option" in r.stderr
     r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-E", str(tmp_path / "none.mch")], tmp_path)
     assert r.returncode != 0
@@ -211,3 +211,17 @@ def test_adjoint_jacobians_from_the_command_line(tmp_path):
     js.write_text(json.dumps(cfg))
     r = run(["-f", str(js), "-n", "1000", "-O", "a", "-s", "bad"], tmp_path)
     assert r.returncode != 0 and "Dir" in r.stderr
+
+
+@pytest.mark.gpu
+def test_momentum_transfer_columns_in_the_mch(tmp_path):
+    # -m 1: a third block of maxmedia columns after the partial paths, and the
+    # M bit in savedetflag
+    r = run(["-f", os.path.join(GOLDEN, "cube2", "cube2.json"), "-n", "20000", "-M", "S", "-C", "0", "-d", "1",
+             "-m", "1", "-s", "mom"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    head, data = read_mch(tmp_path / "mom.mch")
+    prop = head[2]
+    assert head[4] == 1 + 3 * prop + 1 and head[-3] & 0x08
+    nscat, mom = data[:, 1:1 + prop], data[:, 1 + 2 * prop:1 + 3 * prop]
+    assert len(data) > 0 and np.all(mom >= 0) and np.all(mom <= 2 * nscat + 1e-3) and mom.sum() > 0

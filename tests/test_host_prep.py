@@ -100,7 +100,7 @@ def test_library_exports_every_declared_function(header):
     lib = C.CDLL(_lib.LIBPATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
-    assert _lib.lib().mmc_hip_abi_version() == _lib.ABI_VERSION == 7
+    assert _lib.lib().mmc_hip_abi_version() == _lib.ABI_VERSION == 8
 
 
 def test_photon_seeds_are_distinct_and_nonzero():
