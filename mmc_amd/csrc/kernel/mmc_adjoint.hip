@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include "mmc_hip.h"
+#include "mmc_kernel.h"
 
 namespace {
 
@@ -241,11 +242,13 @@ struct Dev {
 extern "C" int32_t mmc_hip_adjoint_grid(const float* field, const float* field_im, const int32_t dim[3],
                                         int32_t maxgate, int32_t ns, int32_t nd, float unitinmm, float* jmua,
                                         float* jd) {
-    if (!field || !dim || maxgate < 1 || ns < 1 || nd < 1 || (!jmua && !jd)) return -1;
-    if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1) return -1;
+    if (!field || !dim || maxgate < 1 || ns < 1 || nd < 1 || (!jmua && !jd))
+        return mmc::report_error(-1, "adjoint: needs fields, maxgate, ns, nd >= 1 and an output");
+    if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1) return mmc::report_error(-1, "adjoint: empty grid");
     const uint64_t ncell = (uint64_t)dim[0] * dim[1] * dim[2];
     const uint64_t nslot = (uint64_t)ns + nd;
-    if ((uint64_t)maxgate * nslot >= (1ull << 32) || ncell >= (1ull << 40)) return -1;
+    if ((uint64_t)maxgate * nslot >= (1ull << 32) || ncell >= (1ull << 40))
+        return mmc::report_error(-1, "adjoint: grid or slot count too large");
     const uint64_t nin = ncell * maxgate * nslot, nout = ncell * ns * nd * (field_im ? 2 : 1);
     Dev dre, dim_, dmua, dd;
     hipError_t e = hipMalloc(&dre.p, sizeof(float) * nin);
@@ -254,7 +257,7 @@ extern "C" int32_t mmc_hip_adjoint_grid(const float* field, const float* field_i
     if (!e && jd) e = hipMalloc(&dd.p, sizeof(float) * nout);
     if (!e) e = hipMemcpy(dre.p, field, sizeof(float) * nin, hipMemcpyHostToDevice);
     if (!e && field_im) e = hipMemcpy(dim_.p, field_im, sizeof(float) * nin, hipMemcpyHostToDevice);
-    if (e) return -100 - (int)e;
+    if (e) return mmc::report_error(-100 - (int)e, hipGetErrorString(e));
     /* scaling of src/mmc_cl_host.c:1436-1461: -Vvox for J_mua, -unitinmm for J_D */
     const float u = unitinmm > 0.f ? unitinmm : 1.f;
     const float vvox = u * u * u;
@@ -266,7 +269,7 @@ extern "C" int32_t mmc_hip_adjoint_grid(const float* field, const float* field_i
     e = hipGetLastError();
     if (!e && jmua) e = hipMemcpy(jmua, dmua.p, sizeof(float) * nout, hipMemcpyDeviceToHost);
     if (!e && jd) e = hipMemcpy(jd, dd.p, sizeof(float) * nout, hipMemcpyDeviceToHost);
-    return e ? -100 - (int)e : 0;
+    return e ? mmc::report_error(-100 - (int)e, hipGetErrorString(e)) : 0;
 }
 
 namespace {
@@ -284,20 +287,23 @@ extern "C" int32_t mmc_hip_adjoint_mesh(const float* field, const float* field_i
                                         const int32_t* elem, const float* evol, const float* deldotdel,
                                         const float* nvol, int32_t maxgate, int32_t ns, int32_t nd, int32_t mode,
                                         float* jmua, float* jd) {
-    if (!field || !elem || nn < 1 || ne < 1 || maxgate < 1 || ns < 1 || nd < 1 || (!jmua && !jd)) return -1;
-    if (mode < 0 || mode > 2) return -1;
+    if (!field || !elem || nn < 1 || ne < 1 || maxgate < 1 || ns < 1 || nd < 1 || (!jmua && !jd))
+        return mmc::report_error(-1, "adjoint: needs fields, a mesh, maxgate, ns, nd >= 1 and an output");
+    if (mode < 0 || mode > 2) return mmc::report_error(-1, "adjoint: mode must be 0, 1 or 2");
     const bool approx = mode == 1, elemout = mode == 2;
     const bool fem_mua = jmua && !approx;
-    if ((fem_mua && !evol) || (jd && !deldotdel) || (jmua && approx && !nvol)) return -1;
+    if ((fem_mua && !evol) || (jd && !deldotdel) || (jmua && approx && !nvol))
+        return mmc::report_error(-1, "adjoint: element volumes, gradient products or node volumes missing");
     const uint64_t nslot = (uint64_t)ns + nd;
-    if ((uint64_t)maxgate * nslot >= (1ull << 32) || (uint64_t)ns * nd >= (1ull << 32)) return -1;
+    if ((uint64_t)maxgate * nslot >= (1ull << 32) || (uint64_t)ns * nd >= (1ull << 32))
+        return mmc::report_error(-1, "adjoint: slot count too large");
     for (uint64_t k = 0; k < 4 * (uint64_t)ne; k++)
-        if (elem[k] < 1 || elem[k] > nn) return -1;
+        if (elem[k] < 1 || elem[k] > nn) return mmc::report_error(-1, "adjoint: element node ids outside 1..nn");
     const int mult = field_im ? 2 : 1;
     const uint64_t nin = (uint64_t)nn * maxgate * nslot;
     const uint64_t nblk = (uint64_t)ns * nd * mult;
     const uint64_t nelemval = (uint64_t)ne * nblk, nout = (elemout ? (uint64_t)ne : (uint64_t)nn) * nblk;
-    if (nblk >= (1ull << 32)) return -1;
+    if (nblk >= (1ull << 32)) return mmc::report_error(-1, "adjoint: too many output blocks");
 
     Dev dre, dimg, delem, devol, dddd, dnvol, demua, ded, dstart, dlist, dmua, dd;
     hipError_t e = upload(dre, field, nin);
@@ -316,7 +322,7 @@ extern "C" int32_t mmc_hip_adjoint_mesh(const float* field, const float* field_i
         if (!start || !list) {
             free(start);
             free(list);
-            return -2;
+            return mmc::report_error(-2, "adjoint: out of host memory");
         }
         for (uint64_t k = 0; k < 4 * (uint64_t)ne; k++) start[elem[k]]++;
         for (int32_t n = 0; n < nn; n++) start[n + 1] += start[n];
@@ -331,7 +337,7 @@ extern "C" int32_t mmc_hip_adjoint_mesh(const float* field, const float* field_i
     }
     if (!e && jmua && !elemout) e = hipMalloc(&dmua.p, sizeof(float) * nout);
     if (!e && jd && !elemout) e = hipMalloc(&dd.p, sizeof(float) * nout);
-    if (e) return -100 - (int)e;
+    if (e) return mmc::report_error(-100 - (int)e, hipGetErrorString(e));
 
     const unsigned int block = 256;
     const unsigned int gne = (unsigned int)(((uint64_t)ne + block - 1) / block);
@@ -358,5 +364,5 @@ extern "C" int32_t mmc_hip_adjoint_mesh(const float* field, const float* field_i
     const void* sd = elemout ? ded.p : dd.p;
     if (!e && jmua) e = hipMemcpy(jmua, smua, sizeof(float) * nout, hipMemcpyDeviceToHost);
     if (!e && jd) e = hipMemcpy(jd, sd, sizeof(float) * nout, hipMemcpyDeviceToHost);
-    return e ? -100 - (int)e : 0;
+    return e ? mmc::report_error(-100 - (int)e, hipGetErrorString(e)) : 0;
 }

@@ -42,6 +42,9 @@ struct SimHot {
 };
 
 /* Per-simulation parameters read at launch / exit / deposit (global, cached). */
+/* sets the message mmc_hip_last_error returns; returns code */
+int report_error(int code, const char* msg);
+
 struct SimCold {
     float4 srcpos, srcdir, srcparam1, srcparam2, bary0;
     float tstep, nout, minenergy, roulettesize;

@@ -35,6 +35,9 @@ static int set_err(int code, const char* fmt, ...) {
     return code;
 }
 
+/* the message mmc_hip_last_error returns, for the other entry points */
+int mmc::report_error(int code, const char* msg) { return set_err(code, "%s", msg); }
+
 #define HIPCHK(x)                                                                         \
     do {                                                                                  \
         hipError_t e_ = (x);                                                              \

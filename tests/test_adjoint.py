@@ -189,3 +189,20 @@ def test_gpu_mesh_adjoint_run_on_a_cube():
     j = MA.jacobian_mesh(tm, f, None, 1, 2)
     assert np.all(j.mua <= 0) and j.mua.min() < 0
     assert j.mua[0, 0].sum() < j.mua[0, 1].sum()
+
+
+def test_adjoint_entry_points_refuse_bad_arguments():
+    # checked before any device call, so this runs without a GPU
+    import ctypes as C
+    from mmc_amd import _lib
+    L = _lib.lib()
+    f = np.zeros(8, np.float32)
+    dim = np.array([2, 2, 2], np.int32)
+    out = np.zeros(8, np.float32)
+    rc = L.mmc_hip_adjoint_grid(_lib.ptr(f, C.c_float), None, _lib.ptr(dim, C.c_int32), 1, 0, 1, C.c_float(1.0),
+                                _lib.ptr(out, C.c_float), None)
+    assert rc == -1 and "ns, nd" in _lib.last_error()
+    elem = np.array([[1, 2, 3, 9]], np.int32)
+    rc = L.mmc_hip_adjoint_mesh(_lib.ptr(f, C.c_float), None, 4, 1, _lib.ptr(elem, C.c_int32),
+                                _lib.ptr(f, C.c_float), None, None, 1, 1, 1, 0, _lib.ptr(out, C.c_float), None)
+    assert rc == -1 and "outside 1..nn" in _lib.last_error()
