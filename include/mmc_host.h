@@ -77,6 +77,22 @@ int32_t mmc_host_grid(const float* node, int32_t nn, float step, float nmin[3], 
  * src/mmc_mesh.c:990-1046).  0, or -1 for a node id outside 1..nn. */
 int32_t mmc_host_deldotdel(const float* node, int32_t nn, const int32_t* elem, int32_t ne, const float* evol,
                            double* ddd);
+/* Area detector over the plane at detpos[0].xy spanned by detparam1.xy and
+ * detparam2.xy, detparam1.w x detparam2.w pixels.  mmc_host_detimage (-x 2;
+ * mesh_getdetimage, src/mmc_mesh.c:2029-2078) adds each detected record's
+ * weight w0 * prod exp(-mua_j L_j unitinmm) to detmap[gate][y][x] (float,
+ * maxgate x ysize x xsize, zeroed by the caller), the gate from the optical
+ * path sum n_j L_j truncated to an integer as the reference does; records
+ * outside the plane are skipped.  rec rows: detid, nscat[prop], ppath[prop],
+ * (mom[prop]), exit p[3], v[3], w0 (colcount columns).  mmc_host_detweight
+ * (mesh_getdetweight, src/mmc_mesh.c:2121-2138) gives each record the
+ * detector pattern's value at its exit pixel (the replay weight of a
+ * patterned detector); -2 if a record lies outside the plane. */
+int32_t mmc_host_detimage(float* detmap, const float* rec, int64_t count, int32_t colcount, int32_t prop,
+                          const float* med, float unitinmm, float tstep, int32_t maxgate, const float detpos0[2],
+                          const float detparam1[4], const float detparam2[4]);
+int32_t mmc_host_detweight(float* weight, const float* rec, int64_t count, int32_t colcount, const float detpos0[2],
+                           const float detparam1[4], const float detparam2[4], const float* detpattern);
 /* Normalizes a field in place and returns the factor (mesh_normalize). */
 double mmc_host_normalize(const mmc_hip_sim* sim, int32_t nn, int32_t ne,
                           const int32_t* elem, const int32_t* type, const float* med,

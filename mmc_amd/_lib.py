@@ -127,6 +127,9 @@ SIGNATURES = {
     "mmc_host_detelem": (C.c_int32, [C.c_int32, i32p, C.c_int32]),
     "mmc_host_grid": (C.c_int32, [f32p, C.c_int32, C.c_float, f32p, i32p]),
     "mmc_host_srcelem": (C.c_int32, [C.c_int32, i32p, i32p]),
+    "mmc_host_detimage": (C.c_int32, [f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, C.c_float, C.c_float,
+                                      C.c_int32, f32p, f32p, f32p]),
+    "mmc_host_detweight": (C.c_int32, [f32p, f32p, C.c_int64, C.c_int32, f32p, f32p, f32p, f32p]),
     "mmc_host_deldotdel": (C.c_int32, [f32p, C.c_int32, i32p, C.c_int32, f32p, f64p]),
     "mmc_host_normalize_grid": (C.c_double, [C.POINTER(SimC), C.c_int64, f64p, C.c_double, C.c_double,
                                              C.c_int32]),

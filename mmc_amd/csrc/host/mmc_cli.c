@@ -47,6 +47,7 @@ typedef struct {
     float detpos[4 * MAXDET];
     float detdir[4 * MAXDET];  /* Optode.Detector[].Dir: adjoint sources' directions (w = focus) */
     int ndetdir;
+    float detparam1[4], detparam2[4]; /* area detector (Optode.Detector Param1/Param2): -x 2 images */
     char adjoint;              /* -O a|d|u|v|w|q: adjoint Jacobian output, 0 if none */
     int adjointmode;           /* mesh mode: 0 linear-basis integrals, 1 nodal approximation */
     float* pattern;
@@ -234,6 +235,11 @@ static void load_json(cli_cfg* c, const char* path) {
             for (const mj_node* d = dets->child; d && i < MAXDET; d = d->next, i++) {
                 vec_of(mj_get(d, "Pos"), c->detpos + 4 * i, 3);
                 c->detpos[4 * i + 3] = (float)mj_num(mj_get(d, "R"), 1.0);
+                if (i == 0) {
+                    const mj_node* q;
+                    if ((q = mj_get(d, "Param1"))) vec_of(q, c->detparam1, 4);
+                    if ((q = mj_get(d, "Param2"))) vec_of(q, c->detparam2, 4);
+                }
                 const mj_node* dd = mj_get(d, "Dir");
                 if (dd) {
                     vec_of(dd, c->detdir + 4 * i, 4);
@@ -1003,6 +1009,22 @@ int main(int argc, char** argv) {
             fclose(fp);
         }
         printf("detected %lld photons\n", (long long)res.detected);
+        if (c.sim.issaveexit == 2) {
+            /* -x 2: time-resolved image of the area detector (mesh_getdetimage,
+             * mesh_savedetimage: <session>.img, float [gate][y][x]) */
+            const int xs = (int)c.detparam1[3], ys = (int)c.detparam2[3];
+            if (xs < 1 || ys < 1) die("%s", "-x 2 needs the area detector's Param1/Param2 (pixel counts in w)");
+            float* img = (float*)calloc((size_t)xs * ys * c.sim.maxgate, sizeof(float));
+            if (!img || mmc_host_detimage(img, res.detdata, saved, reclen, prop, c.med, c.unitinmm, c.sim.tstep,
+                                          c.sim.maxgate, c.detpos, c.detparam1, c.detparam2))
+                die("%s", "can not form the detector image");
+            snprintf(fname, sizeof(fname), "%s.img", c.session);
+            FILE* fp = fopen(fname, "wb");
+            if (!fp || fwrite(img, sizeof(float), (size_t)xs * ys * c.sim.maxgate, fp) != (size_t)xs * ys * c.sim.maxgate)
+                die("can not write %s", fname);
+            fclose(fp);
+            free(img);
+        }
     }
     if (c.savetraj) {
         /* the trajectory file: the history header with colcount 6 and

@@ -20,6 +20,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -495,4 +496,58 @@ int32_t mmc_host_format_id(const char* name) {
     for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
         if (!strcmp(name, names[i])) return i;
     return -1;
+}
+
+/* ---- area detectors (post-processing of detected records) ---- */
+
+/* record columns: detid, nscat[prop], ppath[prop], (mom[prop]), p[3], v[3], w0 */
+static int det_bin(const float* row, int32_t colcount, const float detpos0[2], const float p1[4], const float p2[4],
+                   int* xi, int* yi) {
+    const float xrange = p1[0] + p2[0], yrange = p1[1] + p2[1];
+    const int xsize = (int)p1[3], ysize = (int)p2[3];
+    const float xloc = row[colcount - 7], yloc = row[colcount - 6];
+    *xi = (int)((xloc - detpos0[0]) / xrange * xsize);
+    *yi = (int)((yloc - detpos0[1]) / yrange * ysize);
+    return !(*xi < 0 || *xi > xsize - 1 || *yi < 0 || *yi > ysize - 1);
+}
+
+int32_t mmc_host_detimage(float* detmap, const float* rec, int64_t count, int32_t colcount, int32_t prop,
+                          const float* med, float unitinmm, float tstep, int32_t maxgate, const float detpos0[2],
+                          const float detparam1[4], const float detparam2[4]) {
+    if (!detmap || (!rec && count > 0) || !med || !detpos0 || !detparam1 || !detparam2 || prop < 1 ||
+        colcount < 2 * prop + 8 || maxgate < 1 || !(tstep > 0.f) || detparam1[3] < 1.f || detparam2[3] < 1.f)
+        return -1;
+    const float R_C0 = 3.335640951981520e-12f; /* src/mmc_const.h:45 */
+    const int xsize = (int)detparam1[3], ysize = (int)detparam2[3];
+    for (int64_t i = 0; i < count; i++) {
+        const float* row = rec + (size_t)i * colcount;
+        float path = 0.f, weight = row[colcount - 1];
+        for (int j = 1; j <= prop; j++) {
+            path += row[j + prop] * med[4 * j + 3];
+            weight *= expf(-row[j + prop] * med[4 * j] * unitinmm);
+        }
+        /* the reference truncates the optical path to an integer before
+         * scaling it to a gate (src/mmc_mesh.c:2053) */
+        int ntg = (int)((float)(int)path * R_C0 / tstep);
+        if (ntg > maxgate - 1) ntg = maxgate - 1;
+        int xi, yi;
+        if (!det_bin(row, colcount, detpos0, detparam1, detparam2, &xi, &yi)) continue;
+        detmap[(size_t)ntg * xsize * ysize + (size_t)yi * xsize + xi] += weight;
+    }
+    return 0;
+}
+
+int32_t mmc_host_detweight(float* weight, const float* rec, int64_t count, int32_t colcount, const float detpos0[2],
+                           const float detparam1[4], const float detparam2[4], const float* detpattern) {
+    if (!weight || (!rec && count > 0) || !detpos0 || !detparam1 || !detparam2 || !detpattern ||
+        detparam1[3] < 1.f || detparam2[3] < 1.f || colcount < 8)
+        return -1;
+    const int xsize = (int)detparam1[3];
+    for (int64_t i = 0; i < count; i++) {
+        int xi, yi;
+        if (!det_bin(rec + (size_t)i * colcount, colcount, detpos0, detparam1, detparam2, &xi, &yi))
+            return -2; /* "photon location not within the detection plane" */
+        weight[i] = detpattern[yi * xsize + xi];
+    }
+    return 0;
 }

@@ -644,7 +644,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         c.outputtype = s.outputtype;
         /* wide-field detectors replace the point detectors (src/mmc_mesh.c:402-406) */
         c.issavedet = (s.issavedet && (s.detnum > 0 || m->isextdet));
-        c.issaveexit = c.issavedet ? s.issaveexit : 0;
+        c.issaveexit = (c.issavedet && s.issaveexit > 0) ? 1 : 0;
         c.detnum = (c.issavedet && !m->isextdet) ? s.detnum : 0;
         c.detbase = (int)dets.size();
         for (int d = 0; d < c.detnum; d++)

@@ -97,11 +97,15 @@ def load_mch(path: str) -> MchFile:
 
 
 def replay_config(cfg: SimConfig, mch: MchFile, med: np.ndarray, outputtype: str = "x",
-                  replaydet: int = 0) -> SimConfig:
+                  replaydet: int = 0, detpattern=None, detparam1=None, detparam2=None) -> SimConfig:
     """The replay run of the saved photons of detector replaydet (0: all),
     with the replay weight w0 * prod_j exp(-mua_j * col_j * unitinmm) and
     time sum_j n_j * col_j * R_C0 over record columns j = 2..maxmedia+1 and
-    media 1..maxmedia, as mesh_loadseedfile computes them in float."""
+    media 1..maxmedia, as mesh_loadseedfile computes them in float.  With a
+    detector pattern (detparam1.w x detparam2.w pixels over the plane at the
+    first detector; records saved with exit positions) the starting weight is
+    the pattern's value at each photon's exit pixel instead of w0
+    (mesh_getdetweight, src/mmc_mesh.c:871-875, 2121-2138)."""
     if mch.seeds is None:
         raise ValueError("the history file holds no seeds (save them with the detected photons)")
     if mch.maxmedia != len(med) - 1:
@@ -112,6 +116,20 @@ def replay_config(cfg: SimConfig, mch: MchFile, med: np.ndarray, outputtype: str
     seeds = mch.seeds[keep]
     med = np.asarray(med, np.float32)
     rw = rec[:, -1].astype(np.float32).copy()
+    if detpattern is not None:
+        import ctypes as C
+        from . import _lib
+        r32 = np.ascontiguousarray(rec, np.float32)
+        pat = np.ascontiguousarray(detpattern, np.float32).ravel()
+        p1, p2 = np.asarray(detparam1, np.float32), np.asarray(detparam2, np.float32)
+        org = np.asarray(cfg.detpos, np.float32).reshape(-1, 4)[0, :2].copy()
+        if pat.size < int(p1[3]) * int(p2[3]):
+            raise ValueError("detector pattern smaller than detparam1.w x detparam2.w")
+        rc = _lib.lib().mmc_host_detweight(_lib.ptr(rw, C.c_float), _lib.ptr(r32, C.c_float), len(r32),
+                                           r32.shape[1], _lib.ptr(org, C.c_float), _lib.ptr(p1, C.c_float),
+                                           _lib.ptr(p2, C.c_float), _lib.ptr(pat, C.c_float))
+        if rc != 0:
+            raise ValueError("a photon's exit lies outside the detection plane (or records lack exit positions)")
     rt = np.zeros(len(rec), np.float32)
     unit = np.float32(mch.unitinmm)
     for j in range(2, mch.maxmedia + 2):

@@ -225,3 +225,30 @@ def test_momentum_transfer_columns_in_the_mch(tmp_path):
     assert head[4] == 1 + 3 * prop + 1 and head[-3] & 0x08
     nscat, mom = data[:, 1:1 + prop], data[:, 1 + 2 * prop:1 + 3 * prop]
     assert len(data) > 0 and np.all(mom >= 0) and np.all(mom <= 2 * nscat + 1e-3) and mom.sum() > 0
+
+
+@pytest.mark.gpu
+def test_area_detector_image(tmp_path):
+    # -x 2 with Param1/Param2 on the first detector: <session>.img holds the
+    # detected weights binned by exit pixel and gate (float [gate][y][x])
+    cfg = json.load(open(os.path.join(GOLDEN, "cube2", "cube2.json")))
+    cfg["Optode"]["Detector"][0].update({"Param1": [20.0, 0.0, 0.0, 10], "Param2": [0.0, 20.0, 0.0, 5]})
+    cfg["Optode"]["Detector"][0]["Pos"] = [20.0, 20.0, 0.0]
+    cfg["Optode"]["Detector"][0]["R"] = 15.0
+    js = tmp_path / "img.json"
+    js.write_text(json.dumps(cfg))
+    r = run(["-f", str(js), "-n", "50000", "-M", "S", "-C", "0", "-d", "1", "-x", "2", "-s", "img"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    head, data = read_mch(tmp_path / "img.mch")
+    img = np.fromfile(tmp_path / "img.img", np.float32)
+    maxgate = 50
+    assert img.size == maxgate * 5 * 10 and img.sum() > 0
+    prop = head[2]
+    x, y = data[:, -7], data[:, -6]
+    # pixel index truncated toward zero, as the reference's float-to-int conversion
+    xi = np.trunc((x - np.float32(20)) / np.float32(20) * np.float32(10))
+    yi = np.trunc((y - np.float32(20)) / np.float32(20) * np.float32(5))
+    inside = (xi >= 0) & (xi <= 9) & (yi >= 0) & (yi <= 4)
+    med = np.array([m["mua"] for m in cfg["Domain"]["Media"]], np.float32)
+    w = data[:, -1] * np.exp(-(data[:, 1 + prop:1 + 2 * prop] * med[1:prop + 1]).sum(1))
+    assert img.sum() == pytest.approx(w[inside].sum(), rel=1e-4)
