@@ -48,6 +48,7 @@ typedef struct {
     float detdir[4 * MAXDET];  /* Optode.Detector[].Dir: adjoint sources' directions (w = focus) */
     int ndetdir;
     float detparam1[4], detparam2[4]; /* area detector (Optode.Detector Param1/Param2): -x 2 images */
+    char detpatternfile[1024];        /* .inp: detector pattern weighting the L/P replays */
     char adjoint;              /* -O a|d|u|v|w|q: adjoint Jacobian output, 0 if none */
     int adjointmode;           /* mesh mode: 0 linear-basis integrals, 1 nodal approximation */
     float* pattern;
@@ -365,7 +366,19 @@ static void load_inp(cli_cfg* c, const char* path) {
                 c->sim.srcpattern = c->pattern;
             }
         }
-        if (detnum == 1 && c->detpos[3] == 0.f) die("%s", "wide-field detectors are not supported in this build");
+        if (detnum == 1 && c->detpos[3] == 0.f) {
+            /* a wide-field detector plane: its two axes (pixel counts in w),
+             * then optionally the pattern file used by the L/P replays
+             * (src/mmc_utils.c:2612-2660) */
+            float* q1 = c->detparam1;
+            float* q2 = c->detparam2;
+            INP_READ(4, "%f %f %f %f", q1, q1 + 1, q1 + 2, q1 + 3);
+            INP_REST();
+            INP_READ(4, "%f %f %f %f", q2, q2 + 1, q2 + 2, q2 + 3);
+            INP_REST();
+            if (q1[3] * q2[3] > 0 && fscanf(fp, "%1023s", word) == 1)
+                snprintf(c->detpatternfile, sizeof(c->detpatternfile), "%s", word);
+        }
     }
 #undef INP_READ
 #undef INP_REST
@@ -431,6 +444,10 @@ static void dump_json(const cli_cfg* c) {
         if (i < c->ndetdir)
             fprintf(fp, ", \"Dir\": [%.9g, %.9g, %.9g, %.9g]", (double)c->detdir[4 * i], (double)c->detdir[4 * i + 1],
                     (double)c->detdir[4 * i + 2], (double)c->detdir[4 * i + 3]);
+        if (i == 0 && c->detparam1[3] * c->detparam2[3] > 0)
+            fprintf(fp, ", \"Param1\": [%.9g, %.9g, %.9g, %.9g], \"Param2\": [%.9g, %.9g, %.9g, %.9g]",
+                    (double)c->detparam1[0], (double)c->detparam1[1], (double)c->detparam1[2], (double)c->detparam1[3],
+                    (double)c->detparam2[0], (double)c->detparam2[1], (double)c->detparam2[2], (double)c->detparam2[3]);
         fprintf(fp, "}");
     }
     fprintf(fp, "\n\t\t]\n\t}\n}\n");
@@ -504,6 +521,18 @@ static void load_seedfile(cli_cfg* c, int prop) {
     c->rseeds = (uint64_t*)malloc(sizeof(uint64_t) * 2 * n);
     c->rweight = (float*)malloc(sizeof(float) * n);
     c->rtime = (float*)malloc(sizeof(float) * n);
+    /* a patterned wide-field detector weights the L/P replays by the
+     * pattern at each photon's exit pixel (src/mmc_mesh.c:871-875) */
+    float* detpat = NULL;
+    if (c->detpatternfile[0] && (c->sim.outputtype == MMC_OUT_WL || c->sim.outputtype == MMC_OUT_WP) &&
+        c->detparam1[3] * c->detparam2[3] > 0) {
+        const size_t np = (size_t)c->detparam1[3] * (size_t)c->detparam2[3];
+        FILE* pf = fopen(c->detpatternfile, "rb");
+        detpat = (float*)calloc(np, sizeof(float));
+        if (!pf || !detpat || fread(detpat, sizeof(float), np, pf) != np)
+            die("can not read the detector pattern %s", c->detpatternfile);
+        fclose(pf);
+    }
     const float R_C0 = 3.335640951981520e-12f; /* src/mmc_const.h:45 */
     size_t m = 0;
     for (size_t i = 0; i < n; i++) {
@@ -512,12 +541,15 @@ static void load_seedfile(cli_cfg* c, int prop) {
         c->rseeds[2 * m] = seeds[2 * i];
         c->rseeds[2 * m + 1] = seeds[2 * i + 1];
         float w = r[col - 1], t = 0.f;
+        if (detpat && mmc_host_detweight(&w, r, 1, (int32_t)col, c->detpos, c->detparam1, c->detparam2, detpat))
+            die("%s", "photon location not within the detection plane");
         for (int j = 2; j < prop + 2; j++) w *= expf(-c->med[4 * (j - 1)] * r[j] * his.unitinmm);
         for (int j = 2; j < prop + 2; j++) t += c->med[4 * (j - 1) + 3] * r[j] * R_C0;
         c->rweight[m] = w;
         c->rtime[m] = t;
         m++;
     }
+    free(detpat);
     free(ppath);
     free(seeds);
     if (m == 0) die("%s", "no saved photon of the replayed detector");

@@ -252,3 +252,20 @@ def test_area_detector_image(tmp_path):
     med = np.array([m["mua"] for m in cfg["Domain"]["Media"]], np.float32)
     w = data[:, -1] * np.exp(-(data[:, 1 + prop:1 + 2 * prop] * med[1:prop + 1]).sum(1))
     assert img.sum() == pytest.approx(w[inside].sum(), rel=1e-4)
+
+
+def test_inp_wide_field_detector_plane(tmp_path):
+    # one detector of radius 0 is a wide-field plane: two axis lines (pixel
+    # counts in w) and the pattern file follow (src/mmc_utils.c:2612-2660)
+    import shutil
+    for f in ("node_onecube.dat", "elem_onecube.dat", "face_onecube.dat", "prop_onecube.dat"):
+        src = os.path.join(GOLDEN, "onecube", f)
+        if os.path.exists(src):
+            shutil.copy(src, tmp_path / f)
+    inp = tmp_path / "wf.inp"
+    inp.write_text("100\n17182818\n2 8 0.0\n0. 0. 1.\n0.e+00 5.e-09 5e-10\nonecube\n3\n1 0\n0.0 0.0 0.0\n"
+                   "pencil\n0 0 0 0\n0 0 0 0\n4 0 0 8\n0 6 0 3\ndetpat.bin\n")
+    r = run(["-f", str(inp), "--dumpjson", "wf.json"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    d = json.load(open(tmp_path / "wf.json"))["Optode"]["Detector"][0]
+    assert d["R"] == 0 and d["Param1"] == [4, 0, 0, 8] and d["Param2"] == [0, 6, 0, 3]
