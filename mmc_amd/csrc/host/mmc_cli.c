@@ -261,6 +261,7 @@ static void load_json(cli_cfg* c, const char* path) {
         c->sim.isspecular = (int)mj_num(mj_get(Session, "DoSpecular"), c->sim.isspecular);
         c->sim.issaveexit = (int)mj_num(mj_get(Session, "DoSaveExit"), c->sim.issaveexit);
         c->sim.ismomentum = (int)mj_num(mj_get(Session, "DoDCS"), c->sim.ismomentum);
+        c->issaveseed = (int)mj_num(mj_get(Session, "DoSaveSeed"), c->issaveseed);
         c->sim.basisorder = (int)mj_num(mj_get(Session, "BasisOrder"), c->sim.basisorder);
         if ((v = mj_get(Session, "OutputFormat"))) {
             c->outputformat = mmc_host_format_id(mj_str(v, "ascii"));
@@ -398,9 +399,9 @@ static void dump_json(const cli_cfg* c) {
             c->session, (long long)s->nphoton, (unsigned)c->seed);
     fprintf(fp, "\t\t\"DoMismatch\": %d,\n\t\t\"DoSaveVolume\": %d,\n\t\t\"DoNormalize\": %d,\n"
                 "\t\t\"DoPartialPath\": %d,\n\t\t\"DoSaveExit\": %d,\n\t\t\"DoSaveSeed\": %d,\n"
-                "\t\t\"DoSpecular\": %d,\n\t\t\"DebugFlag\": %d,\n\t\t\"BasisOrder\": %d,\n",
+                "\t\t\"DoSpecular\": %d,\n\t\t\"DoDCS\": %d,\n\t\t\"DebugFlag\": %d,\n\t\t\"BasisOrder\": %d,\n",
             s->isreflect, c->issave2pt, c->isnormalized, s->issavedet, s->issaveexit, c->issaveseed, s->isspecular,
-            c->savetraj ? MMC_DL_TRAJ : 0, s->basisorder);
+            s->ismomentum, c->savetraj ? MMC_DL_TRAJ : 0, s->basisorder);
     const char* fmts[] = {"ascii", "bin", "nii", "hdr", "mc2", "tx3", "jnii", "bnii"};
     fprintf(fp, "\t\t\"RayTracer\": \"%c\",\n\t\t\"OutputFormat\": \"%s\",\n\t\t\"OutputType\": \"%c\"\n\t},\n",
             c->method, fmts[c->outputformat],
