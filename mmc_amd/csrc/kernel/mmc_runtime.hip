@@ -386,6 +386,8 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
             (double)p1[0] * p1[1] * p1[2] > 2e9)
             return set_err(-1, "3-D pattern source needs a pattern of srcparam1.x*y*z (>= 1 each) voxels");
     }
+    if (s->srcnum > 1 && s->srctype != MMC_SRC_PATTERN)
+        return set_err(-1, "shared patterns (srcnum %d) need a pattern source", s->srcnum);
     if (s->srctype == MMC_SRC_PENCILARRAY && !(s->srcparam1[3] >= 2.f && s->srcparam2[3] >= 2.f))
         return set_err(-1, "pencil-array source needs at least 2 beams along each axis (srcparam1.w, srcparam2.w)");
     if (s->srctype == MMC_SRC_CONE) {

@@ -320,6 +320,9 @@ def test_invalid_settings_fail_with_a_message(slab):
         (tm, dict(srctype="pattern", srcparam1=(10, 0, 0, 4), srcparam2=(0, 10, 0, 4),
                   srcpattern=np.ones((9, 4, 4), np.float32)), "srcnum"),
         (tm, dict(isreflect=7), "boundary"),
+        (tm, dict(srctype="cone", srcparam1=(0.01, 0, 0, 0)), "half angle"),
+        (tm, dict(srctype="pencilarray", srcparam1=(10, 0, 0, 1), srcparam2=(0, 10, 0, 3)), "pencil-array"),
+        (tm, dict(srctype="pattern3d", srcparam1=(4, 4, 4, 0)), "3-D pattern"),
     ]
     for mesh, kw, msg in cases:
         base = dict(nphoton=100, srcpos=(50.0, 50.0, 0.0) if mesh is tm else (30.0, 30.0, 0.0),
