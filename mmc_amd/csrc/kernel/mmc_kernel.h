@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "mmc_hip.h"
+#include "mmc_host.h"
 
 namespace mmc {
 
@@ -165,6 +166,10 @@ __global__ void transport(KArgs a);
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
                              uint64_t repstride);
 __global__ void fill_zero_u64(unsigned long long* p, int n);
+/* host C library agreement tables (mmc_host_libm_table) into the current
+ * device's copy; 0 or a negative error */
+int upload_libm_tables(const uint8_t* tlog, const uint8_t* taz, const uint8_t* tiso, const uint8_t* tpi,
+                       const mmc_libm_escape* esc, int32_t nesc);
 
 }  // namespace mmc
 

@@ -76,8 +76,107 @@ __device__ __forceinline__ float rng_next(uint64_t& s0r, uint64_t& s1r) {
     uint32_t lo = (uint32_t)(s1r + s0);
     return __uint_as_float(0x3F800000u | (lo >> 9)) - 1.0f;
 }
+/* the generator output k (23 bits) behind a draw u = k / 2^23 */
+__device__ __forceinline__ uint32_t draw_bits(float u) { return __float_as_uint(u + 1.0f) & 0x7FFFFFu; }
+
+/* ---- host C library agreement (built by mmc_host_libm_table) ------------
+ * The reference draws scattering lengths, azimuths and isotropic zeniths
+ * through the host C library's logf/sincosf/acosf.  Each of those calls sees
+ * a function of one generator output k, so for every k a 2-bit code says how
+ * the C library's value differs from the deterministic FP32 evaluation of
+ * mmc_fp32math.h: 0 equal, 1 raw bits + 1, 2 raw bits - 1, 3 exact values in
+ * the escape list.  One copy per device, filled once per process. */
+__device__ uint8_t g_fix_log[MMC_FIXB_LOG];      /* logf(u + 1e-6f) */
+__device__ uint8_t g_fix_az[MMC_FIXB_AZ];        /* sincosf((float)(2 pi u)) */
+__device__ uint8_t g_fix_iso[MMC_FIXB_ISO];      /* acosf(2u - 1), then sincosf of it */
+__device__ uint8_t g_fix_pi[MMC_FIXB_PI];        /* sincosf((float)(pi u)) */
+__device__ mmc_libm_escape g_fix_esc[MMC_LIBM_MAXESC];
+__device__ int32_t g_fix_nesc;
+
+__device__ __forceinline__ float fix1(float y, uint32_t c) {
+    const uint32_t b = __float_as_uint(y);
+    return c == 1u ? __uint_as_float(b + 1u) : (c == 2u ? __uint_as_float(b - 1u) : y);
+}
+/* exact values of table `which` at k from the escape list */
+__device__ __noinline__ void fix_escape(uint32_t which, uint32_t k, float* v0, float* v1, float* v2) {
+    const int32_t key = (int32_t)((which << 23) | k);
+    const int n = min(g_fix_nesc, MMC_LIBM_MAXESC);
+    for (int i = 0; i < n; i++)
+        if (g_fix_esc[i].key == key) {
+            *v0 = __uint_as_float(g_fix_esc[i].v[0]);
+            *v1 = __uint_as_float(g_fix_esc[i].v[1]);
+            *v2 = __uint_as_float(g_fix_esc[i].v[2]);
+            return;
+        }
+}
+
+/* rand_next_scatlen (src/mmc_rand_common.h:50-51): -logf(u + EPS) */
 __device__ __forceinline__ float rng_scatlen(uint64_t& a, uint64_t& b) {
-    return -mmc_logf(rng_next(a, b) + EPSF);
+    const float u = rng_next(a, b);
+    const uint32_t k = draw_bits(u);
+    const uint32_t c = (g_fix_log[k >> 2] >> (2 * (k & 3))) & 3u;
+    float l = mmc_logf(u + EPSF);
+    if (c == 3u) {
+        float d1, d2;
+        fix_escape(MMC_FIX_LOG, k, &l, &d1, &d2);
+    } else {
+        l = fix1(l, c);
+    }
+    return -l;
+}
+/* sin and cos of (float)(2 pi u) or (float)(pi u) (which = MMC_FIX_AZ or
+ * MMC_FIX_PI), as the host C library rounds them */
+__device__ __forceinline__ void sincos_draw(uint32_t which, float u, float* s, float* c) {
+    const uint32_t k = draw_bits(u);
+    const uint8_t* tab = which == MMC_FIX_AZ ? g_fix_az : g_fix_pi;
+    const uint32_t code = (tab[k >> 1] >> (4 * (k & 1))) & 15u;
+    mmc_sincosf(which == MMC_FIX_AZ ? mmc_twopi(u) : mmc_pi(u), s, c);
+    if (code == 15u) {
+        float d;
+        fix_escape(which, k, s, c, &d);
+    } else {
+        *s = fix1(*s, code & 3u);
+        *c = fix1(*c, code >> 2);
+    }
+}
+/* theta = acosf(2u - 1) and its sin and cos (isotropic zenith), as the host
+ * C library rounds them */
+__device__ __forceinline__ float acos_draw(float u, float* s, float* c) {
+    const uint32_t k = draw_bits(u);
+    const uint32_t code = g_fix_iso[k];
+    float t = fix1(mmc_acosf(2.f * u - 1.f), code & 3u);
+    mmc_sincosf(t, s, c);
+    if ((code & 3u) == 3u) {
+        fix_escape(MMC_FIX_ISO, k, &t, s, c);
+    } else {
+        *s = fix1(*s, (code >> 2) & 3u);
+        *c = fix1(*c, (code >> 4) & 3u);
+    }
+    return t;
+}
+/* sin and cos of an angle that is not a function of one draw: correctly
+ * rounded through double precision (the host C library's sinf/cosf are
+ * within 0.56 ulp, so they agree wherever they round correctly) */
+__device__ __forceinline__ void sincos_cr(float x, float* s, float* c) {
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
+}
+
+/* Copies the host-built agreement tables into this device's copy. */
+int upload_libm_tables(const uint8_t* tlog, const uint8_t* taz, const uint8_t* tiso, const uint8_t* tpi,
+                       const mmc_libm_escape* esc, int32_t nesc) {
+    if (nesc < 0 || nesc > MMC_LIBM_MAXESC) return report_error(-1, "libm agreement: too many escapes");
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_fix_log), tlog, MMC_FIXB_LOG) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_az), taz, MMC_FIXB_AZ) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_iso), tiso, MMC_FIXB_ISO) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_pi), tpi, MMC_FIXB_PI) != hipSuccess ||
+        (nesc > 0 && hipMemcpyToSymbol(HIP_SYMBOL(g_fix_esc), esc, sizeof(mmc_libm_escape) * (size_t)nesc) !=
+                         hipSuccess) ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_nesc), &nesc, sizeof(int32_t)) != hipSuccess)
+        return report_error(-1, "libm agreement: table upload failed");
+    return 0;
 }
 
 /* rotatevector (src/mmc_mesh.h:336-357): turn (vx, vy, vz) by the zenith
@@ -156,7 +255,7 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
             } else if (st == MMC_SRC_FOURIER) {
                 const float kx = floorf(P1.w), ky = floorf(P2.w);
                 float sn, cs;
-                mmc_sincosf(mmc_twopi(kx * rx + ky * ry + P1.w - kx), &sn, &cs);
+                sincos_cr(mmc_twopi(kx * rx + ky * ry + P1.w - kx), &sn, &cs);
                 w = (cs * (1.f - P2.w + ky) + 1.f) * 0.5f;
             }
             ox += (P1.x + P2.x) * 0.5f;
@@ -175,33 +274,34 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
         pz = S.z + rx * P1.z + ry * uz;
         if (st == MMC_SRC_FOURIERX2D) {
             float sa, ca, sb, cb;
-            mmc_sincosf(mmc_twopi(P2.x * rx + P2.z), &sa, &ca);
-            mmc_sincosf(mmc_twopi(P2.y * ry + P2.w), &sb, &cb);
+            sincos_cr(mmc_twopi(P2.x * rx + P2.z), &sa, &ca);
+            sincos_cr(mmc_twopi(P2.y * ry + P2.w), &sb, &cb);
             w = (sa * sb + 1.f) * 0.5f;
         } else {
             float sn, cs;
-            mmc_sincosf(mmc_twopi(P2.x * rx + P2.y * ry + P2.z), &sn, &cs);
+            sincos_cr(mmc_twopi(P2.x * rx + P2.y * ry + P2.z), &sn, &cs);
             w = (cs * (1.f - P2.w) + 1.f) * 0.5f;
         }
         ox += (P1.x + ux) * 0.5f;
         oy += (P1.y + uy) * 0.5f;
         oz += (P1.z + uz) * 0.5f;
     } else if (st == MMC_SRC_DISK || st == MMC_SRC_RING || st == MMC_SRC_GAUSSIAN) {
-        float phi;
-        if (st == MMC_SRC_RING && (P1.z > 0.f || P1.w > 0.f))
-            phi = fabsf(P1.z - P1.w) * rng_next(r0, r1) + fminf(P1.z, P1.w);
-        else
-            phi = mmc_twopi(rng_next(r0, r1));
         float sphi, cphi;
-        mmc_sincosf(phi, &sphi, &cphi);
+        if (st == MMC_SRC_RING && (P1.z > 0.f || P1.w > 0.f))
+            sincos_cr(fabsf(P1.z - P1.w) * rng_next(r0, r1) + fminf(P1.z, P1.w), &sphi, &cphi);
+        else
+            sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
         float rr;
+        /* the Gaussian radius takes the double-precision log of the draw,
+         * converted to float for sqrtf (src/mmc_raytrace.c:2444-2449) */
         if (st != MMC_SRC_GAUSSIAN) {
             rr = sqrtf(rng_next(r0, r1) * fabsf(P1.x * P1.x - P1.y * P1.y) + P1.y * P1.y);
         } else if (fabsf(D.w) < 1e-5f || fabsf(P1.y) < 1e-5f) {
-            rr = sqrtf(-mmc_logf(rng_next(r0, r1))) * P1.x;
+            rr = sqrtf((float)(-log((double)rng_next(r0, r1)))) * P1.x;
         } else {
             const float z0 = (float)((double)(P1.x * P1.x) * 3.141592653589793 / (double)P1.y);  /* Rayleigh range */
-            rr = sqrtf(-mmc_logf(rng_next(r0, r1)) * (1.f + (D.w * D.w / (z0 * z0)))) * P1.x;
+            const float f = 1.f + (D.w * D.w / (z0 * z0));
+            rr = sqrtf((float)(-log((double)rng_next(r0, r1)) * (double)f)) * P1.x;
         }
         if (D.z > -1.f + EPSF && D.z < 1.f - EPSF) {
             const float t0 = 1.f - D.z * D.z;
@@ -214,37 +314,44 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
             py = S.y + rr * sphi;
         }
     } else if (st == MMC_SRC_CONE || st == MMC_SRC_ISOTROPIC || st == MMC_SRC_ARCSINE) {
-        float sphi, cphi;
-        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
-        float ang;
+        float sphi, cphi, sth, cth;
+        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
         if (st == MMC_SRC_CONE) {
             int k = 0;
+            float ang, uu;
             do {
-                ang = (P1.y > 0.f) ? mmc_twopi(rng_next(r0, r1)) : mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
+                uu = rng_next(r0, r1);
+                ang = (P1.y > 0.f) ? mmc_twopi(uu) : acos_draw(uu, &sth, &cth);
             } while (ang > P1.x && ++k < kConeTries);
-            ang = (ang > P1.x) ? P1.x : ang;
+            if (ang > P1.x)
+                sincos_cr(P1.x, &sth, &cth);
+            else if (P1.y > 0.f)
+                sincos_draw(MMC_FIX_AZ, uu, &sth, &cth);
         } else if (st == MMC_SRC_ISOTROPIC) {
-            ang = mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
+            acos_draw(rng_next(r0, r1), &sth, &cth);
         } else {
-            ang = mmc_pi(rng_next(r0, r1));
+            sincos_draw(MMC_FIX_PI, rng_next(r0, r1), &sth, &cth);
         }
-        float sth, cth;
-        mmc_sincosf(ang, &sth, &cth);
         rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
         return false;
     } else if (st == MMC_SRC_ZGAUSSIAN) {
         float sphi, cphi;
-        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
+        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
+        /* the second draw comes from the photon's own stream: the
+         * reference takes it from ran0, which in mmc_run_mp is a
+         * thread-private generator state that rng_init never seeds
+         * (src/mmc_host.c:227,265,280-283; src/mmc_rand_xorshift128p.c:82-84),
+         * so its values cannot be reproduced */
         const float u1 = rng_next(r0, r1);
         const float u2 = rng_next(r0, r1);
-        const float ang = sqrtf(-2.f * mmc_logf(u1)) * (1.f - 2.f * u2) * P1.x;
+        const float ang = sqrtf((float)(-2.0 * log((double)u1))) * (1.f - 2.f * u2) * P1.x;
         float sth, cth;
-        mmc_sincosf(ang, &sth, &cth);
+        sincos_cr(ang, &sth, &cth);
         rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
         canfocus = false;
     } else if (st == MMC_SRC_HYPERBOLOID) {
         float sphi, cphi;
-        mmc_sincosf(mmc_twopi(rng_next(r0, r1)), &sphi, &cphi);
+        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
         const float rr = sqrtf(0.5f * rng_scatlen(r0, r1)) * P1.x;   /* waist radius */
         const float tilt = (P1.z > 0.f) ? (-P1.y / P1.z) : 0.f;
         float bx = -rr * sphi, by = rr * cphi, bz = P1.z;
@@ -282,19 +389,17 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
         py = S.y + t * P1.y;
         pz = S.z + t * P1.z;
         if (st == MMC_SRC_LINE) {
-            /* a direction across the line; normalized here (the reference
-             * keeps the unnormalized cross product, which scales path lengths) */
+            /* a direction across the line: the cross product as the
+             * reference leaves it, not normalized (src/mmc_raytrace.c:2552-2562);
+             * the max() only keeps a rounding-negative radicand from
+             * producing NaN */
             const float tt = 1.f - 2.f * rng_next(r0, r1);
             const float ss = 1.f - 2.f * rng_next(r0, r1);
             const float pp = sqrtf(fmaxf(1.f - vx * vx - vy * vy, 0.f)) * (rng_next(r0, r1) > 0.5f ? 1.f : -1.f);
             const float qx = vy * pp - vz * ss, qy = vz * tt - vx * pp, qz = vx * ss - vy * tt;
-            const float n2 = qx * qx + qy * qy + qz * qz;
-            if (n2 > 0.f) {
-                const float n = 1.f / sqrtf(n2);
-                vx = qx * n;
-                vy = qy * n;
-                vz = qz * n;
-            }
+            vx = qx;
+            vy = qy;
+            vz = qz;
         }
         ox += P1.x * 0.5f;
         oy += P1.y * 0.5f;
@@ -1199,8 +1304,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     const float g = med(ctype).z;
                     float sphi, cphi, stheta, ctheta, tmp0, tmp1;
                     float nextslen = rng_scatlen(r0, r1);
-                    tmp0 = mmc_twopi(rng_next(r0, r1));
-                    mmc_sincosf(tmp0, &sphi, &cphi);
+                    sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
                     if (g > EPSF) {
                         tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * rng_next(r0, r1));
                         tmp0 *= tmp0;
@@ -1210,8 +1314,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         stheta = sqrtf(1.f - tmp0 * tmp0);
                         ctheta = tmp0;
                     } else {
-                        float theta = mmc_acosf(2.f * rng_next(r0, r1) - 1.f);
-                        mmc_sincosf(theta, &stheta, &ctheta);
+                        acos_draw(rng_next(r0, r1), &stheta, &ctheta);
                     }
                     float qx, qy, qz;
                     if (vz > -1.f + EPSF && vz < 1.f - EPSF) {

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <chrono>
 #include <vector>
@@ -84,6 +85,43 @@ extern "C" int32_t mmc_hip_device_info(char* name, int32_t len, int64_t* hbm_byt
 
 extern "C" void mmc_hip_photon_seeds(uint64_t seed, int64_t first, int64_t n, uint64_t* out) {
     for (int64_t i = 0; i < n; i++) photon_seed(seed, (uint64_t)(first + i), out[2 * i], out[2 * i + 1]);
+}
+
+/* ---- host C library agreement tables --------------------------------------
+ * Built once per process from the C library this process runs with
+ * (mmc_host_libm_table) and copied once into each device's copy of the
+ * kernel's tables, before that device's first launch.  Host copies are
+ * static, so later launches allocate nothing for them. */
+static std::mutex g_libm_mu;
+static uint8_t g_tab_log[MMC_FIXB_LOG], g_tab_az[MMC_FIXB_AZ], g_tab_iso[MMC_FIXB_ISO], g_tab_pi[MMC_FIXB_PI];
+static mmc_libm_escape g_tab_esc[MMC_LIBM_MAXESC];
+static int g_tab_nesc = -1;
+static bool g_tab_dev[64];
+
+static int ensure_libm_tables() {
+    std::lock_guard<std::mutex> lk(g_libm_mu);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err(-1, "device index %d out of range", dev);
+    if (g_tab_dev[dev]) return 0;
+    if (g_tab_nesc < 0) {
+        if (!__builtin_cpu_supports("fma"))
+            return set_err(-1, "the host CPU lacks FMA, needed to build the C library agreement tables");
+        uint8_t* tabs[4] = {g_tab_log, g_tab_az, g_tab_iso, g_tab_pi};
+        int n = 0;
+        for (int w = 0; w < 4; w++) {
+            const int e = mmc_host_libm_table(w, tabs[w], g_tab_esc + n, MMC_LIBM_MAXESC - n);
+            if (e < 0 || n + e > MMC_LIBM_MAXESC)
+                return set_err(-1, "C library agreement table %d: %d values outside one ulp (limit %d)", w, e,
+                               MMC_LIBM_MAXESC - n);
+            n += e;
+        }
+        g_tab_nesc = n;
+    }
+    int rc = upload_libm_tables(g_tab_log, g_tab_az, g_tab_iso, g_tab_pi, g_tab_esc, g_tab_nesc);
+    if (rc) return rc;
+    g_tab_dev[dev] = true;
+    return 0;
 }
 
 template <typename T>
@@ -547,6 +585,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     if (sl->lo.p) (void)hipFree(sl->lo.p);
     sl->field.p = sl->lo.p = nullptr;
     if (!m || !sims || nsim < 1 || !res) return set_err(-1, "null argument");
+    {
+        int rc = ensure_libm_tables();
+        if (rc) return rc;
+    }
     for (int i = 0; i < nsim; i++) {
         int rc = check_sim(m, sims + i, sims[0].method);
         if (rc) return set_err(rc, "simulation %d: %s", i, std::string(g_err).c_str());

@@ -24,21 +24,31 @@
 #include <omp.h>
 #endif
 
-/* ORC_PORTABLE_MATH replaces glibc's expf/logf/sincosf/acosf with the
- * deterministic FP32 routines of include/mmc_fp32math.h (the library the GPU
- * kernel uses), so that single photons can be compared bit for bit; the
- * default build keeps glibc, as the reference CPU build does. */
+/* Elementary functions.  The reference CPU build calls glibc's expf, logf,
+ * sincosf/sinf/cosf and acosf (no -ffast-math), and double-precision log for
+ * the Gaussian sources.  Scattering lengths, azimuths and isotropic zeniths
+ * are functions of one generator output, and the GPU reproduces glibc's
+ * values on all 2^23 of them, so both builds call glibc there.
+ * ORC_PORTABLE_MATH -- the GPU's arithmetic, for bit-for-bit comparison of
+ * single photons -- differs from the default build (the reference's
+ * arithmetic) only where an argument is not a function of one draw: expf is
+ * the deterministic FP32 routine of include/mmc_fp32math.h, sin/cos of such
+ * angles are correctly rounded through double precision (O_SINCOSF_C), and
+ * the RF phase rotation, which exists only in the reference's GPU code, uses
+ * the FP32 routine (O_SINCOSF_RF). */
 #ifdef ORC_PORTABLE_MATH
 #include "mmc_fp32math.h"
+static inline void orc_sincos_cr(float x, float* s, float* c) {
+    *s = (float)sin((double)x);
+    *c = (float)cos((double)x);
+}
 #define O_EXPF(x) mmc_expf(x)
-#define O_LOGF(x) mmc_logf(x)
-#define O_SINCOSF(x, s, c) mmc_sincosf(x, s, c)
-#define O_ACOSF(x) mmc_acosf(x)
+#define O_SINCOSF_C(x, s, c) orc_sincos_cr(x, s, c)
+#define O_SINCOSF_RF(x, s, c) mmc_sincosf(x, s, c)
 #else
 #define O_EXPF(x) expf(x)
-#define O_LOGF(x) logf(x)
-#define O_SINCOSF(x, s, c) sincosf(x, s, c)
-#define O_ACOSF(x) acosf(x)
+#define O_SINCOSF_C(x, s, c) sincosf(x, s, c)
+#define O_SINCOSF_RF(x, s, c) sincosf(x, s, c)
 #endif
 
 #define EPS        1e-6f              /* src/mmc_rand_common.h:46 */
@@ -87,8 +97,88 @@ void orc_rand_seed(const uint32_t seed[4], uint64_t t[2]) {
 
 /* src/mmc_rand_common.h:50-51 */
 float orc_rand_scatlen(uint64_t t[2]) {
-    return -O_LOGF(orc_rand_next(t) + EPS);
+    return -logf(orc_rand_next(t) + EPS);
 }
+
+#ifdef ORC_PORTABLE_MATH
+/* ------------------------------------------------------------------------ */
+/* Check of the GPU's agreement tables (mmc_host_libm_table): the kernel's    */
+/* rule -- FP32 function of mmc_fp32math.h, then the 2-bit code of the table  */
+/* (0 keep, 1 raw bits + 1, 2 raw bits - 1, 3 escape list) -- applied over   */
+/* every generator output k, counting the values that differ from glibc.     */
+/* Returns the number of differing values.                                   */
+/* ------------------------------------------------------------------------ */
+static float orc_fix1(float y, unsigned c) {
+    uint32_t b;
+    memcpy(&b, &y, 4);
+    b = (c == 1u) ? b + 1u : (c == 2u ? b - 1u : b);
+    memcpy(&y, &b, 4);
+    return y;
+}
+static int orc_fix_escape(int which, uint32_t k, const int32_t* esc, int nesc, float v[3]) {
+    for (int i = 0; i < nesc; i++)
+        if (esc[4 * i] == (int32_t)(((uint32_t)which << 23) | k)) {
+            memcpy(v, esc + 4 * i + 1, 12);
+            return 1;
+        }
+    return 0;
+}
+static int orc_same(float a, float b) { return memcmp(&a, &b, 4) == 0; }
+
+int64_t orc_libm_verify(int which, const uint8_t* tab, const int32_t* esc, int nesc) {
+    int64_t bad = 0;
+    for (uint32_t k = 0; k < (1u << 23); k++) {
+        uint32_t bits = 0x3F800000u | k;
+        float u;
+        memcpy(&u, &bits, 4);
+        u -= 1.0f;
+        float v[3];
+        if (which == 0) {            /* logf(u + EPS) */
+            unsigned c = (tab[k >> 2] >> (2 * (k & 3))) & 3u;
+            float y = mmc_logf(u + EPS);
+            if (c == 3u) {
+                if (!orc_fix_escape(0, k, esc, nesc, v)) { bad++; continue; }
+                y = v[0];
+            } else {
+                y = orc_fix1(y, c);
+            }
+            bad += !orc_same(y, logf(u + EPS));
+        } else if (which == 1 || which == 3) {   /* sincosf((float)(2 pi u)) or (float)(pi u) */
+            unsigned c = (tab[k >> 1] >> (4 * (k & 1))) & 15u;
+            float a = (which == 1) ? (float)(TWO_PI * u) : (float)(M_PI * u);
+            float s, co, gs, gc;
+            mmc_sincosf(a, &s, &co);
+            if (c == 15u) {
+                if (!orc_fix_escape(which, k, esc, nesc, v)) { bad++; continue; }
+                s = v[0];
+                co = v[1];
+            } else {
+                s = orc_fix1(s, c & 3u);
+                co = orc_fix1(co, c >> 2);
+            }
+            sincosf(a, &gs, &gc);
+            bad += !orc_same(s, gs) || !orc_same(co, gc);
+        } else {                     /* t = acosf(2u - 1), sincosf(t) */
+            unsigned c = tab[k];
+            float t = orc_fix1(mmc_acosf(2.f * u - 1.f), c & 3u), s, co, gs, gc;
+            mmc_sincosf(t, &s, &co);
+            if ((c & 3u) == 3u) {
+                if (!orc_fix_escape(2, k, esc, nesc, v)) { bad++; continue; }
+                t = v[0];
+                s = v[1];
+                co = v[2];
+            } else {
+                s = orc_fix1(s, (c >> 2) & 3u);
+                co = orc_fix1(co, (c >> 4) & 3u);
+            }
+            float gt = acosf(2.f * u - 1.f);
+            sincosf(gt, &gs, &gc);
+            bad += !orc_same(t, gt) || !orc_same(s, gs) || !orc_same(co, gc);
+        }
+    }
+    return bad;
+}
+#endif
 
 /* ------------------------------------------------------------------------ */
 /* small vector helpers with the reference's evaluation order               */
@@ -389,6 +479,12 @@ typedef struct {
     int framelen;
     double raytet;
     float rw, rt;              /* replay: the photon's detected weight and time */
+    /* floating-point tie cases of the current photon: ray-tetrahedron tests
+     * whose exit face was decided by a tie rule (Badouel: two or more faces
+     * at the same smallest distance, resolved by maskmap; Plucker: more than
+     * one face passing the sign test, resolved by face order), and fixphoton
+     * nudges (no face found) */
+    int32_t ties, nudges;
 } octx;
 
 #define DELTA_MUA 1e-4f        /* src/mmc_mesh.h:73 */
@@ -464,9 +560,13 @@ static float orc_badouel(oray* r, octx* c) {
     float m13 = (T[1] < T[3]) ? T[1] : T[3];
     baryx = (m02 < m13) ? m02 : m13;
     /* maskmap: highest tied lane (src/mmc_raytrace.c:152,1467) */
-    int faceidx = 4;
+    int faceidx = 4, ntied = 0;
     for (int i = 0; i < 4; i++)
-        if (T[i] == baryx) faceidx = i;
+        if (T[i] == baryx) {
+            faceidx = i;
+            ntied++;
+        }
+    if (ntied > 1 && baryx < 1e10f) c->ties++;
     r->faceid = O_faceorder[faceidx];
 
     if (r->faceid >= 0 && baryx >= 0) {
@@ -527,7 +627,7 @@ static float orc_badouel(oray* r, octx* c) {
                 float attr = totalloss < 1.f ? (1.f - totalloss) : 1.f;
                 float phase = cfg->omega * n * R_C0 * r->Lmove;
                 float sphi, cphi;
-                O_SINCOSF(phase, &sphi, &cphi);
+                O_SINCOSF_RF(phase, &sphi, &cphi);
                 float w0r = currweight, w0i = r->weight_im;
                 float nre = attr * (w0r * cphi + w0i * sphi);
                 float nim = attr * (-w0r * sphi + w0i * cphi);
@@ -561,7 +661,7 @@ static float orc_badouel(oray* r, octx* c) {
                 /* RF: the complex weight turns by the same phase in every
                  * segment (src/mmc_core.cl:1043-1081) */
                 float sr = currweight, si = r->weight_im, scos = 1.f, ssin = 0.f;
-                if (rf) O_SINCOSF(cfg->omega * n * R_C0 * dstep, &ssin, &scos);
+                if (rf) O_SINCOSF_RF(cfg->omega * n * R_C0 * dstep, &ssin, &scos);
                 const int dx = cfg->griddim[0], dxy = cfg->griddim[0] * cfg->griddim[1];
                 for (int i = 0; i < seg; i++) {
                     int ix = (int)(sx * inv), iy = (int)(sy * inv), iz = (int)(sz * inv);
@@ -673,6 +773,16 @@ static float orc_plucker(oray* r, octx* c) {
         w[i] = dpps(r->vec, M) + dpps(pcrx, D);
     }
     r->pout_undef = 1;
+    {
+        /* faces passing the sign test (a tie when more than one does) */
+        int npass = 0;
+        for (int i = 0; i < 4; i++) {
+            float wa = w[O_fc[i][0]], wb = (i >= 2) ? -w[O_fc[i][1]] : w[O_fc[i][1]], wc = w[O_fc[i][2]];
+            uint32_t a = fbits(wa), b = fbits(wb), cc = fbits(wc);
+            npass += ((a & 0x80000000u) & (b & 0x80000000u) & (cc ^ 0x80000000u)) != 0;
+        }
+        if (npass > 1) c->ties++;
+    }
     for (int i = 0; i < 4; i++) {
         if (i >= 2) w[O_fc[i][1]] = -w[O_fc[i][1]];
         uint32_t a = fbits(w[O_fc[i][0]]), b = fbits(w[O_fc[i][1]]), cc = fbits(w[O_fc[i][2]]);
@@ -828,7 +938,7 @@ static float orc_scatter_mom(float g, float dir[3], uint64_t t[2], float* mom) {
     float sphi, cphi, tmp0, stheta, ctheta, tmp1;
     float nextslen = orc_rand_scatlen(t);
     tmp0 = (float)(TWO_PI * orc_rand_next(t));
-    O_SINCOSF(tmp0, &sphi, &cphi);
+    sincosf(tmp0, &sphi, &cphi);
     if (g > EPS) {
         tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * orc_rand_next(t));
         tmp0 *= tmp0;
@@ -838,8 +948,8 @@ static float orc_scatter_mom(float g, float dir[3], uint64_t t[2], float* mom) {
         stheta = (float)sqrt(1.f - tmp0 * tmp0);
         ctheta = tmp0;
     } else {
-        float theta = O_ACOSF(2.f * orc_rand_next(t) - 1.f);
-        O_SINCOSF(theta, &stheta, &ctheta);
+        float theta = acosf(2.f * orc_rand_next(t) - 1.f);
+        sincosf(theta, &stheta, &ctheta);
     }
     float px, py, pz;
     if (dir[2] > -1.f + EPS && dir[2] < 1.f - EPS) {
@@ -985,7 +1095,7 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
                 r->weight = (cfg->srcnum == 1) ? cfg->srcpattern[r->posidx] : 1.f;
             } else if (st == ORC_ST_FOURIER) {
                 float sn, cs;
-                O_SINCOSF((float)((floorf(P1[3]) * rx + floorf(P2[3]) * ry + P1[3] - floorf(P1[3])) * TWO_PI),
+                O_SINCOSF_C((float)((floorf(P1[3]) * rx + floorf(P2[3]) * ry + P1[3] - floorf(P1[3])) * TWO_PI),
                           &sn, &cs);
                 r->weight = (cs * (1.f - P2[3] + floorf(P2[3])) + 1.f) * 0.5f;
             }
@@ -1004,12 +1114,12 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
         r->p0.z = S[2] + rx * P1[2] + ry * v2.z;
         if (st == ORC_ST_FOURIERX2D) {
             float sa, ca, sb, cb;
-            O_SINCOSF((float)((P2[0] * rx + P2[2]) * TWO_PI), &sa, &ca);
-            O_SINCOSF((float)((P2[1] * ry + P2[3]) * TWO_PI), &sb, &cb);
+            O_SINCOSF_C((float)((P2[0] * rx + P2[2]) * TWO_PI), &sa, &ca);
+            O_SINCOSF_C((float)((P2[1] * ry + P2[3]) * TWO_PI), &sb, &cb);
             r->weight = (sa * sb + 1.f) * 0.5f;
         } else {
             float sn, cs;
-            O_SINCOSF((float)((P2[0] * rx + P2[1] * ry + P2[2]) * TWO_PI), &sn, &cs);
+            O_SINCOSF_C((float)((P2[0] * rx + P2[1] * ry + P2[2]) * TWO_PI), &sn, &cs);
             r->weight = (cs * (1.f - P2[3]) + 1.f) * 0.5f;
         }
         origin.x += (P1[0] + v2.x) * 0.5f;
@@ -1017,18 +1127,22 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
         origin.z += (P1[2] + v2.z) * 0.5f;
     } else if (st == ORC_ST_DISK || st == ORC_ST_RING || st == ORC_ST_GAUSSIAN) {
         float sphi, cphi, phi, r0;
-        if (st == ORC_ST_RING && (P1[2] > 0.f || P1[3] > 0.f))
+        if (st == ORC_ST_RING && (P1[2] > 0.f || P1[3] > 0.f)) {
             phi = fabsf(P1[2] - P1[3]) * orc_rand_next(ran) + fminf(P1[2], P1[3]);
-        else
+            O_SINCOSF_C(phi, &sphi, &cphi);
+        } else {
             phi = (float)(TWO_PI * orc_rand_next(ran));
-        O_SINCOSF(phi, &sphi, &cphi);
+            sincosf(phi, &sphi, &cphi);
+        }
+        /* double-precision log, converted to float for sqrtf
+         * (src/mmc_raytrace.c:2444-2449) */
         if (st == ORC_ST_DISK || st == ORC_ST_RING) {
             r0 = sqrtf(orc_rand_next(ran) * fabsf(P1[0] * P1[0] - P1[1] * P1[1]) + P1[1] * P1[1]);
         } else if (fabsf(focus) < 1e-5f || fabsf(P1[1]) < 1e-5f) {
-            r0 = sqrtf(-O_LOGF(orc_rand_next(ran))) * P1[0];
+            r0 = sqrtf(-log(orc_rand_next(ran))) * P1[0];
         } else {
             float z0 = (float)(P1[0] * P1[0] * M_PI / P1[1]);     /* Rayleigh range */
-            r0 = sqrtf(-O_LOGF(orc_rand_next(ran)) * (1.f + (focus * focus / (z0 * z0)))) * P1[0];
+            r0 = sqrtf(-log(orc_rand_next(ran)) * (1.f + (focus * focus / (z0 * z0)))) * P1[0];
         }
         if (D[2] > -1.f + EPS && D[2] < 1.f - EPS) {
             float tmp0 = 1.f - D[2] * D[2];
@@ -1043,36 +1157,46 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
     } else if (st == ORC_ST_CONE || st == ORC_ST_ISOTROPIC || st == ORC_ST_ARCSINE) {
         float ang, stheta, ctheta, sphi, cphi;
         ang = (float)(TWO_PI * orc_rand_next(ran));
-        O_SINCOSF(ang, &sphi, &cphi);
+        sincosf(ang, &sphi, &cphi);
+        int clamped = 0;
         if (st == ORC_ST_CONE) {
             int k = 0;
             do {
-                ang = (P1[1] > 0) ? (float)(TWO_PI * orc_rand_next(ran)) : O_ACOSF(2.f * orc_rand_next(ran) - 1.f);
+                ang = (P1[1] > 0) ? (float)(TWO_PI * orc_rand_next(ran)) : acosf(2.f * orc_rand_next(ran) - 1.f);
             } while (ang > P1[0] && ++k < ORC_CONE_TRIES);
-            if (ang > P1[0]) ang = P1[0];
+            if (ang > P1[0]) {
+                ang = P1[0];
+                clamped = 1;
+            }
         } else if (st == ORC_ST_ISOTROPIC) {
-            ang = O_ACOSF(2.f * orc_rand_next(ran) - 1.f);
+            ang = acosf(2.f * orc_rand_next(ran) - 1.f);
         } else {
             ang = (float)(M_PI * orc_rand_next(ran));
         }
-        O_SINCOSF(ang, &stheta, &ctheta);
+        if (clamped)
+            O_SINCOSF_C(ang, &stheta, &ctheta);
+        else
+            sincosf(ang, &stheta, &ctheta);
         orc_rotate(&r->vec, stheta, ctheta, sphi, cphi);
         canfocus = 0;
         if (r->eid > 0) return 0;
     } else if (st == ORC_ST_ZGAUSSIAN) {
         float ang, stheta, ctheta, sphi, cphi;
         ang = (float)(TWO_PI * orc_rand_next(ran));
-        O_SINCOSF(ang, &sphi, &cphi);
+        sincosf(ang, &sphi, &cphi);
+        /* the reference's second draw comes from ran0, a thread-private state
+         * that mmc_run_mp never seeds (src/mmc_host.c:227,265,280-283,
+         * src/mmc_rand_xorshift128p.c:82-84); the photon's own stream here */
         float u1 = orc_rand_next(ran);
         float u2 = orc_rand_next(ran);
-        ang = sqrtf(-2.f * O_LOGF(u1)) * (1.f - 2.f * u2) * P1[0];
-        O_SINCOSF(ang, &stheta, &ctheta);
+        ang = sqrtf(-2.f * log(u1)) * (1.f - 2.f * u2) * P1[0];
+        O_SINCOSF_C(ang, &stheta, &ctheta);
         orc_rotate(&r->vec, stheta, ctheta, sphi, cphi);
         canfocus = 0;
     } else if (st == ORC_ST_HYPERBOLOID) {
         float sphi, cphi;
         float r0 = (float)(TWO_PI * orc_rand_next(ran));
-        O_SINCOSF(r0, &sphi, &cphi);
+        sincosf(r0, &sphi, &cphi);
         r0 = sqrtf(0.5f * orc_rand_scatlen(ran)) * P1[0];
         float tilt = (P1[2] > 0.f) ? (-P1[1] / P1[2]) : 0.f;
         float bx = -r0 * sphi, by = r0 * cphi, bz = P1[2];
@@ -1115,14 +1239,11 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
             float ss = 1.f - 2.f * orc_rand_next(ran);
             float q = 1.f - r->vec.x * r->vec.x - r->vec.y * r->vec.y;
             float pp = sqrtf(q > 0.f ? q : 0.f) * (orc_rand_next(ran) > 0.5f ? 1.f : -1.f);
+            /* the cross product as it is, not normalized (src/mmc_raytrace.c:2552-2562) */
             v3 vv = {r->vec.y * pp - r->vec.z * ss, r->vec.z * tt - r->vec.x * pp, r->vec.x * ss - r->vec.y * tt};
-            float n2 = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z;
-            if (n2 > 0.f) {
-                float n = 1.f / sqrtf(n2);
-                r->vec.x = vv.x * n;
-                r->vec.y = vv.y * n;
-                r->vec.z = vv.z * n;
-            }
+            r->vec.x = vv.x;
+            r->vec.y = vv.y;
+            r->vec.z = vv.z;
         }
         origin.x += P1[0] * 0.5f;
         origin.y += P1[1] * 0.5f;
@@ -1237,6 +1358,7 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
         if (r.pout_undef) {
             if (fixcount++ < MAX_TRIAL) {
                 orc_fixphoton(&r, m);
+                c->nudges++;
                 continue;
             }
             r.eid = ORC_ID_UNDEFINED;
@@ -1280,6 +1402,7 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
             fixcount = 0;
             while (r.pout_undef && fixcount++ < MAX_TRIAL) {
                 orc_fixphoton(&r, m);
+                c->nudges++;
                 r.slen = orc_trace(&r, c);
                 PPATH_ADD(r.Lmove > 0.f && m->type[r.eid - 1] > 0);
             }
@@ -1445,7 +1568,13 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
             double lw = 0.0, aw = 0.0;
             c.rw = cfg->replayweight ? cfg->replayweight[id] : 1.f;
             c.rt = cfg->replaytime ? cfg->replaytime[id] : 0.f;
+            c.ties = 0;
+            c.nudges = 0;
             if (orc_onephoton(id, &c, ran, &det, &lw, &aw, rec) < 0) err = 1;
+            if (rec && out->ties) {
+                out->ties[2 * (id - cfg->recfirst)] = c.ties;
+                out->ties[2 * (id - cfg->recfirst) + 1] = c.nudges;
+            }
             lsum += lw;
             asum += aw;
         }

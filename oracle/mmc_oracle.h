@@ -133,6 +133,8 @@ typedef struct {
     double runtime_ms;
     int error;
     double* weight_im;       /* RF: imaginary part of the field, or NULL */
+    int32_t* ties;           /* reccount x 2 per recorded photon: tie-decided ray-tet tests, fixphoton
+                                nudges; or NULL */
 } orc_out;
 
 /* RNG (src/mmc_rand_xorshift128p.c:55-83, src/mmc_rand_common.h:50-51) */

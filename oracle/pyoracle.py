@@ -51,7 +51,7 @@ class OOut(C.Structure):
     _fields_ = [("weight", f64p), ("launchweight", C.c_double), ("absorbweight", C.c_double),
                 ("raytet", C.c_double), ("detdata", f32p), ("detphoton", i64p), ("maxdet", C.c_int),
                 ("detected", C.c_int), ("reclen", C.c_int), ("rec", C.c_void_p), ("runtime_ms", C.c_double),
-                ("error", C.c_int), ("weight_im", f64p)]
+                ("error", C.c_int), ("weight_im", f64p), ("ties", i32p)]
 
 
 PHOTON_DTYPE = np.dtype([
@@ -274,9 +274,14 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     out.detphoton = p(detph, C.c_int64) if maxdet else C.cast(None, i64p)
     out.maxdet = int(maxdet)
     recs = None
+    ties = None
     if records:
         recs = np.zeros(int(records[1]), PHOTON_DTYPE)
         out.rec = recs.ctypes.data_as(C.c_void_p)
+        ties = np.zeros((int(records[1]), 2), np.int32)
+        out.ties = p(ties, C.c_int32)
+    else:
+        out.ties = C.cast(None, i32p)
     rc = L.orc_run(C.byref(cfg), C.byref(mesh.c), C.byref(out))
     if rc != 0:
         raise RuntimeError("oracle run failed (a launch found no enclosing element)")
@@ -285,6 +290,8 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     n = min(out.detected, maxdet)
     r.detected, r.detdata, r.detphoton = out.detected, det[:n], detph[:n]
     r.photons, r.runtime_ms, r.reclen = recs, out.runtime_ms, out.reclen
+    # per recorded photon: ray-tet tests decided by a tie rule, fixphoton nudges
+    r.ties = ties
     r.field_im = field_im
     r.normalizer = None
     if normalize:
