@@ -179,4 +179,134 @@ MMC_MATHFN float mmc_twopi(float u) { return (float)(6.283185307179586 * (double
  * (src/mmc_raytrace.c:2478). */
 MMC_MATHFN float mmc_pi(float u) { return (float)(3.141592653589793 * (double)u); }
 
+/* ---- double-precision evaluations of the generator-driven draws ----------
+ * The host C library computes logf, sinf/cosf and acosf of a float argument
+ * by evaluating in double and rounding to float; its results are within one
+ * ulp of the exact value but not always correctly rounded.  The functions
+ * below evaluate the same quantities in double to about 2^-50 relative error
+ * (Taylor polynomials, Horner with explicit fused multiply-adds; IEEE
+ * add/multiply/divide only), so their rounding to float is the C library's
+ * result except near a rounding midpoint, where a table built from the C
+ * library decides (mmc_host_libm_table).  Host and GPU compute bit-identical
+ * doubles here, which is what lets a host-built table steer the GPU.
+ *
+ * The coefficients are read from a table K (MMC_DCOEF_LIST) rather than
+ * written as literals: on the GPU, literal double constants are loop
+ * invariants the compiler keeps in registers for the whole photon loop, while
+ * table entries are fetched by scalar loads where they are used. */
+#define MMC_DFMA(a, b, c) __builtin_fma((a), (b), (c))
+
+/* K[0..10]   1/23, 1/21, ..., 1/3          (2 atanh series of log)
+ * K[11..18]  1/17!, -1/15!, ..., -1/3!     (sin)
+ * K[19..26]  -1/18!, 1/16!, ..., 1/4!      (cos)
+ * K[27..48]  asin series, n = 22 .. 1: (2n)!/(4^n (n!)^2 (2n+1))
+ * K[49..50]  ln 2 = K[49] + K[50]  (K[49]: 42 significant bits)
+ * K[51..52]  pi/2 = K[51] + K[52];  K[53] = 2/pi
+ * K[54..55]  pi = K[54] + K[55] */
+#define MMC_NDCOEF 56
+#define MMC_DCOEF_LIST                                                                      \
+    0x1.642c8590b2164p-5, 0x1.8618618618618p-5, 0x1.af286bca1af28p-5, 0x1.e1e1e1e1e1e1ep-5, \
+    0x1.1111111111111p-4, 0x1.3b13b13b13b14p-4, 0x1.745d1745d1746p-4, 0x1.c71c71c71c71cp-4, \
+    0x1.2492492492492p-3, 0x1.999999999999ap-3, 0x1.5555555555555p-2,                       \
+    0x1.952c77030ad4ap-49, -0x1.ae7f3e733b81fp-41, 0x1.6124613a86d09p-33,                  \
+    -0x1.ae64567f544e4p-26, 0x1.71de3a556c734p-19, -0x1.a01a01a01a01ap-13,                 \
+    0x1.1111111111111p-7, -0x1.5555555555555p-3,                                           \
+    -0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45, -0x1.93974a8c07c9dp-37,                 \
+    0x1.1eed8eff8d898p-29, -0x1.27e4fb7789f5cp-22, 0x1.a01a01a01a01ap-16,                  \
+    -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5,                                          \
+    0x1.5c5f56efaaaabp-9, 0x1.750de64d7d05fp-9, 0x1.90cb77f60c7cep-9, 0x1.b026f57b13b14p-9, \
+    0x1.d3d2a8e0dd67dp-9, 0x1.fcaf8fb6db6dbp-9, 0x1.15ee9d45d1746p-8, 0x1.31683bdef7bdfp-8, \
+    0x1.51ba308d3dcb1p-8, 0x1.782dda12f684cp-8, 0x1.a6863d70a3d71p-8, 0x1.df3bd37a6f4dfp-8, \
+    0x1.12ef3cf3cf3cfp-7, 0x1.3fde50d79435ep-7, 0x1.7a87878787878p-7, 0x1.c99999999999ap-7, \
+    0x1.1c4ec4ec4ec4fp-6, 0x1.6e8ba2e8ba2e9p-6, 0x1.f1c71c71c71c7p-6, 0x1.6db6db6db6db7p-5, \
+    0x1.3333333333333p-4, 0x1.5555555555555p-3,                                            \
+    0x1.62e42fefa3800p-1, 0x1.ef35793c76730p-45,                                           \
+    0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54, 0x1.45f306dc9c883p-1,                     \
+    0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53
+
+/* log(x) for a normal float x > 0: x = 2^k m, m in [sqrt(1/2), sqrt(2)),
+ * s = (m - 1)/(m + 1), log m = 2 atanh(s) = 2 sum_n s^(2n+1)/(2n+1) to n = 11
+ * (|s| <= 0.1716, so the first omitted term is below 2^-56 of the sum) */
+MMC_MATHFN double mmc_log_dk(float x, const double* K) {
+    const uint32_t ix = mmc_fbits(x);
+    int k = (int)(ix >> 23) - 127;
+    uint32_t mb = (ix & 0x007FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) { /* m > sqrt(2): halve it */
+        mb -= 0x00800000u;
+        k += 1;
+    }
+    const double f = (double)mmc_bitsf(mb) - 1.0;   /* exact */
+    const double s = f / (2.0 + f);                  /* 2 + f exact */
+    const double z = s * s;
+    double p = K[0];
+    for (int i = 1; i <= 10; i++) p = MMC_DFMA(z, p, K[i]);
+    const double s2 = s + s;
+    const double logm = MMC_DFMA(s2 * z, p, s2);
+    const double kd = (double)k;
+    return MMC_DFMA(kd, K[49], MMC_DFMA(kd, K[50], logm));
+}
+
+/* sin and cos of a float |a| < 2^20: a = j pi/2 + r, |r| <= pi/4 (pi/2 in
+ * two parts; each fused multiply-add rounds once), Taylor series to r^17 and
+ * r^18 */
+MMC_MATHFN void mmc_sincos_dk(float a, double* sp, double* cp, const double* K) {
+    const double x = (double)a;
+    const double big = 0x1.8p52;
+    const double j = (x * K[53] + big) - big;        /* nearest integer */
+    const int q = (int)j;
+    double r = MMC_DFMA(-j, K[51], x);
+    r = MMC_DFMA(-j, K[52], r);
+    const double z = r * r;
+    double ps = K[11];
+    for (int i = 12; i <= 18; i++) ps = MMC_DFMA(z, ps, K[i]);
+    const double s = MMC_DFMA(r * z, ps, r);
+    double pc = K[19];
+    for (int i = 20; i <= 26; i++) pc = MMC_DFMA(z, pc, K[i]);
+    pc = MMC_DFMA(z, pc, -0.5);
+    const double c = MMC_DFMA(z, pc, 1.0);
+    const double s2 = (q & 1) ? c : s, c2 = (q & 1) ? s : c;
+    *sp = (q & 2) ? -s2 : s2;
+    *cp = ((q + 1) & 2) ? -c2 : c2;
+}
+
+/* asin(y) for |y| <= 1/2: sum_n (2n)!/(4^n (n!)^2 (2n+1)) y^(2n+1) to n = 22 */
+MMC_MATHFN double mmc_asin_half_dk(double y, const double* K) {
+    const double z = y * y;
+    double p = K[27];
+    for (int i = 28; i <= 48; i++) p = MMC_DFMA(z, p, K[i]);
+    return MMC_DFMA(y * z, p, y);
+}
+
+/* acos(x) for a float -1 <= x <= 1: pi/2 - asin(x) for |x| <= 1/2, else
+ * 2 asin(sqrt((1 - |x|)/2)) (from pi for x < 0).  (1 - |x|)/2 is a float, its
+ * square root is the correctly rounded float root refined by one Newton step
+ * in double. */
+MMC_MATHFN double mmc_acos_dk(float xf, const double* K) {
+    const double x = (double)xf;
+    if (xf <= 0.5f && xf >= -0.5f) return K[51] - (mmc_asin_half_dk(x, K) - K[52]);
+    const float zf = (1.f - (xf > 0.f ? xf : -xf)) * 0.5f;      /* exact */
+    const double z = (double)zf;
+    const double s0 = (double)MMC_SQRTF(zf);
+    const double t = (zf > 0.f) ? s0 + MMC_DFMA(-s0, s0, z) / (s0 + s0) : 0.0;
+    const double a = 2.0 * mmc_asin_half_dk(t, K);
+    return (xf > 0.f) ? a : K[54] - (a - K[55]);
+}
+
+#if !defined(__HIPCC__)
+/* host C: the same evaluations with the coefficient table in static memory */
+static const double mmc_dcoef_host[MMC_NDCOEF] = {MMC_DCOEF_LIST};
+static inline double mmc_log_d(float x) { return mmc_log_dk(x, mmc_dcoef_host); }
+static inline void mmc_sincos_d(float a, double* s, double* c) { mmc_sincos_dk(a, s, c, mmc_dcoef_host); }
+static inline double mmc_acos_d(float x) { return mmc_acos_dk(x, mmc_dcoef_host); }
+#endif
+
+/* the float next to f, above it (up) or below it */
+MMC_MATHFN float mmc_fstep(float f, int up) {
+    const uint32_t b = mmc_fbits(f);
+    if (f == 0.f) return mmc_bitsf(up ? 1u : 0x80000001u);
+    return mmc_bitsf(((f > 0.f) == (up != 0)) ? b + 1u : b - 1u);
+}
+/* the float next to f in the direction of the double e (f = (float)e) */
+MMC_MATHFN float mmc_fnext(float f, double e) { return mmc_fstep(f, e > (double)f); }
+
 #endif

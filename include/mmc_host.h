@@ -146,35 +146,40 @@ int32_t mmc_host_format_id(const char* name);
  * The reference CPU tracer takes every scattering length, scattering azimuth
  * and isotropic zenith from glibc's logf / sincosf / acosf
  * (src/mmc_rand_common.h:50-51, src/mmc_mesh.c:1662-1690,
- * src/mmc_raytrace.c:2434-2480), which are not correctly rounded.  Each such
- * call sees a function of one generator output u = k / 2^23 (k: the 23
- * mantissa bits of xorshift128+, src/mmc_rand_xorshift128p.c:55-70), so a
- * table over k records, per value, how the C library's result differs from
- * the deterministic FP32 function of mmc_fp32math.h: code 0 equal, 1 raw
- * bits + 1, 2 raw bits - 1, 3 exact values in the escape list.
- *   MMC_FIX_LOG  logf(u + 1e-6f)                   2 bits per k
- *   MMC_FIX_AZ   sincosf((float)(2 pi u))          4 bits per k (sin, cos)
- *   MMC_FIX_ISO  t = acosf(2u - 1); sincosf(t)     1 byte per k (acos, sin, cos)
- *   MMC_FIX_PI   sincosf((float)(pi u))            4 bits per k (sin, cos)
- * mmc_host_libm_table fills table `which` (mmc_host_libm_table_bytes bytes)
- * from the C library this process runs with and writes up to maxesc escape
- * entries (key = which << 23 | k, v = exact values' bits); returns the number
- * of escapes, or -1.  Requires a host CPU with FMA. */
+ * src/mmc_raytrace.c:2434-2480), which are within one ulp but not correctly
+ * rounded.  Each such call sees a function of one generator output
+ * u = k / 2^23 (k: the 23 mantissa bits of xorshift128+,
+ * src/mmc_rand_xorshift128p.c:55-70).  The GPU evaluates each value in double
+ * (mmc_log_d, mmc_sincos_d, mmc_acos_d of mmc_fp32math.h) and rounds it to
+ * float f; a table over k holds one bit per value, set when the C library
+ * returns instead the float o next to f on the far side of the double, and
+ * per region of 2^MMC_LIBM_REGSHIFT consecutive k and per value a threshold D
+ * (float, -1 for none): every set bit of the region has its double within
+ * D |o - f| of the midpoint (f + o)/2, so the GPU reads bits only there.
+ *   which         values of generator output k            bits per k
+ *   MMC_FIX_LOG   logf(u + 1e-6f)                          1
+ *   MMC_FIX_AZ    sincosf((float)(2 pi u))                 2 (sin, cos)
+ *   MMC_FIX_ISO   t = acosf(2u - 1); sincosf(t)            4 (acos, sin, cos, unused)
+ *   MMC_FIX_PI    sincosf((float)(pi u))                   2 (sin, cos)
+ * Bit b of value v at k: byte (k * bits) / 8, bit (k * bits) % 8 + v.
+ * mmc_host_libm_table fills the bits (mmc_host_libm_table_bytes bytes) and
+ * the thresholds (MMC_LIBM_NREG * mmc_host_libm_nval floats, [region][value])
+ * from the C library this process runs with; returns the number of values the
+ * rule cannot reproduce (the C library's result is neither f nor o; 0 on
+ * glibc), or -1 on bad arguments.  Needs a host CPU with FMA. */
 #define MMC_FIX_LOG 0
 #define MMC_FIX_AZ  1
 #define MMC_FIX_ISO 2
 #define MMC_FIX_PI  3
-#define MMC_FIXB_LOG (1 << 21)
-#define MMC_FIXB_AZ  (1 << 22)
-#define MMC_FIXB_ISO (1 << 23)
-#define MMC_FIXB_PI  (1 << 22)
-#define MMC_LIBM_MAXESC 64
-typedef struct mmc_libm_escape {
-    int32_t key;
-    uint32_t v[3];
-} mmc_libm_escape;
+#define MMC_FIXB_LOG (1 << 20)
+#define MMC_FIXB_AZ  (1 << 21)
+#define MMC_FIXB_ISO (1 << 22)
+#define MMC_FIXB_PI  (1 << 21)
+#define MMC_LIBM_REGSHIFT 16
+#define MMC_LIBM_NREG (1 << (23 - MMC_LIBM_REGSHIFT))
+int32_t mmc_host_libm_nval(int32_t which);
 int64_t mmc_host_libm_table_bytes(int32_t which);
-int32_t mmc_host_libm_table(int32_t which, uint8_t* out, mmc_libm_escape* esc, int32_t maxesc);
+int32_t mmc_host_libm_table(int32_t which, uint8_t* bits, float* thr);
 
 #ifdef __cplusplus
 }

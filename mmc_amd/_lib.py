@@ -146,7 +146,8 @@ SIGNATURES = {
                                        C.c_float, C.c_int32, C.c_int32]),
     "mmc_host_format_id": (C.c_int32, [C.c_char_p]),
     "mmc_host_libm_table_bytes": (C.c_int64, [C.c_int32]),
-    "mmc_host_libm_table": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
+    "mmc_host_libm_table": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p]),
+    "mmc_host_libm_nval": (C.c_int32, [C.c_int32]),
 }
 
 

@@ -168,8 +168,8 @@ __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, doubl
 __global__ void fill_zero_u64(unsigned long long* p, int n);
 /* host C library agreement tables (mmc_host_libm_table) into the current
  * device's copy; 0 or a negative error */
-int upload_libm_tables(const uint8_t* tlog, const uint8_t* taz, const uint8_t* tiso, const uint8_t* tpi,
-                       const mmc_libm_escape* esc, int32_t nesc);
+int upload_libm_tables(const uint8_t* blog, const uint8_t* baz, const uint8_t* biso, const uint8_t* bpi,
+                       const float* tlog, const float* taz, const float* tiso, const float* tpi);
 
 }  // namespace mmc
 

@@ -82,99 +82,120 @@ __device__ __forceinline__ uint32_t draw_bits(float u) { return __float_as_uint(
 /* ---- host C library agreement (built by mmc_host_libm_table) ------------
  * The reference draws scattering lengths, azimuths and isotropic zeniths
  * through the host C library's logf/sincosf/acosf.  Each of those calls sees
- * a function of one generator output k, so for every k a 2-bit code says how
- * the C library's value differs from the deterministic FP32 evaluation of
- * mmc_fp32math.h: 0 equal, 1 raw bits + 1, 2 raw bits - 1, 3 exact values in
- * the escape list.  One copy per device, filled once per process. */
-__device__ uint8_t g_fix_log[MMC_FIXB_LOG];      /* logf(u + 1e-6f) */
-__device__ uint8_t g_fix_az[MMC_FIXB_AZ];        /* sincosf((float)(2 pi u)) */
-__device__ uint8_t g_fix_iso[MMC_FIXB_ISO];      /* acosf(2u - 1), then sincosf of it */
-__device__ uint8_t g_fix_pi[MMC_FIXB_PI];        /* sincosf((float)(pi u)) */
-__device__ mmc_libm_escape g_fix_esc[MMC_LIBM_MAXESC];
-__device__ int32_t g_fix_nesc;
+ * a function of one generator output k.  The kernel evaluates the value in
+ * double and rounds it to float f; the C library returns either f or the
+ * float o next to f on the far side of the double, and a host-built bit per
+ * k and value says which.  The bit can only be set where the double lies
+ * near the rounding midpoint of f and o: within D |o - f| of it, D being the
+ * threshold of the region of k (2^MMC_LIBM_REGSHIFT consecutive outputs), so
+ * the bits are read for a few percent of the draws only.  One copy per
+ * device, filled once per process. */
+__device__ uint8_t g_lm_log[MMC_FIXB_LOG];      /* logf(u + 1e-6f): 1 bit per k */
+__device__ uint8_t g_lm_az[MMC_FIXB_AZ];        /* sincosf((float)(2 pi u)): sin, cos */
+__device__ uint8_t g_lm_iso[MMC_FIXB_ISO];      /* acosf(2u - 1), then sincosf of it: 4 bits */
+__device__ uint8_t g_lm_pi[MMC_FIXB_PI];        /* sincosf((float)(pi u)): sin, cos */
+constexpr int kThrLog = 0, kThrAz = MMC_LIBM_NREG, kThrIso = 3 * MMC_LIBM_NREG, kThrPi = 6 * MMC_LIBM_NREG;
+__device__ float g_lm_thr[8 * MMC_LIBM_NREG];    /* [table][region][value] thresholds */
+/* coefficients of the double evaluations (mmc_fp32math.h), in the constant
+ * address space so that wave-uniform reads of them are scalar loads */
+__constant__ double g_dcoef[MMC_NDCOEF] = {MMC_DCOEF_LIST};
+typedef const __attribute__((address_space(4))) double* kconst_ptr;
 
-__device__ __forceinline__ float fix1(float y, uint32_t c) {
-    const uint32_t b = __float_as_uint(y);
-    return c == 1u ? __uint_as_float(b + 1u) : (c == 2u ? __uint_as_float(b - 1u) : y);
-}
-/* exact values of table `which` at k from the escape list */
-__device__ __noinline__ void fix_escape(uint32_t which, uint32_t k, float* v0, float* v1, float* v2) {
-    const int32_t key = (int32_t)((which << 23) | k);
-    const int n = min(g_fix_nesc, MMC_LIBM_MAXESC);
-    for (int i = 0; i < n; i++)
-        if (g_fix_esc[i].key == key) {
-            *v0 = __uint_as_float(g_fix_esc[i].v[0]);
-            *v1 = __uint_as_float(g_fix_esc[i].v[1]);
-            *v2 = __uint_as_float(g_fix_esc[i].v[2]);
-            return;
-        }
+/* the coefficient table, through an opaque copy of its address: its entries
+ * are loaded (scalar loads) where a polynomial needs them instead of being
+ * held in registers across the photon loop */
+__device__ __forceinline__ const double* dk() {
+    kconst_ptr p = (kconst_ptr)g_dcoef;
+    asm volatile("" : "+s"(p));
+    return (const double*)p;
 }
 
-/* rand_next_scatlen (src/mmc_rand_common.h:50-51): -logf(u + EPS) */
-__device__ __forceinline__ float rng_scatlen(uint64_t& a, uint64_t& b) {
-    const float u = rng_next(a, b);
+/* true when the C library may round e to the far neighbour of f = (float)e:
+ * e within D |o - f| of the midpoint (every operation here is exact) */
+__device__ __forceinline__ bool lm_near(double e, float f, float D) {
+    const double fd = (double)f;
+    if (e == fd) return false;
+    const double od = (double)mmc_fnext(f, e);
+    return fabs(e - 0.5 * (fd + od)) <= (double)D * fabs(od - fd);
+}
+__device__ __forceinline__ float lm_pick(float f, double e, uint32_t bit) { return bit ? mmc_fnext(f, e) : f; }
+
+/* The thresholds of the two draws of every scattering event (log, then
+ * azimuth sin and cos) are staged in LDS by each workgroup: thr_lds =
+ * [log: MMC_LIBM_NREG][azimuth: MMC_LIBM_NREG x 2]. */
+constexpr int kThrLds = 3 * MMC_LIBM_NREG;
+
+/* -logf(u + EPS) of a draw u as the host C library rounds it
+ * (rand_next_scatlen, src/mmc_rand_common.h:50-51); tab: the table byte of
+ * the draw's generator output (g_lm_log[k >> 3]), fetched by the caller */
+__device__ __forceinline__ float scatlen_of(float u, uint32_t tab, const float* thr_lds) {
     const uint32_t k = draw_bits(u);
-    const uint32_t c = (g_fix_log[k >> 2] >> (2 * (k & 3))) & 3u;
-    float l = mmc_logf(u + EPSF);
-    if (c == 3u) {
-        float d1, d2;
-        fix_escape(MMC_FIX_LOG, k, &l, &d1, &d2);
-    } else {
-        l = fix1(l, c);
-    }
+    const double e = mmc_log_dk(u + EPSF, dk());
+    float l = (float)e;
+    if (lm_near(e, l, thr_lds[k >> MMC_LIBM_REGSHIFT])) l = lm_pick(l, e, (tab >> (k & 7)) & 1u);
     return -l;
 }
-/* sin and cos of (float)(2 pi u) or (float)(pi u) (which = MMC_FIX_AZ or
- * MMC_FIX_PI), as the host C library rounds them */
-__device__ __forceinline__ void sincos_draw(uint32_t which, float u, float* s, float* c) {
-    const uint32_t k = draw_bits(u);
-    const uint8_t* tab = which == MMC_FIX_AZ ? g_fix_az : g_fix_pi;
-    const uint32_t code = (tab[k >> 1] >> (4 * (k & 1))) & 15u;
-    mmc_sincosf(which == MMC_FIX_AZ ? mmc_twopi(u) : mmc_pi(u), s, c);
-    if (code == 15u) {
-        float d;
-        fix_escape(which, k, s, c, &d);
-    } else {
-        *s = fix1(*s, code & 3u);
-        *c = fix1(*c, code >> 2);
-    }
+__device__ __forceinline__ float rng_scatlen(uint64_t& a, uint64_t& b, const float* thr_lds) {
+    const float u = rng_next(a, b);
+    return scatlen_of(u, g_lm_log[draw_bits(u) >> 3], thr_lds);
 }
-/* theta = acosf(2u - 1) and its sin and cos (isotropic zenith), as the host
- * C library rounds them */
-__device__ __forceinline__ float acos_draw(float u, float* s, float* c) {
+/* acosf(2u - 1) of a draw u, as the host C library rounds it */
+__device__ __forceinline__ float acos_corrected(float u) {
     const uint32_t k = draw_bits(u);
-    const uint32_t code = g_fix_iso[k];
-    float t = fix1(mmc_acosf(2.f * u - 1.f), code & 3u);
-    mmc_sincosf(t, s, c);
-    if ((code & 3u) == 3u) {
-        fix_escape(MMC_FIX_ISO, k, &t, s, c);
-    } else {
-        *s = fix1(*s, (code >> 2) & 3u);
-        *c = fix1(*c, (code >> 4) & 3u);
-    }
+    const double et = mmc_acos_dk(2.f * u - 1.f, dk());
+    float t = (float)et;
+    if (lm_near(et, t, g_lm_thr[kThrIso + 3 * (k >> MMC_LIBM_REGSHIFT)]))
+        t = lm_pick(t, et, (g_lm_iso[k >> 1] >> (4 * (k & 1))) & 1u);
     return t;
 }
-/* sin and cos of an angle that is not a function of one draw: correctly
- * rounded through double precision (the host C library's sinf/cosf are
- * within 0.56 ulp, so they agree wherever they round correctly) */
-__device__ __forceinline__ void sincos_cr(float x, float* s, float* c) {
-    double sd, cd;
-    sincos((double)x, &sd, &cd);
-    *s = (float)sd;
-    *c = (float)cd;
+
+/* kinds of angle evaluated by angle_draw */
+enum { kAngNone = 0, kAngAz = 1, kAngPi = 2, kAngIso = 3, kAngCr = 4 };
+
+/* sin and cos of one angle: kAngAz (float)(2 pi u), kAngPi (float)(pi u) or
+ * kAngIso acosf(2u - 1) of a draw u, each as the host C library rounds it
+ * (the isotropic zenith's sine and cosine are those of the C library's
+ * angle), or kAngCr an angle given directly (|arg| < 2^20; the double
+ * evaluation rounded, i.e. correctly rounded except within about 2^-29 ulp of
+ * a midpoint, where the C library's sinf/cosf, within 0.56 ulp, agree
+ * wherever they round correctly).  Callers keep one call site per path, so
+ * the double evaluation is emitted once there. */
+__device__ __forceinline__ void angle_draw(int kind, float arg, float* s, float* c, const float* thr_lds) {
+    const uint32_t k = draw_bits(arg);
+    const float t = kind == kAngAz    ? mmc_twopi(arg)
+                    : kind == kAngPi  ? mmc_pi(arg)
+                    : kind == kAngIso ? acos_corrected(arg)
+                                      : arg;
+    double es, ec;
+    mmc_sincos_dk(t, &es, &ec, dk());
+    float fs = (float)es, fc = (float)ec;
+    if (kind != kAngCr) {
+        const uint32_t reg = k >> MMC_LIBM_REGSHIFT;
+        const float* thr = kind == kAngAz ? thr_lds + MMC_LIBM_NREG + 2 * reg
+                                          : g_lm_thr + (kind == kAngPi ? kThrPi + 2 * reg : kThrIso + 3 * reg + 1);
+        if (lm_near(es, fs, thr[0]) || lm_near(ec, fc, thr[1])) {
+            const uint32_t code = kind == kAngIso ? (g_lm_iso[k >> 1] >> (4 * (k & 1) + 1))
+                                                  : ((kind == kAngAz ? g_lm_az : g_lm_pi)[k >> 2] >> (2 * (k & 3)));
+            fs = lm_pick(fs, es, code & 1u);
+            fc = lm_pick(fc, ec, code & 2u);
+        }
+    }
+    *s = fs;
+    *c = fc;
 }
 
 /* Copies the host-built agreement tables into this device's copy. */
-int upload_libm_tables(const uint8_t* tlog, const uint8_t* taz, const uint8_t* tiso, const uint8_t* tpi,
-                       const mmc_libm_escape* esc, int32_t nesc) {
-    if (nesc < 0 || nesc > MMC_LIBM_MAXESC) return report_error(-1, "libm agreement: too many escapes");
-    if (hipMemcpyToSymbol(HIP_SYMBOL(g_fix_log), tlog, MMC_FIXB_LOG) != hipSuccess ||
-        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_az), taz, MMC_FIXB_AZ) != hipSuccess ||
-        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_iso), tiso, MMC_FIXB_ISO) != hipSuccess ||
-        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_pi), tpi, MMC_FIXB_PI) != hipSuccess ||
-        (nesc > 0 && hipMemcpyToSymbol(HIP_SYMBOL(g_fix_esc), esc, sizeof(mmc_libm_escape) * (size_t)nesc) !=
-                         hipSuccess) ||
-        hipMemcpyToSymbol(HIP_SYMBOL(g_fix_nesc), &nesc, sizeof(int32_t)) != hipSuccess)
+int upload_libm_tables(const uint8_t* blog, const uint8_t* baz, const uint8_t* biso, const uint8_t* bpi,
+                       const float* tlog, const float* taz, const float* tiso, const float* tpi) {
+    const size_t nr = MMC_LIBM_NREG * sizeof(float);
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_lm_log), blog, MMC_FIXB_LOG) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_az), baz, MMC_FIXB_AZ) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_iso), biso, MMC_FIXB_ISO) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_pi), bpi, MMC_FIXB_PI) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_thr), tlog, nr, kThrLog * sizeof(float)) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_thr), taz, 2 * nr, kThrAz * sizeof(float)) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_thr), tiso, 3 * nr, kThrIso * sizeof(float)) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(g_lm_thr), tpi, 2 * nr, kThrPi * sizeof(float)) != hipSuccess)
         return report_error(-1, "libm agreement: table upload failed");
     return 0;
 }
@@ -211,16 +232,27 @@ constexpr int kConeTries = 1 << 16;        /* bound of the cone source's rejecti
  * the given element and barycentric weights.  posidx: the pattern pixel. */
 __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pattern, uint64_t& r0, uint64_t& r1,
                                               float& px, float& py, float& pz, float& vx, float& vy, float& vz,
-                                              float& w, int& posidx) {
+                                              float& w, int& posidx, const float* thr_lds) {
     const int st = C.srctype;
     const float4 P1 = C.srcparam1, P2 = C.srcparam2, S = C.srcpos, D = C.srcdir;
     float ox = S.x, oy = S.y, oz = S.z;      /* focus origin */
     bool canfocus = true;
     if (st == MMC_SRC_PENCIL) return false;
-    if (st == MMC_SRC_PLANAR || st == MMC_SRC_PATTERN || st == MMC_SRC_PATTERN3D || st == MMC_SRC_FOURIER ||
-        st == MMC_SRC_PENCILARRAY) {
-        const float rx = rng_next(r0, r1);
-        const float ry = rng_next(r0, r1);
+    const bool planar = st == MMC_SRC_PLANAR || st == MMC_SRC_PATTERN || st == MMC_SRC_PATTERN3D ||
+                        st == MMC_SRC_FOURIER || st == MMC_SRC_PENCILARRAY;
+    const bool fourierx = st == MMC_SRC_FOURIERX || st == MMC_SRC_FOURIERX2D;
+    const bool disk = st == MMC_SRC_DISK || st == MMC_SRC_RING || st == MMC_SRC_GAUSSIAN;
+    const bool dirsrc = st == MMC_SRC_CONE || st == MMC_SRC_ISOTROPIC || st == MMC_SRC_ARCSINE;
+    /* Every source needs at most two angles; each is requested as (kind,
+     * argument) in the order the reference draws it and evaluated by one
+     * angle_draw, so that the double evaluation is emitted twice here. */
+    int kind = kAngNone;
+    float arg = 0.f;
+    /* ---- phase 1: draws up to the first angle ---- */
+    float rx = 0.f, ry = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+    if (planar) {
+        rx = rng_next(r0, r1);
+        ry = rng_next(r0, r1);
         if (st == MMC_SRC_PATTERN3D) {
             const float rz = rng_next(r0, r1);
             px = S.x + rx * P1.x;
@@ -254,43 +286,79 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
                 w = (C.srcnum == 1) ? pattern[C.patbase + posidx] : 1.f;
             } else if (st == MMC_SRC_FOURIER) {
                 const float kx = floorf(P1.w), ky = floorf(P2.w);
-                float sn, cs;
-                sincos_cr(mmc_twopi(kx * rx + ky * ry + P1.w - kx), &sn, &cs);
-                w = (cs * (1.f - P2.w + ky) + 1.f) * 0.5f;
+                kind = kAngCr;
+                arg = mmc_twopi(kx * rx + ky * ry + P1.w - kx);
             }
             ox += (P1.x + P2.x) * 0.5f;
             oy += (P1.y + P2.y) * 0.5f;
             oz += (P1.z + P2.z) * 0.5f;
         }
-    } else if (st == MMC_SRC_FOURIERX || st == MMC_SRC_FOURIERX2D) {
-        const float rx = rng_next(r0, r1);
-        const float ry = rng_next(r0, r1);
+    } else if (fourierx) {
+        rx = rng_next(r0, r1);
+        ry = rng_next(r0, r1);
         const float vw = P1.w * (1.f / sqrtf(P1.x * P1.x + P1.y * P1.y + P1.z * P1.z));
-        const float ux = vw * (D.y * P1.z - D.z * P1.y);
-        const float uy = vw * (D.z * P1.x - D.x * P1.z);
-        const float uz = vw * (D.x * P1.y - D.y * P1.x);
+        ux = vw * (D.y * P1.z - D.z * P1.y);
+        uy = vw * (D.z * P1.x - D.x * P1.z);
+        uz = vw * (D.x * P1.y - D.y * P1.x);
         px = S.x + rx * P1.x + ry * ux;
         py = S.y + rx * P1.y + ry * uy;
         pz = S.z + rx * P1.z + ry * uz;
-        if (st == MMC_SRC_FOURIERX2D) {
-            float sa, ca, sb, cb;
-            sincos_cr(mmc_twopi(P2.x * rx + P2.z), &sa, &ca);
-            sincos_cr(mmc_twopi(P2.y * ry + P2.w), &sb, &cb);
-            w = (sa * sb + 1.f) * 0.5f;
-        } else {
-            float sn, cs;
-            sincos_cr(mmc_twopi(P2.x * rx + P2.y * ry + P2.z), &sn, &cs);
-            w = (cs * (1.f - P2.w) + 1.f) * 0.5f;
-        }
+        kind = kAngCr;
+        arg = (st == MMC_SRC_FOURIERX2D) ? mmc_twopi(P2.x * rx + P2.z) : mmc_twopi(P2.x * rx + P2.y * ry + P2.z);
         ox += (P1.x + ux) * 0.5f;
         oy += (P1.y + uy) * 0.5f;
         oz += (P1.z + uz) * 0.5f;
-    } else if (st == MMC_SRC_DISK || st == MMC_SRC_RING || st == MMC_SRC_GAUSSIAN) {
-        float sphi, cphi;
-        if (st == MMC_SRC_RING && (P1.z > 0.f || P1.w > 0.f))
-            sincos_cr(fabsf(P1.z - P1.w) * rng_next(r0, r1) + fminf(P1.z, P1.w), &sphi, &cphi);
-        else
-            sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
+    } else if (disk && st == MMC_SRC_RING && (P1.z > 0.f || P1.w > 0.f)) {
+        kind = kAngCr;                        /* ring sector: azimuth between two angles */
+        arg = fabsf(P1.z - P1.w) * rng_next(r0, r1) + fminf(P1.z, P1.w);
+    } else if (disk || dirsrc || st == MMC_SRC_ZGAUSSIAN || st == MMC_SRC_HYPERBOLOID) {
+        kind = kAngAz;                        /* azimuth of the draw */
+        arg = rng_next(r0, r1);
+    }
+    float s1 = 0.f, c1 = 1.f;
+    if (kind != kAngNone) angle_draw(kind, arg, &s1, &c1, thr_lds);
+    /* ---- phase 2: draws up to the second angle ---- */
+    kind = kAngNone;
+    if (st == MMC_SRC_FOURIERX2D) {
+        kind = kAngCr;
+        arg = mmc_twopi(P2.y * ry + P2.w);
+    } else if (st == MMC_SRC_CONE) {
+        /* rejection of zeniths beyond the half angle (uniform in angle or in
+         * solid angle), bounded */
+        int k = 0;
+        float ang, uu;
+        do {
+            uu = rng_next(r0, r1);
+            ang = (P1.y > 0.f) ? mmc_twopi(uu) : acos_corrected(uu);
+        } while (ang > P1.x && ++k < kConeTries);
+        kind = (ang > P1.x) ? kAngCr : ((P1.y > 0.f) ? kAngAz : kAngIso);
+        arg = (ang > P1.x) ? P1.x : uu;
+    } else if (st == MMC_SRC_ISOTROPIC) {
+        kind = kAngIso;
+        arg = rng_next(r0, r1);
+    } else if (st == MMC_SRC_ARCSINE) {
+        kind = kAngPi;
+        arg = rng_next(r0, r1);
+    } else if (st == MMC_SRC_ZGAUSSIAN) {
+        /* the second draw comes from the photon's own stream: the
+         * reference takes it from ran0, which in mmc_run_mp is a
+         * thread-private generator state that rng_init never seeds
+         * (src/mmc_host.c:227,265,280-283; src/mmc_rand_xorshift128p.c:82-84),
+         * so its values cannot be reproduced */
+        const float u1 = rng_next(r0, r1);
+        const float u2 = rng_next(r0, r1);
+        kind = kAngCr;
+        arg = sqrtf((float)(-2.0 * log((double)u1))) * (1.f - 2.f * u2) * P1.x;
+    }
+    float s2 = 0.f, c2 = 1.f;
+    if (kind != kAngNone) angle_draw(kind, arg, &s2, &c2, thr_lds);
+    /* ---- phase 3: positions, weights and directions ---- */
+    if (st == MMC_SRC_FOURIER) {
+        w = (c1 * (1.f - P2.w + floorf(P2.w)) + 1.f) * 0.5f;
+    } else if (fourierx) {
+        w = (st == MMC_SRC_FOURIERX2D) ? (s1 * s2 + 1.f) * 0.5f : (c1 * (1.f - P2.w) + 1.f) * 0.5f;
+    } else if (disk) {
+        const float sphi = s1, cphi = c1;
         float rr;
         /* the Gaussian radius takes the double-precision log of the draw,
          * converted to float for sqrtf (src/mmc_raytrace.c:2444-2449) */
@@ -313,46 +381,15 @@ __device__ __forceinline__ bool launch_source(const SimCold& C, const float* pat
             px = S.x + rr * cphi;
             py = S.y + rr * sphi;
         }
-    } else if (st == MMC_SRC_CONE || st == MMC_SRC_ISOTROPIC || st == MMC_SRC_ARCSINE) {
-        float sphi, cphi, sth, cth;
-        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
-        if (st == MMC_SRC_CONE) {
-            int k = 0;
-            float ang, uu;
-            do {
-                uu = rng_next(r0, r1);
-                ang = (P1.y > 0.f) ? mmc_twopi(uu) : acos_draw(uu, &sth, &cth);
-            } while (ang > P1.x && ++k < kConeTries);
-            if (ang > P1.x)
-                sincos_cr(P1.x, &sth, &cth);
-            else if (P1.y > 0.f)
-                sincos_draw(MMC_FIX_AZ, uu, &sth, &cth);
-        } else if (st == MMC_SRC_ISOTROPIC) {
-            acos_draw(rng_next(r0, r1), &sth, &cth);
-        } else {
-            sincos_draw(MMC_FIX_PI, rng_next(r0, r1), &sth, &cth);
-        }
-        rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
+    } else if (dirsrc) {
+        rotate_dir(vx, vy, vz, s2, c2, s1, c1);
         return false;
     } else if (st == MMC_SRC_ZGAUSSIAN) {
-        float sphi, cphi;
-        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
-        /* the second draw comes from the photon's own stream: the
-         * reference takes it from ran0, which in mmc_run_mp is a
-         * thread-private generator state that rng_init never seeds
-         * (src/mmc_host.c:227,265,280-283; src/mmc_rand_xorshift128p.c:82-84),
-         * so its values cannot be reproduced */
-        const float u1 = rng_next(r0, r1);
-        const float u2 = rng_next(r0, r1);
-        const float ang = sqrtf((float)(-2.0 * log((double)u1))) * (1.f - 2.f * u2) * P1.x;
-        float sth, cth;
-        sincos_cr(ang, &sth, &cth);
-        rotate_dir(vx, vy, vz, sth, cth, sphi, cphi);
+        rotate_dir(vx, vy, vz, s2, c2, s1, c1);
         canfocus = false;
     } else if (st == MMC_SRC_HYPERBOLOID) {
-        float sphi, cphi;
-        sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
-        const float rr = sqrtf(0.5f * rng_scatlen(r0, r1)) * P1.x;   /* waist radius */
+        const float sphi = s1, cphi = c1;
+        const float rr = sqrtf(0.5f * rng_scatlen(r0, r1, thr_lds)) * P1.x;   /* waist radius */
         const float tilt = (P1.z > 0.f) ? (-P1.y / P1.z) : 0.f;
         float bx = -rr * sphi, by = rr * cphi, bz = P1.z;
         const float nb = 1.f / sqrtf(bx * bx + by * by + bz * bz);
@@ -530,6 +567,13 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     for (int i = tid; i < nstage * nmed; i += nthr) s_med[i] = a.media[i];
     for (int i = tid; i <= nsim; i += nthr) s_cum[i] = a.cum[i];
     for (int k = 0; k < nslots; k++) *LS(k) = 0.f;
+    __shared__ float s_thr[kThrLds];
+    /* the current element's geometry, per lane: Badouel's four face planes
+     * (nx, ny, nz, d as float4 rows) or Plucker's 12 node coordinates; kept
+     * while the photon scatters inside the element, reloaded after a
+     * crossing */
+
+    for (int i = tid; i < kThrLds; i += nthr) s_thr[i] = g_lm_thr[kThrLog + i];   /* log, azimuth */
     __syncthreads();
 
     const int lane = __lane_id();
@@ -557,9 +601,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     int ctx = CTX_OUTER, fixcount = 0;
     float eabs = 0.f, eabs_c = 0.f;
     uint32_t raytet = 0, raytet_start = 0;
-    int cache_e = -1;                   /* element whose geometry is in cNX..cfnb */
-    int4 cfnb = make_int4(0, 0, 0, 0), cee = cfnb;
+    int cache_e = -1;                   /* element whose geometry is in cNX..cDD and cfnb */
     float4 cNX = make_float4(0.f, 0.f, 0.f, 0.f), cNY = cNX, cNZ = cNX, cDD = cNX;
+    int4 cfnb = make_int4(0, 0, 0, 0), cee = cfnb;
     /* pending nodal deposits (Plucker) to the nodes of element cache_e at
      * field offset nbase */
     float na0 = 0.f, na1 = 0.f, na2 = 0.f, na3 = 0.f;
@@ -572,6 +616,18 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     float wm = 0.f;                     /* RF: weight magnitude (absorption ledger, roulette) */
     float slen0 = 0.f;                  /* scattering length at the last scattering event */
     int posidx = 0;                     /* launch pixel of a pattern source */
+    /* The first two draws of the next scattering event (scattering length,
+     * azimuth) are evaluated one event ahead, from a copy of the generator
+     * state: their floats (pf_l, pf_s, pf_c), the directions of their far
+     * neighbours and low bits of their generator outputs (pf_bits), and the
+     * agreement table bytes (pf_tl, pf_ta) -- fetched only for draws near a
+     * rounding midpoint, so that the fetch latency hides behind the steps in
+     * between.  Valid while no other draw intervenes (reflection and roulette
+     * draws invalidate them). */
+    float pf_l = 0.f, pf_s = 0.f, pf_c = 1.f;
+    uint32_t pf_tl = 0u, pf_ta = 0u, pf_bits = 0u;
+    bool pf_ok = false;
+
     /* the lane's simulation index into the tables, and its hot parameters */
     auto sidx = [&]() -> int { return sim; };
     auto hot_of = [&]() -> SimHot { return STAGE ? s_hot[sim] : a.hot[sim]; };
@@ -702,8 +758,20 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     ctx = CTX_OUTER; fixcount = 0;
                     eabs = 0.f; eabs_c = 0.f;
                     raytet_start = raytet;
-                    slen = rng_scatlen(r0, r1);
-                    const bool search = launch_source(C, a.pattern, r0, r1, px, py, pz, vx, vy, vz, w, posidx);
+                    /* drop the element cache: a new photon reloads its first
+                     * element once, and the launch code below (source
+                     * geometry, angle evaluations in double) gets the cache's
+                     * registers */
+                    cache_e = -1;
+                    cfnb = make_int4(0, 0, 0, 0);
+                    cee = cfnb;
+                    cNX = make_float4(0.f, 0.f, 0.f, 0.f);
+                    cNY = cNX;
+                    cNZ = cNX;
+                    cDD = cNX;
+                    na0 = na1 = na2 = na3 = 0.f;
+                    slen = rng_scatlen(r0, r1, s_thr);
+                    const bool search = launch_source(C, a.pattern, r0, r1, px, py, pz, vx, vy, vz, w, posidx, s_thr);
                     if (C.srctype == MMC_SRC_PATTERN && nsrc > 1)
                         for (int k = 0; k < nsrc; k++) {
                             const float pk = a.pattern[C.patbase + posidx * nsrc + k];
@@ -787,6 +855,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     }
                     if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */
                     if (hot_of().flags & kFlagTraj) savetraj();
+                    pf_ok = false;
                 }
             }
         }
@@ -1243,6 +1312,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                             float Rt = (Re - Im) / (Re + Im);
                             Re = t1 * Icos * Icos + t0 * t2 * t2;
                             Rt = (Rt + (Re - Im) / (Re + Im)) * 0.5f;
+                            pf_ok = false;
                             if (rng_next(r0, r1) <= Rt) {
                                 float k = -2.f * Icos;
                                 vx = 1.f * vx + k * pnx; vy = 1.f * vy + k * pny; vz = 1.f * vz + k * pnz;
@@ -1287,6 +1357,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                  * and scales both parts (src/mmc_core.cl:2090-2110) */
                 if ((hot.flags & kFlagRoulette) && (RF ? wm : w) < hot.minenergy) {
                     const float rsize = hot.roulettesize;
+                    pf_ok = false;
                     if (rng_next(r0, r1) * rsize <= 1.f) {
                         float wb = RF ? wm : w;
                         w *= rsize;
@@ -1302,36 +1373,75 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 if (!reason) {
                     /* mc_next_scatter (src/mmc_mesh.c:1644-1715) */
                     const float g = med(ctype).z;
-                    float sphi, cphi, stheta, ctheta, tmp0, tmp1;
-                    float nextslen = rng_scatlen(r0, r1);
-                    sincos_draw(MMC_FIX_AZ, rng_next(r0, r1), &sphi, &cphi);
-                    if (g > EPSF) {
-                        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * rng_next(r0, r1));
-                        tmp0 *= tmp0;
-                        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
-                        tmp0 = (tmp0 > 1.f) ? 1.f : tmp0;
-                        tmp0 = (tmp0 < -1.f) ? -1.f : tmp0;
-                        stheta = sqrtf(1.f - tmp0 * tmp0);
-                        ctheta = tmp0;
-                    } else {
-                        acos_draw(rng_next(r0, r1), &stheta, &ctheta);
+                    float stheta = 0.f, ctheta = 1.f;
+                    /* pass 0 evaluates this event's first two draws when
+                     * they were not evaluated ahead; pass 1 applies them,
+                     * draws the zenith, turns the direction and evaluates
+                     * the next event's first two draws */
+                    #pragma unroll 1
+                    for (int pass = pf_ok ? 1 : 0; pass < 2; pass++) {
+                        if (pass == 1) {
+                            (void)rng_next(r0, r1);          /* the two draws evaluated ahead */
+                            (void)rng_next(r0, r1);
+                            const float uz = rng_next(r0, r1);
+                            const uint32_t kl = pf_bits & 7u, ka = (pf_bits >> 3) & 3u;
+                            const float nextslen =
+                                -(((pf_tl >> kl) & 1u) ? mmc_fstep(pf_l, (pf_bits >> 5) & 1u) : pf_l);
+                            const float sphi = ((pf_ta >> (2 * ka)) & 1u) ? mmc_fstep(pf_s, (pf_bits >> 6) & 1u) : pf_s;
+                            const float cphi =
+                                ((pf_ta >> (2 * ka + 1)) & 1u) ? mmc_fstep(pf_c, (pf_bits >> 7) & 1u) : pf_c;
+                            float tmp0, tmp1;
+                            if (g > EPSF) {
+                                tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * uz);
+                                tmp0 *= tmp0;
+                                tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+                                tmp0 = (tmp0 > 1.f) ? 1.f : tmp0;
+                                tmp0 = (tmp0 < -1.f) ? -1.f : tmp0;
+                                stheta = sqrtf(1.f - tmp0 * tmp0);
+                                ctheta = tmp0;
+                            } else {
+                                angle_draw(kAngIso, uz, &stheta, &ctheta, s_thr);
+                            }
+                            float qx, qy, qz;
+                            if (vz > -1.f + EPSF && vz < 1.f - EPSF) {
+                                tmp0 = 1.f - vz * vz;
+                                tmp1 = 1.f / sqrtf(tmp0);
+                                tmp1 = stheta * tmp1;
+                                qx = tmp1 * (vx * vz * cphi - vy * sphi) + vx * ctheta;
+                                qy = tmp1 * (vy * vz * cphi + vx * sphi) + vy * ctheta;
+                                qz = -tmp1 * tmp0 * cphi + vz * ctheta;
+                            } else {
+                                qx = stheta * cphi;
+                                qy = stheta * sphi;
+                                qz = (vz > 0.f) ? ctheta : -ctheta;
+                            }
+                            vx = qx; vy = qy; vz = qz;
+                            slen = nextslen;
+                            slen0 = nextslen;
+                        }
+                        /* the next two draws from a copy of the state:
+                         * -logf(u + EPS) and sincosf((float)(2 pi u)) in
+                         * double, rounded, with the table bytes of those near
+                         * a midpoint (src/mmc_rand_common.h:50-51,
+                         * src/mmc_mesh.c:1662-1666) */
+                        uint64_t a0 = r0, a1 = r1;
+                        const float u1 = rng_next(a0, a1), u2 = rng_next(a0, a1);
+                        const uint32_t k1 = draw_bits(u1), k2 = draw_bits(u2);
+                        const double el = mmc_log_dk(u1 + EPSF, dk());
+                        double es, ec;
+                        mmc_sincos_dk(mmc_twopi(u2), &es, &ec, dk());
+                        pf_l = (float)el;
+                        pf_s = (float)es;
+                        pf_c = (float)ec;
+                        const float* tha = s_thr + MMC_LIBM_NREG + 2 * (k2 >> MMC_LIBM_REGSHIFT);
+                        const bool nl = lm_near(el, pf_l, s_thr[k1 >> MMC_LIBM_REGSHIFT]);
+                        const bool na = lm_near(es, pf_s, tha[0]) || lm_near(ec, pf_c, tha[1]);
+                        pf_tl = nl ? (uint32_t)g_lm_log[k1 >> 3] : 0u;
+                        pf_ta = na ? (uint32_t)g_lm_az[k2 >> 2] : 0u;
+                        pf_bits = (k1 & 7u) | ((k2 & 3u) << 3) | ((uint32_t)(el > (double)pf_l) << 5) |
+                                  ((uint32_t)(es > (double)pf_s) << 6) | ((uint32_t)(ec > (double)pf_c) << 7);
+                        pf_ok = true;
                     }
-                    float qx, qy, qz;
-                    if (vz > -1.f + EPSF && vz < 1.f - EPSF) {
-                        tmp0 = 1.f - vz * vz;
-                        tmp1 = 1.f / sqrtf(tmp0);
-                        tmp1 = stheta * tmp1;
-                        qx = tmp1 * (vx * vz * cphi - vy * sphi) + vx * ctheta;
-                        qy = tmp1 * (vy * vz * cphi + vx * sphi) + vy * ctheta;
-                        qz = -tmp1 * tmp0 * cphi + vz * ctheta;
-                    } else {
-                        qx = stheta * cphi;
-                        qy = stheta * sphi;
-                        qz = (vz > 0.f) ? ctheta : -ctheta;
-                    }
-                    vx = qx; vy = qy; vz = qz;
-                    slen = nextslen;
-                    slen0 = nextslen;
                     if (hot.flags & kFlagTraj) savetraj();
                     if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
                     /* momentum transfer (src/mmc_raytrace.c:2073-2075) */

@@ -93,9 +93,9 @@ extern "C" void mmc_hip_photon_seeds(uint64_t seed, int64_t first, int64_t n, ui
  * kernel's tables, before that device's first launch.  Host copies are
  * static, so later launches allocate nothing for them. */
 static std::mutex g_libm_mu;
-static uint8_t g_tab_log[MMC_FIXB_LOG], g_tab_az[MMC_FIXB_AZ], g_tab_iso[MMC_FIXB_ISO], g_tab_pi[MMC_FIXB_PI];
-static mmc_libm_escape g_tab_esc[MMC_LIBM_MAXESC];
-static int g_tab_nesc = -1;
+static uint8_t g_tab_bits[4][MMC_FIXB_ISO];
+static float g_tab_thr[4][MMC_LIBM_NREG * 3];
+static bool g_tab_built = false;
 static bool g_tab_dev[64];
 
 static int ensure_libm_tables() {
@@ -104,21 +104,19 @@ static int ensure_libm_tables() {
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return set_err(-1, "device index %d out of range", dev);
     if (g_tab_dev[dev]) return 0;
-    if (g_tab_nesc < 0) {
+    if (!g_tab_built) {
         if (!__builtin_cpu_supports("fma"))
             return set_err(-1, "the host CPU lacks FMA, needed to build the C library agreement tables");
-        uint8_t* tabs[4] = {g_tab_log, g_tab_az, g_tab_iso, g_tab_pi};
-        int n = 0;
         for (int w = 0; w < 4; w++) {
-            const int e = mmc_host_libm_table(w, tabs[w], g_tab_esc + n, MMC_LIBM_MAXESC - n);
-            if (e < 0 || n + e > MMC_LIBM_MAXESC)
-                return set_err(-1, "C library agreement table %d: %d values outside one ulp (limit %d)", w, e,
-                               MMC_LIBM_MAXESC - n);
-            n += e;
+            const int e = mmc_host_libm_table(w, g_tab_bits[w], g_tab_thr[w]);
+            if (e != 0)
+                return set_err(-1, "C library agreement table %d: %d values the tables cannot reproduce", w, e);
         }
-        g_tab_nesc = n;
+        g_tab_built = true;
     }
-    int rc = upload_libm_tables(g_tab_log, g_tab_az, g_tab_iso, g_tab_pi, g_tab_esc, g_tab_nesc);
+    int rc = upload_libm_tables(g_tab_bits[MMC_FIX_LOG], g_tab_bits[MMC_FIX_AZ], g_tab_bits[MMC_FIX_ISO],
+                                g_tab_bits[MMC_FIX_PI], g_tab_thr[MMC_FIX_LOG], g_tab_thr[MMC_FIX_AZ],
+                                g_tab_thr[MMC_FIX_ISO], g_tab_thr[MMC_FIX_PI]);
     if (rc) return rc;
     g_tab_dev[dev] = true;
     return 0;

@@ -39,8 +39,10 @@
 #ifdef ORC_PORTABLE_MATH
 #include "mmc_fp32math.h"
 static inline void orc_sincos_cr(float x, float* s, float* c) {
-    *s = (float)sin((double)x);
-    *c = (float)cos((double)x);
+    double sd, cd;
+    mmc_sincos_d(x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
 }
 #define O_EXPF(x) mmc_expf(x)
 #define O_SINCOSF_C(x, s, c) orc_sincos_cr(x, s, c)
@@ -102,78 +104,122 @@ float orc_rand_scatlen(uint64_t t[2]) {
 
 #ifdef ORC_PORTABLE_MATH
 /* ------------------------------------------------------------------------ */
-/* Check of the GPU's agreement tables (mmc_host_libm_table): the kernel's    */
-/* rule -- FP32 function of mmc_fp32math.h, then the 2-bit code of the table  */
-/* (0 keep, 1 raw bits + 1, 2 raw bits - 1, 3 escape list) -- applied over   */
-/* every generator output k, counting the values that differ from glibc.     */
-/* Returns the number of differing values.                                   */
+/* Check of the GPU's agreement tables (mmc_host_libm_table), restating the  */
+/* kernel's rule: the double evaluation of mmc_fp32math.h rounded to float  */
+/* f, replaced by its far neighbour o when the double lies within D |o - f|  */
+/* of the midpoint (D: the region's threshold) and the table bit is set.     */
+/* Applied over every generator output k; returns the number of values that  */
+/* differ from glibc's logf / sincosf / acosf.                               */
 /* ------------------------------------------------------------------------ */
-static float orc_fix1(float y, unsigned c) {
-    uint32_t b;
-    memcpy(&b, &y, 4);
-    b = (c == 1u) ? b + 1u : (c == 2u ? b - 1u : b);
-    memcpy(&y, &b, 4);
-    return y;
-}
-static int orc_fix_escape(int which, uint32_t k, const int32_t* esc, int nesc, float v[3]) {
-    for (int i = 0; i < nesc; i++)
-        if (esc[4 * i] == (int32_t)(((uint32_t)which << 23) | k)) {
-            memcpy(v, esc + 4 * i + 1, 12);
-            return 1;
-        }
-    return 0;
+static float orc_lm(double e, float D, const uint8_t* bits, uint32_t bitpos) {
+    float f = (float)e;
+    if (e == (double)f) return f;
+    const float o = mmc_fnext(f, e);
+    const double fd = f, od = o;
+    if (fabs(e - 0.5 * (fd + od)) <= (double)D * fabs(od - fd) && ((bits[bitpos >> 3] >> (bitpos & 7)) & 1u))
+        return o;
+    return f;
 }
 static int orc_same(float a, float b) { return memcmp(&a, &b, 4) == 0; }
 
-int64_t orc_libm_verify(int which, const uint8_t* tab, const int32_t* esc, int nesc) {
+/* number of generator outputs whose draw of table `which` (0 log, 1 azimuth)
+ * lies within its region's threshold of a rounding midpoint, i.e. reads the
+ * table on the GPU */
+int64_t orc_libm_lookups(int which, const float* thr) {
+    int64_t n = 0;
+    for (uint32_t k = 0; k < (1u << 23); k++) {
+        uint32_t ub = 0x3F800000u | k;
+        float u;
+        memcpy(&u, &ub, 4);
+        u -= 1.0f;
+        const uint32_t reg = k >> 16;
+        double e[2];
+        int nv = 1;
+        float D[2] = {thr[reg], 0.f};
+        if (which == 0) {
+            e[0] = mmc_log_d(u + EPS);
+        } else {
+            mmc_sincos_d((float)(TWO_PI * u), &e[0], &e[1]);
+            nv = 2;
+            D[0] = thr[2 * reg];
+            D[1] = thr[2 * reg + 1];
+        }
+        int near = 0;
+        for (int v = 0; v < nv; v++) {
+            const float f = (float)e[v];
+            if (e[v] == (double)f) continue;
+            const double fd = f, od = mmc_fnext(f, e[v]);
+            near |= fabs(e[v] - 0.5 * (fd + od)) <= (double)D[v] * fabs(od - fd);
+        }
+        n += near;
+    }
+    return n;
+}
+
+/* largest relative difference of the double evaluations from glibc's double
+ * log / sin / cos / acos over every generator output: 0 log(u + EPS), 1 sin
+ * and cos of (float)(2 pi u), 2 acos(2u - 1), 3 sin and cos of (float)(pi u) */
+double orc_libm_dd_maxrel(int which) {
+    double mx = 0.0;
+    for (uint32_t k = 0; k < (1u << 23); k++) {
+        uint32_t ub = 0x3F800000u | k;
+        float u;
+        memcpy(&u, &ub, 4);
+        u -= 1.0f;
+        double e[2], r[2];
+        int nv = 1;
+        if (which == 0) {
+            e[0] = mmc_log_d(u + EPS);
+            r[0] = log((double)(u + EPS));
+        } else if (which == 2) {
+            e[0] = mmc_acos_d(2.f * u - 1.f);
+            r[0] = acos((double)(2.f * u - 1.f));
+        } else {
+            const float a = (which == 1) ? (float)(TWO_PI * u) : (float)(M_PI * u);
+            mmc_sincos_d(a, &e[0], &e[1]);
+            r[0] = sin((double)a);
+            r[1] = cos((double)a);
+            nv = 2;
+        }
+        for (int v = 0; v < nv; v++)
+            if (r[v] != 0.0) {
+                const double d = fabs(e[v] - r[v]) / fabs(r[v]);
+                if (d > mx) mx = d;
+            }
+    }
+    return mx;
+}
+
+int64_t orc_libm_verify(int which, const uint8_t* bits, const float* thr) {
     int64_t bad = 0;
     for (uint32_t k = 0; k < (1u << 23); k++) {
-        uint32_t bits = 0x3F800000u | k;
+        uint32_t ub = 0x3F800000u | k;
         float u;
-        memcpy(&u, &bits, 4);
+        memcpy(&u, &ub, 4);
         u -= 1.0f;
-        float v[3];
-        if (which == 0) {            /* logf(u + EPS) */
-            unsigned c = (tab[k >> 2] >> (2 * (k & 3))) & 3u;
-            float y = mmc_logf(u + EPS);
-            if (c == 3u) {
-                if (!orc_fix_escape(0, k, esc, nesc, v)) { bad++; continue; }
-                y = v[0];
-            } else {
-                y = orc_fix1(y, c);
-            }
+        const uint32_t reg = k >> 16;
+        if (which == 0) {                        /* logf(u + EPS): 1 bit per k */
+            const float y = orc_lm(mmc_log_d(u + EPS), thr[reg], bits, k);
             bad += !orc_same(y, logf(u + EPS));
-        } else if (which == 1 || which == 3) {   /* sincosf((float)(2 pi u)) or (float)(pi u) */
-            unsigned c = (tab[k >> 1] >> (4 * (k & 1))) & 15u;
-            float a = (which == 1) ? (float)(TWO_PI * u) : (float)(M_PI * u);
-            float s, co, gs, gc;
-            mmc_sincosf(a, &s, &co);
-            if (c == 15u) {
-                if (!orc_fix_escape(which, k, esc, nesc, v)) { bad++; continue; }
-                s = v[0];
-                co = v[1];
-            } else {
-                s = orc_fix1(s, c & 3u);
-                co = orc_fix1(co, c >> 2);
-            }
+        } else if (which == 1 || which == 3) {   /* sincosf((float)(2 pi u)) or (float)(pi u): 2 bits */
+            const float a = (which == 1) ? (float)(TWO_PI * u) : (float)(M_PI * u);
+            double es, ec;
+            float gs, gc;
+            mmc_sincos_d(a, &es, &ec);
+            const float s = orc_lm(es, thr[2 * reg], bits, 2 * k);
+            const float c = orc_lm(ec, thr[2 * reg + 1], bits, 2 * k + 1);
             sincosf(a, &gs, &gc);
-            bad += !orc_same(s, gs) || !orc_same(co, gc);
-        } else {                     /* t = acosf(2u - 1), sincosf(t) */
-            unsigned c = tab[k];
-            float t = orc_fix1(mmc_acosf(2.f * u - 1.f), c & 3u), s, co, gs, gc;
-            mmc_sincosf(t, &s, &co);
-            if ((c & 3u) == 3u) {
-                if (!orc_fix_escape(2, k, esc, nesc, v)) { bad++; continue; }
-                t = v[0];
-                s = v[1];
-                co = v[2];
-            } else {
-                s = orc_fix1(s, (c >> 2) & 3u);
-                co = orc_fix1(co, (c >> 4) & 3u);
-            }
-            float gt = acosf(2.f * u - 1.f);
+            bad += !orc_same(s, gs) + !orc_same(c, gc);
+        } else {                                 /* t = acosf(2u - 1), sincosf(t): 4 bits */
+            const float t = orc_lm(mmc_acos_d(2.f * u - 1.f), thr[3 * reg], bits, 4 * k);
+            double es, ec;
+            float gs, gc;
+            mmc_sincos_d(t, &es, &ec);
+            const float s = orc_lm(es, thr[3 * reg + 1], bits, 4 * k + 1);
+            const float c = orc_lm(ec, thr[3 * reg + 2], bits, 4 * k + 2);
+            const float gt = acosf(2.f * u - 1.f);
             sincosf(gt, &gs, &gc);
-            bad += !orc_same(t, gt) || !orc_same(s, gs) || !orc_same(co, gc);
+            bad += !orc_same(t, gt) + !orc_same(s, gs) + !orc_same(c, gc);
         }
     }
     return bad;

@@ -1,12 +1,14 @@
-"""Agreement tables between the deterministic FP32 functions and glibc.
+"""Agreement tables between the GPU's draws and glibc.
 
-The GPU kernel draws scattering lengths, azimuths and isotropic zeniths as the
-FP32 functions of include/mmc_fp32math.h corrected by a per-generator-output
-code from the library's tables (mmc_host_libm_table).  The oracle restates the
-kernel's rule for applying a code and checks, over all 2^23 generator outputs,
-that the corrected values equal glibc's logf / sincosf / acosf -- the
-functions the reference CPU tracer calls (src/mmc_rand_common.h:50-51,
-src/mmc_mesh.c:1662-1690, src/mmc_raytrace.c:2434-2480).
+The GPU kernel draws scattering lengths, azimuths and isotropic zeniths as
+double evaluations (include/mmc_fp32math.h) rounded to float and, within a
+per-region distance of the rounding midpoint, replaced by the far neighbour
+where the library's table bit says so (mmc_host_libm_table).  The oracle
+restates that rule and checks, over all 2^23 generator outputs, that the
+values equal glibc's logf / sincosf / acosf -- the functions the reference
+CPU tracer calls (src/mmc_rand_common.h:50-51, src/mmc_mesh.c:1662-1690,
+src/mmc_raytrace.c:2434-2480).  The double evaluations are pinned against
+glibc's double-precision log / sin / cos / acos.
 """
 import ctypes as C
 
@@ -17,48 +19,75 @@ import pyoracle as O
 from mmc_amd import _lib
 
 NAMES = {0: "log", 1: "azimuth", 2: "isotropic", 3: "arcsine"}
-
-
-class Esc(C.Structure):
-    _fields_ = [("key", C.c_int32), ("v", C.c_uint32 * 3)]
+NREG = 1 << 7
 
 
 def build_table(which):
     L = _lib.lib()
-    n = L.mmc_host_libm_table_bytes(which)
-    tab = np.zeros(n, np.uint8)
-    esc = (Esc * 64)()
-    ne = L.mmc_host_libm_table(which, tab.ctypes.data_as(C.c_void_p), C.cast(esc, C.c_void_p), 64)
-    assert 0 <= ne <= 64
-    flat = np.array([[e.key, *[np.int32(np.uint32(x)) for x in e.v]] for e in esc[:ne]], np.int32).reshape(-1, 4)
-    return tab, np.ascontiguousarray(flat)
+    nval = L.mmc_host_libm_nval(which)
+    bits = np.zeros(L.mmc_host_libm_table_bytes(which), np.uint8)
+    thr = np.zeros(NREG * nval, np.float32)
+    bad = L.mmc_host_libm_table(which, bits.ctypes.data_as(C.c_void_p), thr.ctypes.data_as(C.c_void_p))
+    assert bad == 0, NAMES[which]
+    return bits, thr
 
 
-def verify(which, tab, esc):
+def verify(which, bits, thr):
     L = O.lib("portable")
     L.orc_libm_verify.restype = C.c_int64
-    L.orc_libm_verify.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]
-    return L.orc_libm_verify(which, tab.ctypes.data_as(C.c_void_p),
-                             esc.ctypes.data_as(C.c_void_p) if len(esc) else None, len(esc))
+    L.orc_libm_verify.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+    return L.orc_libm_verify(which, bits.ctypes.data_as(C.c_void_p), thr.ctypes.data_as(C.c_void_p))
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return {w: build_table(w) for w in NAMES}
 
 
 @pytest.mark.parametrize("which", [0, 1, 2, 3])
-def test_corrected_values_equal_glibc_on_every_generator_output(which):
-    tab, esc = build_table(which)
-    # the FP32 functions alone differ from glibc on a large share of inputs
-    # (the codes are not all zero) ...
-    assert np.count_nonzero(tab) > 10000, NAMES[which]
-    # ... and with the codes applied they agree everywhere
-    assert verify(which, tab, esc) == 0, NAMES[which]
+def test_corrected_values_equal_glibc_on_every_generator_output(tables, which):
+    bits, thr = tables[which]
+    # the rounded double evaluations alone differ from glibc on tens of
+    # thousands of generator outputs (glibc is not correctly rounded) ...
+    assert np.unpackbits(bits).sum() > 10000, NAMES[which]
+    # ... and with the table applied they agree everywhere
+    assert verify(which, bits, thr) == 0, NAMES[which]
 
 
-def test_a_wrong_code_is_detected():
-    tab, esc = build_table(0)
-    k = int(np.flatnonzero(tab)[0])            # a byte holding a nonzero code
-    bad = tab.copy()
-    bad[k] ^= 0x3                              # flips the code of generator output 4k
-    assert verify(0, bad, esc) >= 1
-    tab1, esc1 = build_table(1)
-    bad1 = tab1.copy()
-    bad1[12345] = (bad1[12345] & 0xF3) | ((((bad1[12345] >> 2) & 3) ^ 1) << 2)   # cos code of output 24690
-    assert verify(1, bad1, esc1) >= 1
+def test_only_a_few_percent_of_the_draws_read_the_table(tables):
+    """Thresholds are tight: the share of draws within a region's threshold
+    of the midpoint (the ones that read a bit) stays small for the two draws
+    of every scattering event (log and azimuth)."""
+    L = O.lib("portable")
+    L.orc_libm_lookups.restype = C.c_int64
+    L.orc_libm_lookups.argtypes = [C.c_int, C.c_void_p]
+    for which, limit in ((0, 0.03), (1, 0.08)):
+        bits, thr = tables[which]
+        share = L.orc_libm_lookups(which, thr.ctypes.data_as(C.c_void_p)) / float(1 << 23)
+        print(f"{NAMES[which]}: {share:.4f} of the draws read the table")
+        assert 0.0 < share < limit
+
+
+def test_a_wrong_bit_or_threshold_is_detected(tables):
+    bits, thr = tables[0]
+    k = int(np.flatnonzero(bits)[0])           # a byte holding a set bit
+    bad = bits.copy()
+    bad[k] = 0
+    assert verify(0, bad, thr) >= 1
+    low = thr.copy()
+    low[thr > 0] *= 0.5                        # thresholds that miss set bits
+    assert verify(0, bits, low) >= 1
+    bits1, thr1 = tables[1]
+    bad1 = bits1.copy()
+    j = int(np.flatnonzero(bits1 & 0x2)[0])    # a set cos bit (generator output 4j)
+    bad1[j] &= 0xFD
+    assert verify(1, bad1, thr1) >= 1
+
+
+def test_double_evaluations_track_glibc_double_precision():
+    L = O.lib("portable")
+    L.orc_libm_dd_maxrel.restype = C.c_double
+    L.orc_libm_dd_maxrel.argtypes = [C.c_int]
+    for which in range(4):
+        rel = L.orc_libm_dd_maxrel(which)
+        assert rel < 2.0 ** -47, (NAMES[which], rel)
