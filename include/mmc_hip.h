@@ -293,6 +293,28 @@ int32_t mmc_hip_slot_wait(mmc_hip_slot_handle slot, mmc_hip_result* res);
  * returns its sum, and zeroes the region for reuse. */
 int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, uint64_t n, double* host, double* sum);
 
+/* ---- allocation-free launch (no counterpart in the reference) --------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Runs the
+ * same launch as mmc_hip_run_chunks (same arguments, same results) but
+ * allocates and frees nothing: the launch's own device buffers (simulation
+ * tables, energy tallies, counters, work queue; detector, photon, trajectory
+ * and replay buffers when the simulations ask for them) are windows of a
+ * static device array of MMC_NOALLOC_SCRATCH bytes that each device keeps
+ * until the process ends.  The caller passes the inputs (a resident mesh)
+ * and the outputs (the arena and res).  A launch whose buffers exceed the
+ * array fails with a message and runs nothing.  The array is shared by every
+ * call on a device and keeps the last call's contents; calls on one device
+ * are serialized by the library, and this entry point's launches must not
+ * overlap other work of the caller that uses it on that device.  The first
+ * call on a device looks up the array's address and copies the host C
+ * library agreement tables (no allocation): run one call before capturing
+ * a graph. */
+#define MMC_NOALLOC_SCRATCH (8u << 20)
+int32_t mmc_hip_run_chunks_noalloc(mmc_hip_mesh_handle h, const mmc_hip_sim* sims, int32_t nsim,
+                                   mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                                   const mmc_hip_tune* tune, mmc_hip_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,8 @@ SIGNATURES = {
     "mmc_hip_run_chunks": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
                                        C.POINTER(TuneC), C.POINTER(ResultC)]),
     "mmc_hip_arena_collect": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64, f64p, f64p]),
+    "mmc_hip_run_chunks_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,
+                                               C.POINTER(TuneC), C.POINTER(ResultC)]),
     "mmc_hip_slot_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
     "mmc_hip_slot_release": (C.c_int32, [C.c_void_p]),
     "mmc_hip_run_chunks_async": (C.c_int32, [C.c_void_p, C.POINTER(SimC), C.c_int32, C.c_void_p, u64p,

@@ -470,12 +470,16 @@ struct DevBuf {
     }
 };
 
-/* Device buffers that persist across launches and only grow. */
+/* Device buffers that persist across launches and only grow; or, for the
+ * allocation-free launch, a window of the static scratch array (fixed:
+ * never allocated or freed here). */
 struct Pooled {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;
     int ensure(size_t bytes) {
         if (bytes <= cap && p) return 0;
+        if (fixed) return set_err(-1, "internal: scratch window of %zu bytes asked for %zu", cap, bytes);
         /* grow geometrically from 64 KiB: a free can wait for the device, so
          * steady-state launches must not reallocate */
         size_t want = std::max<size_t>(bytes, 64 << 10);
@@ -490,7 +494,7 @@ struct Pooled {
         return 0;
     }
     ~Pooled() {
-        if (p) (void)hipFree(p);
+        if (p && !fixed) (void)hipFree(p);
     }
 };
 
@@ -548,6 +552,11 @@ struct mmc_hip_slotres {
     std::vector<float> h_rw, h_rt;
     std::vector<SimGrid> h_grid;
     std::chrono::steady_clock::time_point t_start;
+    /* allocation-free launches: the per-launch buffers are carved from this
+     * device's scratch array (bump_base, bump_cap bytes) instead of pooled
+     * allocations; the null stream, no events */
+    unsigned char* bump_base = nullptr;
+    size_t bump_cap = 0, bump_used = 0;
     ~mmc_hip_slotres() {
         if (st) {
             (void)hipStreamSynchronize(st);
@@ -578,6 +587,20 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
                             const uint64_t* arena_off) {
     if (sl->busy) return set_err(-1, "launch slot busy: wait for its launch first");
     sl->t_start = std::chrono::steady_clock::now();
+    sl->bump_used = 0;
+    /* a per-launch device buffer: pooled, or a window of the scratch array */
+    auto need = [sl](Pooled& b, size_t bytes) -> int {
+        if (!sl->bump_base) return b.ensure(bytes);
+        const size_t off = (sl->bump_used + 255) & ~(size_t)255;
+        if (off + bytes > sl->bump_cap)
+            return set_err(-1, "allocation-free launch: its buffers need more than the %zu bytes of scratch",
+                           sl->bump_cap);
+        b.p = sl->bump_base + off;
+        b.cap = bytes;
+        b.fixed = true;
+        sl->bump_used = off + std::max<size_t>(bytes, 1);
+        return 0;
+    };
     /* own fields left by a launch that failed before it was enqueued */
     if (sl->field.p) (void)hipFree(sl->field.p);
     if (sl->lo.p) (void)hipFree(sl->lo.p);
@@ -747,25 +770,26 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         nrep = auto_replicas((uint64_t)fieldlen * sizeof(float) * (comp ? 2 : 1));
     }
     int rc;
-    if ((rc = sl->d_hot.ensure(sizeof(SimHot) * nsim)) || (rc = sl->d_cold.ensure(sizeof(SimCold) * nsim)) ||
-        (rc = sl->d_med.ensure(sizeof(float4) * media.size())) || (rc = sl->d_cum.ensure(sizeof(int64_t) * cum.size())) ||
-        (rc = sl->d_det.ensure(sizeof(float4) * dets.size())) || (rc = sl->d_pat.ensure(sizeof(float) * pats.size())) ||
-        (rc = sl->d_acc.ensure(sizeof(double) * nsim * kAccStride)) ||
-        (rc = sl->d_detcnt.ensure(sizeof(unsigned long long) * (nsim + 2))) ||
-        (rc = sl->d_queue.ensure(sizeof(unsigned long long))))
+    if (sl->bump_base && !arena) return set_err(-1, "allocation-free launch: needs an arena for its fields");
+    if ((rc = need(sl->d_hot, sizeof(SimHot) * nsim)) || (rc = need(sl->d_cold, sizeof(SimCold) * nsim)) ||
+        (rc = need(sl->d_med, sizeof(float4) * media.size())) || (rc = need(sl->d_cum, sizeof(int64_t) * cum.size())) ||
+        (rc = need(sl->d_det, sizeof(float4) * dets.size())) || (rc = need(sl->d_pat, sizeof(float) * pats.size())) ||
+        (rc = need(sl->d_acc, sizeof(double) * nsim * kAccStride)) ||
+        (rc = need(sl->d_detcnt, sizeof(unsigned long long) * (nsim + 2))) ||
+        (rc = need(sl->d_queue, sizeof(unsigned long long))))
         return rc;
     if (!seeds.empty() &&
-        ((rc = sl->d_seeds.ensure(sizeof(uint64_t) * seeds.size())) || (rc = sl->d_rw.ensure(sizeof(float) * rws.size())) ||
-         (rc = sl->d_rt.ensure(sizeof(float) * rts.size()))))
+        ((rc = need(sl->d_seeds, sizeof(uint64_t) * seeds.size())) || (rc = need(sl->d_rw, sizeof(float) * rws.size())) ||
+         (rc = need(sl->d_rt, sizeof(float) * rts.size()))))
         return rc;
-    if (!grids.empty() && (rc = sl->d_grid.ensure(sizeof(SimGrid) * grids.size()))) return rc;
+    if (!grids.empty() && (rc = need(sl->d_grid, sizeof(SimGrid) * grids.size()))) return rc;
     if (!arena) {
         if ((rc = dalloc((float**)&sl->field.p, (size_t)fieldlen * nrep, nullptr))) return rc;
         if (comp && (rc = dalloc((float**)&sl->lo.p, (size_t)fieldlen * nrep, nullptr))) return rc;
     }
     if (savedet && maxdet > 0) {
-        if ((rc = sl->d_detdata.ensure(sizeof(float) * (size_t)maxdet * reclen)) ||
-            (rc = sl->d_detph.ensure(sizeof(int64_t) * (size_t)maxdet)))
+        if ((rc = need(sl->d_detdata, sizeof(float) * (size_t)maxdet * reclen)) ||
+            (rc = need(sl->d_detph, sizeof(int64_t) * (size_t)maxdet)))
             return rc;
     }
     /* per-photon records: one window, taken from the first simulation's request */
@@ -776,7 +800,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             reccount = std::min(res[i].reccount, sims[i].nphoton - res[i].recfirst);
             break;
         }
-    if (reccount > 0 && (rc = sl->d_rec.ensure(sizeof(mmc_hip_photon) * (size_t)reccount))) return rc;
+    if (reccount > 0 && (rc = need(sl->d_rec, sizeof(mmc_hip_photon) * (size_t)reccount))) return rc;
     /* trajectory records: one buffer per launch, returned through the first
      * result that provides one */
     int trajres = -1;
@@ -787,7 +811,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             maxtraj = res[i].maxtraj;
             break;
         }
-    if (maxtraj > 0 && (rc = sl->d_traj.ensure(sizeof(float) * MMC_TRAJ_RECLEN * (size_t)maxtraj))) return rc;
+    if (maxtraj > 0 && (rc = need(sl->d_traj, sizeof(float) * MMC_TRAJ_RECLEN * (size_t)maxtraj))) return rc;
 
     hipStream_t st = sl->st;
     HIPCHK(hipMemcpyAsync(sl->d_hot.p, hot.data(), sizeof(SimHot) * nsim, hipMemcpyHostToDevice, st));
@@ -809,7 +833,10 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         HIPCHK(hipMemsetAsync(sl->field.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
         if (sl->lo.p) HIPCHK(hipMemsetAsync(sl->lo.p, 0, sizeof(float) * (fieldlen ? fieldlen * nrep : 1), st));
     }
-    HIPCHK(hipMemsetAsync(sl->d_acc.p, 0, sizeof(double) * nsim * kAccStride, st));
+    /* energy tallies, queue and counters are cleared by fill kernels (no
+     * memset nodes when a caller captures the launch in a graph) */
+    hipLaunchKernelGGL(fill_zero_u64, dim3((nsim * kAccStride + 63) / 64), dim3(64), 0, st,
+                       (unsigned long long*)sl->d_acc.p, nsim * kAccStride);
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)sl->d_queue.p, 1);
     hipLaunchKernelGGL(fill_zero_u64, dim3((nsim + 65) / 64), dim3(64), 0, st,
                        (unsigned long long*)sl->d_detcnt.p, nsim + 2);
@@ -909,13 +936,13 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     int bpc = want > 0 ? std::min(want, std::max(maxbpc, 1)) : std::max(maxbpc, 1);
     int grid = ncu * bpc;
 
-    HIPCHK(hipEventRecord(sl->e0, st));
+    if (sl->e0) HIPCHK(hipEventRecord(sl->e0, st));
     if (total > 0) {
         void* kargs[] = {&a};
         HIPCHK(hipLaunchKernel(kfn, dim3(grid), dim3(block), kargs, shmem, st));
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipEventRecord(sl->e1, st));
+    if (sl->e1) HIPCHK(hipEventRecord(sl->e1, st));
 
     /* state slot_finish needs */
     sl->busy = true;
@@ -974,7 +1001,12 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
     HIPCHK(hipMemcpyAsync(cnt.data(), sl->d_detcnt.p, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     float kms = 0.f;
-    HIPCHK(hipEventElapsedTime(&kms, sl->e0, sl->e1));
+    if (sl->e0 && sl->e1) {
+        HIPCHK(hipEventElapsedTime(&kms, sl->e0, sl->e1));
+    } else {
+        /* allocation-free launch (no events): the wall time of the launch */
+        kms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sl->t_start).count();
+    }
     for (int i = 0; i < nsim; i++) {
         mmc_hip_result& r = res[i];
         const double* A = acc.data() + (size_t)i * kAccStride;
@@ -1141,6 +1173,44 @@ extern "C" int32_t mmc_hip_run_chunks(mmc_hip_mesh_handle m, const mmc_hip_sim* 
                                       const mmc_hip_tune* tune, mmc_hip_result* res) {
     if (!arena || !fieldoff) return set_err(-1, "null arena");
     return launch_batch(m, sims, nsim, tune, res, arena, fieldoff);
+}
+
+/* ---- allocation-free launch ------------------------------------------------
+ * mmc_hip_run_chunks without device allocation: the per-launch buffers
+ * (simulation tables, energy tallies, counters, work queue, and detector,
+ * photon, trajectory and replay buffers when asked for) are windows of
+ * `scratch`, a file-scope device array of MMC_NOALLOC_SCRATCH bytes of which
+ * each device holds its own copy until the process ends.  Its address on
+ * each device is looked up once, with that device current, and only that
+ * address is handed to copies and kernels.  Launches go to the null stream;
+ * calls on one device are serialized here, and launches of other callers on
+ * the same device must not overlap them (they would share the array). */
+static __device__ __align__(256) unsigned char scratch[MMC_NOALLOC_SCRATCH];
+static void* g_scratch_ptr[64];
+static mmc_hip_slotres g_noalloc_slot[64];
+static std::mutex g_noalloc_mu[64];
+
+extern "C" int32_t mmc_hip_run_chunks_noalloc(mmc_hip_mesh_handle m, const mmc_hip_sim* sims, int32_t nsim,
+                                              mmc_hip_arena_handle arena, const uint64_t* fieldoff,
+                                              const mmc_hip_tune* tune, mmc_hip_result* res) {
+    if (!arena || !fieldoff) return set_err(-1, "null arena");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err(-1, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[dev]);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_err(-1, "scratch: device address lookup failed (%s)", hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    mmc_hip_slotres* sl = &g_noalloc_slot[dev];
+    sl->bump_base = (unsigned char*)g_scratch_ptr[dev];
+    sl->bump_cap = MMC_NOALLOC_SCRATCH;
+    int32_t rc = slot_enqueue(sl, m, sims, nsim, tune, res, arena, fieldoff);
+    if (!rc) rc = slot_finish(sl, res);
+    return rc;
 }
 
 extern "C" int32_t mmc_hip_slot_create(mmc_hip_slot_handle* out) {
