@@ -37,13 +37,15 @@ def log(*a):
 
 
 def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None):
-    """Time the CPU oracle (OpenMP restatement of mmc_run_mp) on a bounded
-    photon sample of the same workload."""
+    """Time the CPU oracle (OpenMP restatement of mmc_run_mp, built -O3
+    -msse4.1 like the reference's sse target) on a bounded photon sample of
+    the same workload, with every core this process may run on."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as O
     node, elem, et, med = mesh_arrays
     om = O.OracleMesh(node, elem, et, med, nodal=False)
-    threads = threads or min(os.cpu_count() or 1, 16)
+    avail = len(os.sched_getaffinity(0))
+    threads = threads or avail
     import copy
     s = copy.copy(sim)
     done, ms = 0, 0.0
@@ -56,8 +58,11 @@ def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None):
         done += chunk
         ms += r.runtime_ms
     return {"value": done / ms, "unit": "photons/ms", "cores": threads, "kind": "port",
-            "sample": f"{done} photons of the same workload (glibc-libm oracle, {threads} OpenMP threads, "
-                      f"{ms / 1000:.1f} s)"}
+            "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+            "sample": f"{done} photons of the same workload (glibc-libm oracle, {threads} OpenMP threads on "
+                      f"{avail} available of {os.cpu_count()} host CPUs, {ms / 1000:.1f} s)",
+            "calibration": "none: the reference itself cannot be compiled here (its mmc_utils.c needs the "
+                           "generated mmc_core.clh), so no reference/oracle speed ratio exists"}
 
 
 def bench_plain(args):
@@ -152,11 +157,13 @@ def bench_batch(args):
             "autotune": {k: best[k] for k in ("block", "blocks_per_cu", "chunk")}, "autotune_s": tune_s}
 
 
-def bench_serve(args):
+def bench_serve(args, deadline):
     """The serving workload: Poisson arrivals (200 requests/s) over three
     resident meshes (60 mm cube 162k tets, slab 3.0M tets, jittered box
     5.2M tets), served by the continuous-batching loop and, for comparison,
-    by the plain drop-in one request at a time."""
+    a sample of the same requests by the plain drop-in one request at a
+    time.  The stream is cut to the requests whose photons fit the time left
+    before `deadline` (at the plain slab rate), which the line states."""
     import mmc_amd as M
     from mmc_amd import meshgen
     from mmc_amd import serve as S
@@ -166,17 +173,38 @@ def bench_serve(args):
               "jitter": M.TetMesh(*meshgen.jittered_box(120, 120, 60, 1.0, amp=0.2)).prepare(nodal=False)}
     log(f"serving meshes ready in {time.time() - t:.1f} s: " + ", ".join(f"{k} {m.ne}" for k, m in meshes.items()))
     hi = args.serve_max_photons
-    reqs = S.poisson_requests(meshes, args.serve_requests, rate=200.0, photons=(1e5, hi))
+    asked = args.serve_requests
+    reqs = S.poisson_requests(meshes, asked, rate=200.0, photons=(1e5, hi))
+    offered = sum(r.cfg.nphoton for r in reqs) / (reqs[-1].arrival * 1e3)      # photons/ms asked for
+    # photons that fit: served once, plus the plain sample; 0.7 of the plain slab rate
+    rate = 0.7 * args.plain_rate
+    left_ms = max(0.0, (deadline - time.time()) * 1e3)
+    every = max(1, args.serve_plain_every)
+    n, cum = 0, 0.0
+    for r in reqs:
+        extra = r.cfg.nphoton * (1.0 + (1.0 / every))
+        if (cum + extra) / rate > left_ms:
+            break
+        cum += extra
+        n += 1
+    n = max(n, min(8, asked))
+    reqs = reqs[:n]
+    log(f"serving {n} of {asked} requests ({sum(r.cfg.nphoton for r in reqs):.3g} photons; offered load "
+        f"{offered:.3g} photons/ms)")
     srv = S.Server(meshes, arena_bytes=16 << 30)
     served = srv.serve(reqs)
     srv.close()
     log(f"served: {served}")
-    plain = S.serve_plain(meshes, S.poisson_requests(meshes, args.serve_requests, rate=200.0, photons=(1e5, hi)))
-    log(f"plain: {plain}")
-    return {"workload": f"{args.serve_requests} requests, Poisson 200/s, meshes cube 162k / slab 3.0M / "
-                        f"jittered 5.2M tets, photons log-uniform [1e5, {hi:.0e}], 1-50 gates, per-tissue media "
-                        "as the batch workload, element fluence",
-            "served": served, "plain": plain,
+    sample = S.poisson_requests(meshes, asked, rate=200.0, photons=(1e5, hi))[:n][::every]
+    plain = S.serve_plain(meshes, sample)
+    log(f"plain ({len(sample)} requests): {plain}")
+    return {"workload": f"Poisson 200 requests/s, meshes cube 162k / slab 3.0M / jittered 5.2M tets, photons "
+                        f"log-uniform [1e5, {hi:.0e}], 1-50 gates, per-tissue media as the batch workload, "
+                        "element fluence",
+            "requests_asked": asked, "requests": n,
+            "offered_photons_per_ms": offered, "capacity_photons_per_ms": served["photons_per_ms"],
+            "served": served,
+            "plain_sample": {"every": every, **plain},
             "speedup": served["photons_per_ms"] / plain["photons_per_ms"]}
 
 
@@ -248,14 +276,20 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=100_000_000)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "all", "large"])
+    ap.add_argument("--mode", default="all", choices=["plain", "batch", "serve", "all", "large"],
+                    help="all (default): the plain slab steps, then the 256-simulation batch and the "
+                         "served request stream")
     ap.add_argument("--large-photons", type=float, default=1e9)
     ap.add_argument("--large-slice", type=float, default=2.5e8)
-    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-plain-every", type=int, default=5,
+                    help="the plain drop-in runs every n-th request of the served stream")
     ap.add_argument("--serve-max-photons", type=float, default=1e8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-sims", type=int, default=16)
+    ap.add_argument("--batch-sims", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--time-budget", type=float, default=480.0,
+                    help="seconds for the whole run; the served stream is cut to fit")
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes device-to-device")
     args = ap.parse_args()
     if args.copy_probe:
@@ -266,11 +300,13 @@ def main():
               "with disjoint seed ranges and one reduction of the fluence, which is not run here",
               file=sys.stderr)
         sys.exit(2)
+    deadline = time.time() + args.time_budget - (0 if args.no_cpu else args.cpu_seconds + 10)
     out, arrays, sim = bench_plain(args)
+    args.plain_rate = out["value"]
     if args.mode in ("batch", "all"):
         out["batched"] = bench_batch(args)
     if args.mode in ("serve", "all"):
-        out["serving"] = bench_serve(args)
+        out["serving"] = bench_serve(args, deadline)
     if args.mode == "large":
         out["large"] = bench_large(args)
     if not args.no_cpu:
