@@ -30,20 +30,23 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_STEP_ELEM = 92.0   # 64 B face planes + 16 B face neighbours + 4 B label + 8 B FP32 atomic RMW
+# Plucker nodal step: 48 B node coordinates + 16 B node ids + 16 B neighbours
+# + 4 x 8 B nodal FP32 atomic RMW + 16 B next-element ids
+BYTES_PER_STEP_NODAL = 128.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None):
+def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None, nodal=False):
     """Time the CPU oracle (OpenMP restatement of mmc_run_mp, built -O3
     -msse4.1 like the reference's sse target) on a bounded photon sample of
     the same workload, with every core this process may run on."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as O
     node, elem, et, med = mesh_arrays
-    om = O.OracleMesh(node, elem, et, med, nodal=False)
+    om = O.OracleMesh(node, elem, et, med, nodal=nodal)
     avail = len(os.sched_getaffinity(0))
     threads = threads or avail
     import copy
@@ -66,15 +69,20 @@ def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None):
 
 
 def bench_plain(args):
+    """The plain drop-in path on a device-resident mesh: the slab workload
+    (default) or, with --workload cube, the 60 mm cube of 2 mm cells with
+    Plucker nodal fluence (for profiling that kernel)."""
     import mmc_amd as M
     from mmc_amd import workloads as W
     from mmc_amd import meshgen
     t = time.time()
-    arrays = meshgen.slab_layers(1.0)
-    mesh = M.TetMesh(*arrays).prepare(nodal=False)
+    cube = args.workload == "cube"
+    arrays = meshgen.cube60(2.0) if cube else meshgen.slab_layers(1.0)
+    mesh = M.TetMesh(*arrays).prepare(nodal=cube)
     res = M.ResidentMesh(mesh)
-    log(f"slab mesh: {mesh.ne} tets, prep+upload {time.time() - t:.1f} s")
-    sim = W.slab_sim(args.photons)
+    log(f"{args.workload} mesh: {mesh.ne} tets, prep+upload {time.time() - t:.1f} s")
+    sim = W.cube60_sim(args.photons) if cube else W.slab_sim(args.photons)
+    bytes_per_step = BYTES_PER_STEP_NODAL if cube else BYTES_PER_STEP_ELEM
     for i in range(args.warmup):
         r = res.run_batch([sim], want_field=False)[0]
         log(f"warmup {i}: {r.kernel_ms:.1f} ms kernel, {r.total_ms:.1f} ms total")
@@ -94,26 +102,43 @@ def bench_plain(args):
     value = photons / total_ms
     k_avg = float(np.mean(kern_ms))
     steps_per_launch = raytet / args.steps
-    achieved = steps_per_launch * BYTES_PER_STEP_ELEM / (k_avg * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(HERE, "profiles", "transport_slab_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f)["bytes_per_step"] * steps_per_launch   # measured HBM bytes per step
+    achieved = steps_per_launch * bytes_per_step / (k_avg * 1e-3) / 1e9
+    # counters of the same kernel and workload (rocprofv3 --pmc summary in profiles/)
+    counters = None
+    cfile = os.path.join(HERE, "profiles", f"transport_{args.workload}_counters.json")
+    if os.path.exists(cfile):
+        with open(cfile) as f:
+            counters = json.load(f)
+    traffic = counters["hbm_bytes_per_step"] * steps_per_launch if counters else None
+    kname = "mmc::transport<1, false, 4, 1>" if cube else "mmc::transport<0, false, 4, 1>"
+    if cube:
+        workload = (f"cube: 60 mm, 2 mm cells x6 = {mesh.ne} tets, mua 0.005, mus 1.0, g 0.01, n 1.37, pencil at "
+                    f"(30,30,0), {args.photons:.0e} photons/step, 1 gate 0-5 ns, nodal fluence (Plucker)")
+    else:
+        workload = ("slab: 100x100x50 mm, 1 mm cells x6 = 3.0M tets, 4 layers + 10 mm sphere, "
+                    f"pencil at (50,50,0), {args.photons:.0e} photons/step, 50 gates x 0.1 ns, "
+                    "element fluence (branch-less Badouel), 16 disk detectors")
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+            "kernel": kname, "kernel_ms": k_avg,
+            "bytes_per_step": bytes_per_step, "steps_per_launch": steps_per_launch}
+    if counters:
+        # the kernel issues vector instructions nearly every cycle: the
+        # HBM fraction above is the algorithmic-bytes view, the limiter is
+        # VALU issue (figures from the committed counter summary)
+        roof["limiter"] = "valu-issue"
+        roof["counters"] = {k: counters[k] for k in ("source", "valu_issue_frac", "active_lane_frac",
+                                                    "valu_per_wave_step", "l2_hit_frac", "hbm_bytes_per_step")
+                            if k in counters}
     out = {
         "metric": "photons/ms on 1 GPU (plain vs batched/served) + p50/p99 solve latency",
         "value": value, "unit": "photons/ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": total_ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "slab: 100x100x50 mm, 1 mm cells x6 = 3.0M tets, 4 layers + 10 mm sphere, "
-                               f"pencil at (50,50,0), {args.photons:.0e} photons/step, 50 gates x 0.1 ns, "
-                               "element fluence (branch-less Badouel), 16 disk detectors",
-                   "photons_per_step": args.photons, "tets": int(mesh.ne), "gates": 50,
+        "config": {"workload": workload,
+                   "photons_per_step": args.photons, "tets": int(mesh.ne), "gates": sim.gates()[0],
                    "steps_per_photon": raytet / photons, "ray_tet_steps_per_ms": raytet / sum(kern_ms)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "mmc::transport<0, false, 4, 1>", "kernel_ms": k_avg,
-                     "bytes_per_step": BYTES_PER_STEP_ELEM, "steps_per_launch": steps_per_launch},
+        "roofline": roof,
     }
     res.release()
     return out, arrays, sim
@@ -276,6 +301,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=100_000_000)
+    ap.add_argument("--workload", default="slab", choices=["slab", "cube"],
+                    help="plain-path workload: the slab (default) or the 60 mm cube (Plucker nodal)")
     ap.add_argument("--mode", default="all", choices=["plain", "batch", "serve", "all", "large"],
                     help="all (default): the plain slab steps, then the 256-simulation batch and the "
                          "served request stream")
@@ -310,7 +337,7 @@ def main():
     if args.mode == "large":
         out["large"] = bench_large(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds, nodal=args.workload == "cube")
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
