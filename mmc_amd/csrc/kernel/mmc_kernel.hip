@@ -848,9 +848,13 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     if (a.savedet)
                         for (int k = 0; k < (2 + a.mom) * a.prop; k++) *LS(ppbase + k) = 0.f;
                     /* replay with weighted path / scattering outputs counts the
-                     * detected weight as launched energy (src/mmc_raytrace.c:1815) */
+                     * detected weight as launched energy (src/mmc_raytrace.c:1815);
+                     * a pattern source first scales it by the launch pixel's
+                     * pattern value (src/mmc_raytrace.c:2394-2399) */
                     {
                         const int rmode = (hot_of().flags >> kReplayShift) & 3;
+                        if ((rmode == kReplayWL || rmode == kReplayWP) && C.srctype == MMC_SRC_PATTERN)
+                            rw *= a.pattern[C.patbase + posidx];
                         kahan_lds(LS(0), LS(1), (rmode == kReplayWL || rmode == kReplayWP) ? rw : w);
                     }
                     if (eid == ID_UNDEF) ctx = -1;   /* ends below without tracing */

@@ -408,11 +408,17 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
         /* the dual grid always deposits per voxel (src/mmc_utils.c:3554-3556) */
         if (!(s->gridstep > 0.f)) return set_err(-1, "dual grid needs a positive voxel edge (gridstep)");
         const SimGrid g = grid_of(m, s);
-        if (!g.ncells || (uint64_t)g.ncells * (uint64_t)s->maxgate >= (1ull << 32))
+        if (!g.ncells || (uint64_t)g.ncells * (uint64_t)s->maxgate >= (1ull << 31))
             return set_err(-1, "dual grid of %g-unit voxels too large (%d x %d x %d voxels, %d gates)",
                            (double)s->gridstep, g.nx, g.ny, g.nz, s->maxgate);
     } else if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1)) {
         return set_err(-1, "supported pairs: Plucker with nodal output, branch-less Badouel with element output");
+    } else {
+        /* the kernel indexes a pattern's field (gate x element or node) in 32 bits */
+        const uint64_t rows = s->basisorder == 1 ? (uint64_t)m->nn : (uint64_t)m->ne;
+        if (s->maxgate > 0 && rows * (uint64_t)s->maxgate >= (1ull << 31))
+            return set_err(-1, "field of %llu %s x %d gates exceeds 2^31 values", (unsigned long long)rows,
+                           s->basisorder == 1 ? "nodes" : "elements", s->maxgate);
     }
     if (s->srctype < MMC_SRC_PENCIL || s->srctype > MMC_SRC_RING)
         return set_err(-1, "unsupported source type %d", s->srctype);
@@ -452,6 +458,9 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
             return set_err(-1, "replay outputs need the photons' seeds and detected weights");
         if (s->outputtype != MMC_OUT_JACOBIAN && !s->replaytime)
             return set_err(-1, "weighted path/scattering outputs need the detected times");
+        if (s->outputtype != MMC_OUT_JACOBIAN && slots_of(s) > 1)
+            return set_err(-1, "replay of the weighted path/scattering outputs takes one source pattern "
+                               "(src/mmc_raytrace.c:2396)");
     }
     if (!(s->omega >= 0.f)) return set_err(-1, "omega must be >= 0");
     if (rf_of(s) && (s->method == MMC_RT_PLUCKER || slots_of(s) != 1))
@@ -756,6 +765,17 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
             maxrec = std::max(maxrec, res[i].reclen);
             maxdet += s.maxdetphoton > 0 ? s.maxdetphoton : 0;
         }
+    }
+    if (arena) {
+        /* the simulations of a launch write disjoint arena regions */
+        std::vector<std::pair<uint64_t, uint64_t>> reg(nsim);
+        for (int i = 0; i < nsim; i++)
+            reg[i] = {fieldoff[i], (uint64_t)datalen_of(m, sims + i) * sims[i].maxgate * (rf_of(sims + i) ? 2 : 1)};
+        std::sort(reg.begin(), reg.end());
+        for (int i = 1; i < nsim; i++)
+            if (reg[i - 1].second > 0 && reg[i].second > 0 && reg[i - 1].first + reg[i - 1].second > reg[i].first)
+                return set_err(-1, "arena regions of the launch overlap at offset %llu",
+                               (unsigned long long)reg[i].first);
     }
     const int64_t total = cum[nsim];
     const int reclen = maxrec;

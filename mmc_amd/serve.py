@@ -53,6 +53,7 @@ class Request:
     detected: int = 0
     fieldsum: float = 0.0
     field: Optional[np.ndarray] = None
+    field_im: Optional[np.ndarray] = None      # RF: imaginary part
 
     @property
     def latency(self) -> float:
@@ -118,7 +119,9 @@ class Scheduler:
         return self.nodelen[r.mesh] if r.cfg.basisorder else self.datalen[r.mesh]
 
     def field_len(self, r: Request) -> int:
-        return self.rows(r) * r.cfg.gates()[0] * r.cfg.srcnum
+        """Arena floats of a request: rows x gates x patterns, twice that for
+        RF (the imaginary block follows the real one)."""
+        return self.rows(r) * r.cfg.gates()[0] * r.cfg.srcnum * (2 if r.cfg.is_rf else 1)
 
     def arrive(self, r: Request, mesh: Optional[TetMesh] = None):
         if mesh is not None and r.cfg.e0 <= 0 and r.cfg.srctype == "pencil":
@@ -254,15 +257,18 @@ class Server:
     def _collect(self, r: Request):
         L = _lib.lib()
         s = C.c_double()
-        if r.want_field:
-            shape = (r.cfg.gates()[0], self.sched.rows(r)) + ((r.cfg.srcnum,) if r.cfg.srcnum > 1 else ())
-            r.field = np.zeros(shape, np.float64)
-            host = ptr(r.field, C.c_double)
-        else:
-            host = C.cast(None, f64p)
-        _lib.check(L.mmc_hip_arena_collect(self.arena, C.c_uint64(r.off), C.c_uint64(r.flen), host, C.byref(s)),
-                   "mmc_hip_arena_collect")
-        r.fieldsum = s.value
+        shape = (r.cfg.gates()[0], self.sched.rows(r)) + ((r.cfg.srcnum,) if r.cfg.srcnum > 1 else ())
+        nre = r.flen // 2 if r.cfg.is_rf else r.flen
+        # the real block, then (RF) the imaginary block that follows it
+        for part in range(2 if r.cfg.is_rf else 1):
+            buf = np.zeros(shape, np.float64) if r.want_field else None
+            host = ptr(buf, C.c_double) if r.want_field else C.cast(None, f64p)
+            _lib.check(L.mmc_hip_arena_collect(self.arena, C.c_uint64(r.off + part * nre), C.c_uint64(nre), host,
+                                               C.byref(s)), "mmc_hip_arena_collect")
+            if part == 0:
+                r.field, r.fieldsum = buf, s.value
+            else:
+                r.field_im = buf
 
     def serve(self, requests: List[Request]) -> dict:
         """Run a recorded arrival stream (arrival times in seconds from now)

@@ -1139,6 +1139,11 @@ static int orc_launch(oray* r, octx* c, uint64_t ran[2]) {
                 ix = ix < xsize - 1 ? ix : xsize - 1;
                 r->posidx = (unsigned int)(iy * xsize + ix);
                 r->weight = (cfg->srcnum == 1) ? cfg->srcpattern[r->posidx] : 1.f;
+                /* seed replay of the weighted path / scattering outputs:
+                 * the photon's detected weight takes its launch pixel's
+                 * pattern value (src/mmc_raytrace.c:2394-2399) */
+                if (cfg->replayweight && (cfg->outputtype == ORC_OT_WL || cfg->outputtype == ORC_OT_WP))
+                    c->rw *= cfg->srcpattern[r->posidx];
             } else if (st == ORC_ST_FOURIER) {
                 float sn, cs;
                 O_SINCOSF_C((float)((floorf(P1[3]) * rx + floorf(P2[3]) * ry + P1[3] - floorf(P1[3])) * TWO_PI),

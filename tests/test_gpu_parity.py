@@ -307,6 +307,37 @@ def test_replay_outputs_match_the_oracle(slab, out):
     assert abs(gr.normalizer / o.normalizer - 1) < 1e-6
 
 
+@pytest.mark.parametrize("out", ["l", "p"])
+def test_pattern_replay_outputs_match_the_oracle(out):
+    # weighted path / scattering outputs of a pattern source: each replayed
+    # photon's weight takes its launch pixel's pattern value
+    # (src/mmc_raytrace.c:2394-2399), on the GPU as in the oracle
+    from mmc_amd import replay as R
+    arrays = M.meshgen.slab_layers(4.0)
+    c = M.meshgen.centroids(arrays[0], arrays[1])
+    srcelem = (np.where(c[:, 2] < 8.0)[0] + 1).astype(np.int32)
+    tm = M.TetMesh(*arrays, srcelem=srcelem).prepare(nodal=False)
+    om = O.OracleMesh(*arrays, nodal=False, srcelem=srcelem)
+    cfg = W.slab_sim(30000, seed=41)
+    cfg.srctype, cfg.srcpos = "pattern", (30.3, 30.7, 0.0)
+    cfg.srcparam1, cfg.srcparam2 = (40.0, 0.0, 0.0, 8), (0.0, 40.0, 0.0, 8)
+    cfg.srcpattern = np.random.default_rng(8).uniform(0.2, 1.0, (8, 8)).astype(np.float32)
+    g = M.run(tm, cfg)
+    assert g.detected > 100
+    seeds_ = R.detected_seeds(cfg, g)
+    mch = R.MchFile(len(arrays[3]) - 1, len(cfg.detpos), g.detdata.shape[1], cfg.nphoton, g.detected, 1.0, 1.0,
+                    g.detdata, seeds_)
+    rcfg = R.replay_config(cfg, mch, arrays[3], out)
+    rcfg.issavedet = 0
+    gr = M.run(tm, rcfg, tune=M.sim.tune_struct(compensated=1), normalize=True)
+    o = O.run(om, rcfg, photonseed=rcfg.photonseed, math="portable", normalize=True)
+    # launched energy: the detected weights times their pixels' pattern values
+    w0 = g.detdata[:, -1].astype(np.float64)
+    assert gr.launchweight == pytest.approx(float((rcfg.replayweight.astype(np.float64) * w0).sum()), rel=1e-5)
+    assert gr.launchweight == pytest.approx(o.launchweight, rel=1e-6)
+    assert np.linalg.norm(gr.field - o.field) <= 1e-5 * np.linalg.norm(o.field)
+
+
 def test_invalid_settings_fail_with_a_message(slab):
     # settings the transport cannot honour are refused before any launch,
     # with the reason in mmc_hip_last_error (no silent fallback)

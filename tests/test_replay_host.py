@@ -98,3 +98,34 @@ def test_oracle_weighted_path_and_scattering_outputs(detected):
     # each scattering closes a segment whose fractions L*mus/slen0 add to 1;
     # steps before the first scattering count 1 each, no step more than 1
     assert nscat * (1 - 1e-4) <= wp.field.sum() <= steps
+
+
+def test_oracle_pattern_replay_weights_take_the_launch_pixel():
+    """Seed replay of the weighted path output with a pattern source: each
+    detected photon's replay weight is scaled by its launch pixel's pattern
+    value (src/mmc_raytrace.c:2394-2399), so the L output sums to
+    sum(rw * pattern(pixel) * path) and the launched energy to
+    sum(rw * pattern(pixel)); the pattern value is the record's launch weight."""
+    arrays = M.meshgen.slab_layers(4.0)
+    c = M.meshgen.centroids(arrays[0], arrays[1])
+    srcelem = (np.where(c[:, 2] < 8.0)[0] + 1).astype(np.int32)      # launch candidates
+    om = O.OracleMesh(*arrays, nodal=False, srcelem=srcelem)
+    pat = np.random.default_rng(8).uniform(0.2, 1.0, (8, 8)).astype(np.float32)
+    cfg = W.slab_sim(30000, seed=41)
+    cfg.srctype, cfg.srcpos = "pattern", (30.3, 30.7, 0.0)
+    cfg.srcparam1, cfg.srcparam2, cfg.srcpattern = (40.0, 0.0, 0.0, 8), (0.0, 40.0, 0.0, 8), pat
+    seeds = M.photon_seeds(cfg.seed, 0, cfg.nphoton)
+    o = O.run(om, cfg, photonseed=seeds, records=(0, cfg.nphoton), maxdet=cfg.nphoton, math="portable")
+    assert o.detected > 100
+    ids = np.sort(o.detphoton)
+    prop = len(arrays[3]) - 1
+    rec = o.detdata[np.argsort(o.detphoton)]
+    ppath = rec[:, 1 + prop:1 + 2 * prop].astype(np.float64).sum(axis=1)
+    w0 = rec[:, -1].astype(np.float64)                      # the launch pixel's pattern value
+    assert len(np.unique(w0)) > 5 and np.all(np.isin(w0.astype(np.float32), pat))
+    rw = np.random.default_rng(3).uniform(0.5, 2.0, len(ids)).astype(np.float32)
+    base = {**cfg.__dict__, "nphoton": len(ids), "issavedet": 0, "minenergy": 0.0, "outputtype": "l",
+            "replayweight": rw, "replaytime": np.full(len(ids), 1e-10, np.float32)}
+    wl = O.run(om, M.SimConfig(**base), photonseed=seeds[ids], math="portable")
+    assert abs(wl.field.sum() / (rw * w0 * ppath).sum() - 1) < 1e-5
+    assert wl.launchweight == pytest.approx(float((rw.astype(np.float64) * w0).sum()), rel=1e-6)

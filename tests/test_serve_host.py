@@ -76,3 +76,18 @@ def test_plan_skips_requests_whose_photons_are_all_in_flight():
     mesh, chunks = s.plan()
     assert mesh == "b" and [(r.rid, n) for r, n in chunks] == [(1, 1000)]
     assert r0 in s.active             # completes only when its launch finishes
+
+
+def test_rf_request_reserves_its_imaginary_block():
+    """An RF request (omega > 0) writes its imaginary part right after the
+    real one, so it reserves twice the floats; the request admitted next to
+    it starts after both blocks."""
+    s = Scheduler(arena_floats=10_000, datalen={"a": 1000}, slice_photons=1000, min_chunk=100)
+    rf = mkreq(0, "a", 2, 500, 0.0)
+    rf.cfg.omega = 1e8
+    plain = mkreq(1, "a", 2, 500, 0.1)
+    s.arrive(rf)
+    s.arrive(plain)
+    assert rf.flen == 2 * plain.flen == 4000
+    s.admit(0.2)
+    assert plain.off >= rf.off + rf.flen
