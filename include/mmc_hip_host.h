@@ -1,5 +1,5 @@
 /*
- * mmc_host.h -- host-side mesh preparation, normalization and output writers
+ * mmc_hip_host.h -- host-side mesh preparation, normalization and output writers
  * of the MI355X photon tracer (part of libmmc_hip.so; no GPU calls).
  *
  * These restate the reference's mmc_prep() chain (src/mmc_host.c:136-165):
@@ -10,8 +10,8 @@
  * plus the writers mesh_saveweight (src/mmc_mesh.c:1726-1800), mcx_savedata
  * (src/mmc_utils.c:919-993) and mcx_savedetphoton (src/mmc_utils.c:4309-4342).
  */
-#ifndef MMC_HOST_H
-#define MMC_HOST_H
+#ifndef MMC_HIP_HOST_H
+#define MMC_HIP_HOST_H
 
 #include <stdint.h>
 #include "mmc_hip.h"

@@ -1,5 +1,5 @@
 """ctypes binding of libmmc_hip.so (the C ABI declared in include/mmc_hip.h and
-include/mmc_host.h).
+include/mmc_hip_host.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` into
 ``mmc_amd/lib/libmmc_hip.so``.  Loading it fails loudly when it is missing:

@@ -4,7 +4,7 @@
  * Reads the reference's JSON input (Session / Forward / Domain / Mesh /
  * Optode, src/mmc_utils.c:1566-2392) or its mesh .dat files, honours the
  * photon-transport subset of its flags (src/mmc_utils.c:103-128,3814-4307),
- * prepares the mesh (include/mmc_host.h), runs the HIP transport through the
+ * prepares the mesh (include/mmc_hip_host.h), runs the HIP transport through the
  * C ABI (include/mmc_hip.h) and writes the reference's outputs:
  * <session>.dat (ascii, "%d\t%e" per node/element per gate, src/mmc_mesh.c:
  * 1726-1800), or .bin/.nii/.hdr+.img/.mc2/.tx3/.jnii/.bnii (src/mmc_utils.c:
@@ -20,7 +20,7 @@
 #include <time.h>
 
 #include "mmc_hip.h"
-#include "mmc_host.h"
+#include "mmc_hip_host.h"
 #include "mmc_json.h"
 
 #define MAXDET 256

@@ -41,7 +41,7 @@
 #include <string.h>
 
 #include "mmc_fp32math.h"
-#include "mmc_host.h"
+#include "mmc_hip_host.h"
 
 #define NK (1u << 23)
 #define NTHREAD 8

@@ -26,7 +26,7 @@
 
 #include <zlib.h>
 
-#include "mmc_host.h"
+#include "mmc_hip_host.h"
 
 /* ------------------------------------------------------------ NIfTI-1 ---- */
 /* The 348-byte NIfTI-1 header (public format, nifti.nimh.nih.gov). */

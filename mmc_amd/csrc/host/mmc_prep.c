@@ -7,7 +7,7 @@
  * device kernels are compared photon by photon against the reference
  * semantics; compile with -ffp-contract=off.
  */
-#include "mmc_host.h"
+#include "mmc_hip_host.h"
 
 #include <math.h>
 #include <stdio.h>

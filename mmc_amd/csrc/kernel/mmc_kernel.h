@@ -9,7 +9,7 @@
 #include <stdint.h>
 
 #include "mmc_hip.h"
-#include "mmc_host.h"
+#include "mmc_hip_host.h"
 
 namespace mmc {
 

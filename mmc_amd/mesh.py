@@ -1,7 +1,7 @@
 """Tetrahedral mesh and its preparation for transport.
 
 ``TetMesh.prepare`` runs the host-side chain of the reference's mmc_prep()
-(src/mmc_host.c:136-165) through the C functions of include/mmc_host.h:
+(src/mmc_host.c:136-165) through the C functions of include/mmc_hip_host.h:
 orientation fix and volumes (mesh_getvolume), face neighbours
 (mesh_getfacenb), the surface-node volume correction and exterior-face
 numbering (tracer_prep) and the face planes (tracer_build).

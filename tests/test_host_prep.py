@@ -1,4 +1,4 @@
-"""The product's host-side mesh preparation (include/mmc_host.h) against the
+"""The product's host-side mesh preparation (include/mmc_hip_host.h) against the
 oracle's restatement and the reference's fixtures, plus the C ABI surface."""
 import ctypes as C
 import os
@@ -92,7 +92,7 @@ def declared_functions(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header", ["mmc_hip.h", "mmc_host.h"])
+@pytest.mark.parametrize("header", ["mmc_hip.h", "mmc_hip_host.h"])
 def test_library_exports_every_declared_function(header):
     path = os.path.join(ROOT, "include", header)
     names = declared_functions(path)
