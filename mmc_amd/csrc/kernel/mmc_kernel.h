@@ -14,7 +14,8 @@
 namespace mmc {
 
 constexpr int kBlock = 256;          /* default threads per workgroup (4 waves) */
-constexpr int kMaxProp = 15;         /* media labels 1..kMaxProp per mesh */
+constexpr int kMaxProp = 4000;       /* media labels 1..kMaxProp per mesh (MAX_PROP, src/mmc_const.h:43) */
+constexpr int kPackedProp = 15;      /* labels that fit the 4 low bits of a packed face neighbour */
 constexpr int kLaneSlots = 8;        /* per-lane LDS accumulators (energy, Kahan terms) */
 constexpr int kMaxSrc = 8;           /* patterns shared by one photon (srcnum) */
 
@@ -77,7 +78,9 @@ constexpr int kAccStride = kAccCount + 2 * kMaxSrc;
 struct KArgs {
     /* mesh, structure of arrays */
     const float4* planes;            /* ne*4: nx, ny, nz, d */
-    const int4* facenb;              /* ne */
+    const int4* facenb;              /* ne: (neighbour << lblshift) | its label when lblshift = 4 */
+    int32_t lblshift;                /* 4: labels packed into facenb; 0: plain neighbour ids, labels
+                                        read from type (more than 15 labels, or 2^27+ elements) */
     const int4* elem;                /* ne, 1-based */
     const int32_t* type;             /* ne */
     const float4* node;              /* nn, w unused */

@@ -62,9 +62,9 @@ __device__ __forceinline__ float pick4a(const float* v, int i) {
 }
 /* entry i (0..3) of a table packed as four nibbles, most significant first */
 __device__ __forceinline__ int nib4(int packed, int i) { return (packed >> (4 * (3 - i))) & 0xF; }
-/* packed face neighbour: (element << 4) | media label of that element */
-__device__ __forceinline__ int nb_id(int packed) { return packed >> 4; }
-__device__ __forceinline__ int nb_type(int packed) { return packed & 15; }
+/* face neighbour: (element << 4) | media label of that element (lblshift
+ * 4), or the element id alone (lblshift 0; the label is read from type) */
+__device__ __forceinline__ int nb_id(int packed, int lblshift) { return packed >> lblshift; }
 
 /* ---- xorshift128+ (src/mmc_rand_xorshift128p.c:55-70) ------------------- */
 __device__ __forceinline__ float rng_next(uint64_t& s0r, uint64_t& s1r) {
@@ -1220,9 +1220,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         if (!isend && npend) flush_nodal();
                         if (isend) {
                             b0 = bo[0]; b1 = bo[1]; b2 = bo[2]; b3 = bo[3];
-                        } else if (nb_id(nexteid) > 0 && faceidx < 4) {
+                        } else if (nb_id(nexteid, a.lblshift) > 0 && faceidx < 4) {
                             /* carry the exit-face barycentrics into the next element */
-                            int4 ne4 = a.elem[nb_id(nexteid) - 1];
+                            int4 ne4 = a.elem[nb_id(nexteid, a.lblshift) - 1];
                             float nbv[4] = {0.f, 0.f, 0.f, 0.f};
                             #pragma unroll
                             for (int j = 0; j < 3; j++) {
@@ -1294,8 +1294,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 int oldeid = eid;
                 int oldtype = ctype;
                 int nbp = nexteid;
-                int neid = nb_id(nbp);
-                int ntype = neid > 0 ? nb_type(nbp) : 0;
+                int neid = nb_id(nbp, a.lblshift);
+                int ntype = neid > 0 ? (a.lblshift ? (nbp & 15) : a.type[neid - 1]) : 0;
                 float nold = med(oldtype).w;
                 if (isreflect && (neid <= 0 || med(ntype).w != nold)) {
                     if (!(neid <= 0 && ((nold == hot.nout && isreflect != MMC_BC_MIRROR) ||

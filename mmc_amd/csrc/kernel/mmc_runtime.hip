@@ -50,6 +50,7 @@ int mmc::report_error(int code, const char* msg) { return set_err(code, "%s", ms
 struct mmc_hip_meshres {
     int nn, ne, prop, nf;
     int isextdet = 0;             /* elements labelled prop+1 are wide-field detectors */
+    int lblshift = 4;             /* facenb packing: 4 (labels in the low bits) or 0 */
     float4* planes = nullptr;
     int4* facenb = nullptr;
     int4* elem = nullptr;
@@ -236,8 +237,11 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
     if (mesh->isextdet != 0 && mesh->isextdet != 1) return set_err(-1, "isextdet must be 0 or 1");
     if (mesh->prop < 1 || mesh->prop + mesh->isextdet > kMaxProp)
         return set_err(-1, "media count %d (+%d detector label) outside 1..%d", mesh->prop, mesh->isextdet, kMaxProp);
-    if ((int64_t)mesh->ne >= (1ll << 27) || (int64_t)mesh->nf >= (1ll << 27))
-        return set_err(-1, "mesh too large for packed face neighbours (ne=%d)", mesh->ne);
+    /* the label of the next element travels in the 4 low bits of the packed
+     * face neighbour while labels fit and ids leave room; otherwise the
+     * neighbour ids are stored as they are and labels come from type */
+    const bool packed = mesh->prop + mesh->isextdet <= kPackedProp && (int64_t)mesh->ne < (1ll << 27) &&
+                        (int64_t)mesh->nf < (1ll << 27);
     const size_t ne = (size_t)mesh->ne, nn = (size_t)mesh->nn;
     /* validate ids on the host before anything reaches the device */
     for (size_t i = 0; i < 4 * ne; i++) {
@@ -256,6 +260,7 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
     m->prop = mesh->prop;
     m->nf = mesh->nf;
     m->isextdet = mesh->isextdet;
+    m->lblshift = packed ? 4 : 0;
     m->med.assign(mesh->med, mesh->med + 4 * (mesh->prop + 1));
     /* the dual grid's extent (mesh_createdualmesh, src/mmc_mesh.c:349-372):
      * the node bounding box widened by EPS; every grid starts at its corner */
@@ -277,7 +282,7 @@ extern "C" int32_t mmc_hip_mesh_upload(const mmc_hip_mesh* mesh, mmc_hip_mesh_ha
         for (int j = 0; j < 4; j++) {
             int nb = mesh->facenb[4 * i + j];
             int t = nb > 0 ? mesh->type[nb - 1] : 0;
-            v[j] = (int)((unsigned)nb << 4) | t;
+            v[j] = packed ? ((int)((unsigned)nb << 4) | t) : nb;
         }
         fnb[i] = make_int4(v[0], v[1], v[2], v[3]);
     }
@@ -630,7 +635,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     /* media rows 0..prop, then the detector label's row: a copy of the
      * background (src/mmc_mesh.c:518-520) */
     const int nmed = m->prop + 1 + m->isextdet;
-    const int block = (tune && tune->block > 0) ? tune->block : kBlock;
+    int block = (tune && tune->block > 0) ? tune->block : kBlock;
     if (block % 64 || block > kBlock) return set_err(-1, "block must be a multiple of 64 and <= %d", kBlock);
     const bool comp = arena ? (arena->lo != nullptr) : (tune && tune->compensated);
 
@@ -865,6 +870,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     memset(&a, 0, sizeof(a));
     a.planes = m->planes;
     a.facenb = m->facenb;
+    a.lblshift = m->lblshift;
     a.elem = m->elem;
     a.type = m->type;
     a.node = m->node;
@@ -923,9 +929,21 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
      * large batches keep 4+ workgroups per CU */
     const size_t simtab = (sizeof(SimHot) + sizeof(float4) * (size_t)nmed) * nsim;
     const bool stage = simtab <= 16384;
-    size_t shmem = (stage ? simtab : 0) + sizeof(int64_t) * (nsim + 1) + sizeof(float) * (size_t)nslots * block;
-    shmem = (shmem + 15) & ~(size_t)15;
-    if (shmem > 160 * 1024) return set_err(-1, "batch of %d simulations needs %zu B of LDS", nsim, shmem);
+    /* LDS: the staged tables plus per-lane slots (energy tallies, and with
+     * detectors a scattering count, partial path (and momentum transfer) per
+     * medium); meshes with many media shrink the workgroup to fit, unless
+     * the caller fixed it.  The kernel's static LDS (agreement thresholds)
+     * comes on top. */
+    const size_t kLdsMax = 160 * 1024 - 2048;
+    auto lds_for = [&](int b) {
+        size_t v = (stage ? simtab : 0) + sizeof(int64_t) * (nsim + 1) + sizeof(float) * (size_t)nslots * b;
+        return (v + 15) & ~(size_t)15;
+    };
+    while (lds_for(block) > kLdsMax && block > 64 && !(tune && tune->block > 0)) block /= 2;
+    const size_t shmem = lds_for(block);
+    if (shmem > kLdsMax)
+        return set_err(-1, "batch of %d simulations needs %zu B of LDS (%d media%s)", nsim, shmem, m->prop,
+                       savedet ? " with detector records" : "");
 
     /* the register budget of the kernel variant follows the requested
      * residency: 4 workgroups per CU by default (128 VGPRs), 5 or more select

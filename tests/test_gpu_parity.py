@@ -338,6 +338,34 @@ def test_pattern_replay_outputs_match_the_oracle(out):
     assert np.linalg.norm(gr.field - o.field) <= 1e-5 * np.linalg.norm(o.field)
 
 
+def test_forty_media_labels_match_the_oracle_per_photon():
+    # more labels than the 4 low bits of a packed face neighbour hold: the
+    # neighbour ids are stored unpacked and labels come from the element
+    # types; 40 one-millimetre layers with their own (mua, mus, g, n), so
+    # photons refract and reflect at every layer boundary
+    node, elem = M.meshgen.t6_lattice(30, 30, 40, 1.0)
+    c = M.meshgen.centroids(node, elem)
+    et = (np.floor(c[:, 2]).astype(np.int32) + 1)
+    rng = np.random.default_rng(12)
+    med = np.zeros((41, 4), np.float32)
+    med[0] = [0, 0, 1, 1]
+    for k in range(1, 41):
+        med[k] = [rng.uniform(0.001, 0.05), rng.uniform(2, 10), rng.uniform(0.7, 0.95), rng.uniform(1.33, 1.45)]
+    tm = M.TetMesh(node, elem, et, med).prepare(nodal=False)
+    om = O.OracleMesh(node, elem, et, med, nodal=False)
+    det = np.array([[20.0, 15.0, 0.0, 2.0], [10.0, 15.0, 0.0, 2.0]], np.float32)
+    cfg = M.SimConfig(nphoton=5000, method="S", srcpos=(15.3, 15.7, 0.0), srcdir=(0.0, 0.0, 1.0), tend=5e-9,
+                      tstep=5e-10, issavedet=1, detpos=det, maxdetphoton=5000, seed=77)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), maxdet=cfg.nphoton, math="portable")
+    ok = compare_photons(g.photons, o.photons)
+    assert ok.all()
+    assert g.detected == o.detected > 0
+    gi, oi = np.argsort(g.detphoton), np.argsort(o.detphoton)
+    assert np.array_equal(g.detdata[gi], o.detdata[oi])
+    assert abs(g.absorbweight - o.absorbweight) <= 1e-6 * o.launchweight
+
+
 def test_invalid_settings_fail_with_a_message(slab):
     # settings the transport cannot honour are refused before any launch,
     # with the reason in mmc_hip_last_error (no silent fallback)
