@@ -59,6 +59,15 @@ void mmc_host_surface_nvol(int32_t nn, int32_t ne, const int32_t* elem, const in
  * mesh_srcdetelem + mesh_loadmedia do (src/mmc_mesh.c:390-427, 511-527);
  * returns their count (mmc_hip_mesh.isextdet = count > 0). */
 int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop);
+/* Nodal output of the reference GPU backend under the Badouel tracer
+ * (src/mmc_cl_host.c:973-1024): a quarter of each element value of every
+ * gate and pattern to each of the element's nodes; felem [gate][elem][pattern]
+ * (an element-output run), fnode [gate][node][pattern] (added to).  The
+ * library's own Badouel nodal output (basisorder 1) follows the reference's
+ * CPU path instead: a third of each step's deposit to each node of the exit
+ * face.  0, or -1 for bad arguments. */
+int32_t mmc_host_elem_to_nodes(int32_t ne, int32_t nn, const int32_t* elem, int32_t maxgate, int32_t srcnum,
+                               const double* felem, double* fnode);
 /* Source elements (label -1): writes their 1-based ids in index order to
  * srcelem (if not NULL; ne entries suffice), relabels them 0 and returns their
  * count -- the candidate list of area sources' launch search

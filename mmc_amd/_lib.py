@@ -127,6 +127,7 @@ SIGNATURES = {
                                      C.c_int32, f32p]),
     "mmc_host_number_exterior": (C.c_int32, [C.c_int32, i32p]),
     "mmc_host_detelem": (C.c_int32, [C.c_int32, i32p, C.c_int32]),
+    "mmc_host_elem_to_nodes": (C.c_int32, [C.c_int32, C.c_int32, i32p, C.c_int32, C.c_int32, f64p, f64p]),
     "mmc_host_grid": (C.c_int32, [f32p, C.c_int32, C.c_float, f32p, i32p]),
     "mmc_host_srcelem": (C.c_int32, [C.c_int32, i32p, i32p]),
     "mmc_host_detimage": (C.c_int32, [f32p, f32p, C.c_int64, C.c_int32, C.c_int32, f32p, C.c_float, C.c_float,

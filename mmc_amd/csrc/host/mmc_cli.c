@@ -51,6 +51,8 @@ typedef struct {
     char detpatternfile[1024];        /* .inp: detector pattern weighting the L/P replays */
     char adjoint;              /* -O a|d|u|v|w|q: adjoint Jacobian output, 0 if none */
     int adjointmode;           /* mesh mode: 0 linear-basis integrals, 1 nodal approximation */
+    int nodalquarter;          /* -M S -C 1: element values split in quarters to nodes (the
+                                  reference GPU backend) instead of exit-face thirds (its CPU path) */
     float* pattern;
     size_t npattern;
     int gpuid;
@@ -577,7 +579,7 @@ static void usage(void) {
         "  -P/--replaydet N      replay only the photons of detector N (0: all)\n"
         "  -s/--session NAME     output file prefix\n"
         "  -M/--method [G|P|S]   dual grid (voxel output, default), Plucker (nodal, -C 1) or\n"
-        "                        branch-less Badouel (element, -C 0)\n"
+        "                        branch-less Badouel (element with -C 0, nodal with -C 1)\n"
         "  --gridsize D          dual-grid voxel edge in mesh units (default 1)\n"
         "  -C/--basisorder [0|1] element (0) or nodal (1) fluence\n"
         "  -O/--outputtype [X|F|E|J|L|P|A|D|U|V|W|Q] flux, fluence, energy deposit; replay:\n"
@@ -585,6 +587,9 @@ static void usage(void) {
         "                        adjoint Jacobians (detectors need Dir): mua, D, mus, musp,\n"
         "                        mua and D, mua and musp -> <session>_jmua, _jd, _jmus, _jmusp\n"
         "  --adjointmode [0|1]   mesh-mode J_mua: linear-basis integrals (0) or nodal approx. (1)\n"
+        "  --nodalsplit [face|quarter]  -M S -C 1: a third of each step to the exit face's nodes\n"
+        "                        (the reference CPU path, default) or element values in quarters\n"
+        "                        to their nodes (the reference GPU backend)\n"
         "  -F/--outputformat [ascii|bin|nii|hdr|mc2|tx3|jnii|bnii]\n"
         "  -Z/--zip [zlib|base64]   jnii/bnii payload: zlib-compressed (default) or raw\n"
         "  -b/--reflect [0|1|2|3] boundary: no reflect, Fresnel, absorb exterior, mirror\n"
@@ -782,6 +787,12 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "-C") || !strcmp(a, "--basisorder")) c.sim.basisorder = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "--gridsize")) c.gridstep = (float)atof(opt_value(argc, argv, &i));
         else if (!strcmp(a, "--adjointmode")) c.adjointmode = atoi(opt_value(argc, argv, &i));
+        else if (!strcmp(a, "--nodalsplit")) {
+            const char* v = opt_value(argc, argv, &i);
+            if (!strcmp(v, "quarter")) c.nodalquarter = 1;
+            else if (!strcmp(v, "face")) c.nodalquarter = 0;
+            else die("%s", "--nodalsplit takes face or quarter");
+        }
         else if (!strcmp(a, "-m") || !strcmp(a, "--momentum")) c.sim.ismomentum = atoi(opt_value(argc, argv, &i));
         else if (!strcmp(a, "-O") || !strcmp(a, "--outputtype")) {
             set_outputtype(&c, opt_value(argc, argv, &i)[0]);
@@ -968,7 +979,25 @@ int main(int argc, char** argv) {
     if (isgrid)
         printf("dual grid: %d x %d x %d voxels of %g from (%g, %g, %g)\n", griddim[0], griddim[1], griddim[2],
                (double)c.gridstep, (double)gridmin[0], (double)gridmin[1], (double)gridmin[2]);
+    /* the reference GPU backend's nodal output under the Badouel tracer: an
+     * element-output run, then quarters of each element value to its nodes */
+    const int quarter = c.nodalquarter && c.sim.method == MMC_RT_BLBADOUEL && c.sim.basisorder == 1;
+    double* felem = NULL;
+    if (quarter) {
+        if (isrf) die("%s", "RF fluence is element or grid output (-C 0 or -M G)");
+        felem = (double*)calloc((size_t)c.ne * c.sim.maxgate, sizeof(double));
+        if (!felem) die("%s", "can not allocate the element field");
+        c.sim.basisorder = 0;
+        res.field = felem;
+    }
     if (mmc_hip_run(&mesh, &c.sim, NULL, &res) != 0) die("%s", mmc_hip_last_error());
+    if (quarter) {
+        c.sim.basisorder = 1;
+        if (mmc_host_elem_to_nodes(c.ne, c.nn, elem, c.sim.maxgate, 1, felem, field))
+            die("%s", "element-to-node split failed");
+        free(felem);
+        res.field = field;
+    }
     printf("speed ...\t%.2f photon/ms, %.0f ray-tetrahedron tests (%.2f test/ms)\n",
            (double)c.sim.nphoton / res.total_ms, res.raytet, res.raytet / res.kernel_ms);
     double normalizer = 1.0;

@@ -245,6 +245,26 @@ int32_t mmc_host_srcelem(int32_t ne, int32_t* type, int32_t* srcelem) {
     return n;
 }
 
+/* The reference GPU backend's nodal output under the Badouel tracer
+ * (src/mmc_cl_host.c:973-1024): each element value of every gate and pattern
+ * goes a quarter to each of its four nodes, added into fnode
+ * ([gate][node][pattern]) from felem ([gate][element][pattern]). */
+int32_t mmc_host_elem_to_nodes(int32_t ne, int32_t nn, const int32_t* elem, int32_t maxgate, int32_t srcnum,
+                               const double* felem, double* fnode) {
+    if (ne < 0 || nn < 1 || maxgate < 1 || srcnum < 1 || !elem || !felem || !fnode) return -1;
+    for (int32_t g = 0; g < maxgate; g++)
+        for (int32_t e = 0; e < ne; e++)
+            for (int32_t k = 0; k < srcnum; k++) {
+                const double w = felem[((size_t)g * ne + e) * srcnum + k] * 0.25;
+                for (int j = 0; j < 4; j++) {
+                    const int32_t v = elem[4 * (size_t)e + j];
+                    if (v < 1 || v > nn) return -1;
+                    fnode[((size_t)g * nn + (v - 1)) * srcnum + k] += w;
+                }
+            }
+    return 0;
+}
+
 int32_t mmc_host_detelem(int32_t ne, int32_t* type, int32_t prop) {
     int32_t n = 0;
     for (int32_t i = 0; i < ne; i++)

@@ -154,7 +154,7 @@ __host__ __device__ __forceinline__ void photon_seed(uint64_t seed, uint64_t id,
 /* Tracer and output of a kernel variant: branch-less Badouel with element
  * output, Plucker with nodal output, branch-less Badouel with dual-grid
  * (voxel) output, and the two Badouel outputs with RF (complex) weights. */
-enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2, kTrElemRF = 3, kTrGridRF = 4 };
+enum { kTrElem = 0, kTrNodal = 1, kTrGrid = 2, kTrElemRF = 3, kTrGridRF = 4, kTrElemNodal = 5 };
 /* where a launch keeps its per-simulation tables: read through the caches
  * or staged in LDS */
 enum { kTabGlobal = 0, kTabLds = 1 };

@@ -527,7 +527,10 @@ __device__ __forceinline__ uint32_t xcd_id() {
  */
 template <int TR, bool COMP, int OCC, int TAB>
 __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
-    constexpr bool NODAL = TR == kTrNodal;
+    constexpr bool NODAL = TR == kTrNodal;        /* Plucker tracer, nodal deposits */
+    /* branch-less Badouel tracer with the linear (nodal) basis: a third of
+     * each step's deposit to each node of the face the ray leaves through */
+    constexpr bool BNODE = TR == kTrElemNodal;
     constexpr bool GRID = TR == kTrGrid || TR == kTrGridRF;
     /* RF forward (omega > 0, mmc_core.cl IS_RF): complex photon weight and
      * complex fluence, imaginary part in a second block of the field */
@@ -889,6 +892,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                 if (!NODAL) {
                     const float4* P = a.planes + 4 * (size_t)e;
                     cNX = P[0]; cNY = P[1]; cNZ = P[2]; cDD = P[3];
+                    if (BNODE) cee = a.elem[e];
                 } else {
                     const float4* Q = a.enode + 3 * (size_t)e;
                     cNX = Q[0]; cNY = Q[1]; cNZ = Q[2];
@@ -946,7 +950,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     tof += Lmove * rc;
                     int g = (rmode == kReplayWL || rmode == kReplayWP) ? (int)(rt * hot.rtstep)
                                                                        : (int)((tof - hot.tstart) * hot.rtstep);
-                    const int framelen = GRID ? a.grid[sidx()].ncells : a.ne;
+                    const int framelen = GRID ? a.grid[sidx()].ncells : (BNODE ? a.nn : a.ne);
                     int tshift = max(min(g, hot.maxgate - 1), 0) * framelen;
                     float dabs = ww;
                     if (RF) {
@@ -990,7 +994,31 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     py = py + vy * Lmove;
                     pz = pz + vz * Lmove;
                     const uint64_t foff = hot.fieldoff;
-                    if (!GRID) {
+                    if (BNODE) {
+                        /* linear basis under the Badouel tracer
+                         * (src/mmc_raytrace.c:1674-1692): ww/3 to each node
+                         * of the exit face out[faceidx] = {0,3,1} {3,2,1}
+                         * {0,2,3} {0,1,2}, i.e. every node but the one
+                         * opposite (2, 0, 1, 3); the thirds of consecutive
+                         * steps in one element and gate add up in registers
+                         * and reach the field when the photon leaves the
+                         * element, changes gate or ends */
+                        const uint32_t loc = (uint32_t)tshift;
+                        const float w3 = ww * (1.f / 3.f);
+                        const int opp = nib4(0x2013, faceidx);
+                        if (npend && loc != nloc) flush_nodal();
+                        if (!npend) {
+                            na0 = na1 = na2 = na3 = 0.f;
+                            nfoff = foff;
+                            nloc = loc;
+                            npend = true;
+                        }
+                        if (opp != 0) na0 += w3;
+                        if (opp != 1) na1 += w3;
+                        if (opp != 2) na2 += w3;
+                        if (opp != 3) na3 += w3;
+                        if (faceid == -2 || !isend) flush_nodal();
+                    } else if (!GRID) {
                         uint32_t newidx = (uint32_t)(e + tshift);
                         if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
                         if (newidx != oldidx) {
@@ -1461,7 +1489,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
 
         /* ============================================ photon ends here ===== */
         {
-            if (NODAL && npend) flush_nodal();
+            if ((NODAL || BNODE) && npend) flush_nodal();
             if (hot.flags & kFlagTraj) savetraj();
             const SimCold& C = a.cold[sidx()];
             const float w0 = *LS(slot_w0);
@@ -1568,7 +1596,9 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     template __global__ void transport<kTrElemRF, false, OCC, TAB>(KArgs);  \
     template __global__ void transport<kTrElemRF, true, OCC, TAB>(KArgs);   \
     template __global__ void transport<kTrGridRF, false, OCC, TAB>(KArgs);  \
-    template __global__ void transport<kTrGridRF, true, OCC, TAB>(KArgs);
+    template __global__ void transport<kTrGridRF, true, OCC, TAB>(KArgs);   \
+    template __global__ void transport<kTrElemNodal, false, OCC, TAB>(KArgs); \
+    template __global__ void transport<kTrElemNodal, true, OCC, TAB>(KArgs);
 MMC_INSTANTIATE(4, kTabLds)
 MMC_INSTANTIATE(5, kTabLds)
 MMC_INSTANTIATE(4, kTabGlobal)

@@ -416,8 +416,10 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
         if (!g.ncells || (uint64_t)g.ncells * (uint64_t)s->maxgate >= (1ull << 31))
             return set_err(-1, "dual grid of %g-unit voxels too large (%d x %d x %d voxels, %d gates)",
                            (double)s->gridstep, g.nx, g.ny, g.nz, s->maxgate);
-    } else if ((s->method == MMC_RT_PLUCKER) != (s->basisorder == 1)) {
-        return set_err(-1, "supported pairs: Plucker with nodal output, branch-less Badouel with element output");
+    } else if (s->method == MMC_RT_PLUCKER && s->basisorder != 1) {
+        return set_err(-1, "the Plucker tracer deposits on nodes (basisorder 1)");
+    } else if (s->basisorder != 0 && s->basisorder != 1) {
+        return set_err(-1, "basisorder must be 0 (elements) or 1 (nodes)");
     } else {
         /* the kernel indexes a pattern's field (gate x element or node) in 32 bits */
         const uint64_t rows = s->basisorder == 1 ? (uint64_t)m->nn : (uint64_t)m->ne;
@@ -468,7 +470,7 @@ static int check_sim(const mmc_hip_meshres* m, const mmc_hip_sim* s, int method0
                                "(src/mmc_raytrace.c:2396)");
     }
     if (!(s->omega >= 0.f)) return set_err(-1, "omega must be >= 0");
-    if (rf_of(s) && (s->method == MMC_RT_PLUCKER || slots_of(s) != 1))
+    if (rf_of(s) && (s->method == MMC_RT_PLUCKER || s->basisorder == 1 || slots_of(s) != 1))
         return set_err(-1, "RF (omega > 0) needs the branch-less Badouel tracer (element or grid output) and one "
                            "pattern");
     if (s->isreflect < MMC_BC_NOREFLECT || s->isreflect > MMC_BC_MIRROR)
@@ -631,7 +633,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     const bool rf = rf_of(&sims[0]);
     const int tracer = sims[0].method == MMC_RT_PLUCKER ? kTrNodal
                        : sims[0].method == MMC_RT_BLBADOUELGRID ? (rf ? kTrGridRF : kTrGrid)
-                                                                : (rf ? kTrElemRF : kTrElem);
+                       : sims[0].basisorder == 1 ? kTrElemNodal
+                                                 : (rf ? kTrElemRF : kTrElem);
     /* media rows 0..prop, then the detector label's row: a copy of the
      * background (src/mmc_mesh.c:518-520) */
     const int nmed = m->prop + 1 + m->isextdet;
@@ -694,6 +697,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         maxsrc = std::max(maxsrc, h.srcnum);
         if (rf_of(&s) != rf_of(&sims[0]) || (rf_of(&s) && s.omega != sims[0].omega))
             return set_err(-1, "simulation %d: all simulations of a batch must share omega", i);
+        if (s.method != MMC_RT_BLBADOUELGRID && s.basisorder != sims[0].basisorder)
+            return set_err(-1, "simulation %d: all simulations of a batch must share the output basis", i);
         const uint64_t flen = (uint64_t)datalen_of(m, &s) * s.maxgate * (rf_of(&s) ? 2 : 1);
         if (arena) {
             if (arena_off[i] + flen > arena->n)
@@ -954,6 +959,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
 #define MMC_KFN(T, O, S) (comp ? (const void*)transport<T, true, O, S> : (const void*)transport<T, false, O, S>)
 #define MMC_PICK(O, S)                                                                 \
     kfn = tracer == kTrNodal    ? MMC_KFN(kTrNodal, O, S)                               \
+          : tracer == kTrElemNodal ? MMC_KFN(kTrElemNodal, O, S)                        \
           : tracer == kTrGrid   ? MMC_KFN(kTrGrid, O, S)                                \
           : tracer == kTrGridRF ? MMC_KFN(kTrGridRF, O, S)                              \
           : tracer == kTrElemRF ? MMC_KFN(kTrElemRF, O, S)                              \
