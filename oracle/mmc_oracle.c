@@ -531,6 +531,7 @@ typedef struct {
      * one face passing the sign test, resolved by face order), and fixphoton
      * nudges (no face found) */
     int32_t ties, nudges;
+    double* srcweight;         /* patterns sharing photons: launched, then absorbed energy per pattern */
 } octx;
 
 #define DELTA_MUA 1e-4f        /* src/mmc_mesh.h:73 */
@@ -572,6 +573,20 @@ static inline const float* med_of(const orc_mesh* m, int t) { return m->med + 4 
 static inline void add_w(double* w, size_t idx, double v) {
 #pragma omp atomic
     w[idx] += v;
+}
+
+/* a deposit v into value idx of the field; with patterns sharing the photon
+ * (srcnum > 1) slot k of the value receives v times the photon's weight in
+ * pattern k, the pattern value at its launch pixel
+ * (src/mmc_raytrace.c:410-423, 1586-1612) */
+static inline void depo_w(const octx* c, const oray* r, size_t idx, double v) {
+    const orc_cfg* cfg = c->cfg;
+    if (cfg->srcnum > 1) {
+        const float* pw = cfg->srcpattern + (size_t)r->posidx * cfg->srcnum;
+        for (int k = 0; k < cfg->srcnum; k++) add_w(c->weight, idx * cfg->srcnum + k, (double)((float)v * pw[k]));
+    } else {
+        add_w(c->weight, idx, v);
+    }
 }
 
 /* branchless_badouel_raytet, src/mmc_raytrace.c:1414-1709 (element output,
@@ -728,7 +743,7 @@ static float orc_badouel(oray* r, octx* c) {
                         dim = (amag2 < EPS) ? (w0i * w0) : (dwi * mua - dwr * kap) / amag2;
                     }
                     if (newidx != r->oldidx) {
-                        add_w(c->weight, r->oldidx, r->oldweight);
+                        depo_w(c, r, r->oldidx, r->oldweight);
                         if (rf) add_w(c->weight_im, r->oldidx, r->oldweight_im);
                         r->oldidx = newidx;
                         r->oldweight = d;
@@ -738,7 +753,7 @@ static float orc_badouel(oray* r, octx* c) {
                         r->oldweight_im += dim;
                     }
                     if (r->faceid == -2 || !r->isend) {
-                        add_w(c->weight, newidx, r->oldweight);
+                        depo_w(c, r, newidx, r->oldweight);
                         if (rf) add_w(c->weight_im, newidx, r->oldweight_im);
                         r->oldweight = 0.f;
                         r->oldweight_im = 0.f;
@@ -759,7 +774,7 @@ static float orc_badouel(oray* r, octx* c) {
                     /* RF: the GPU reference flushes only a register whose real
                      * part is positive (src/mmc_core.cl:905) */
                     if (!rf) {
-                        add_w(c->weight, r->oldidx, r->oldweight);
+                        depo_w(c, r, r->oldidx, r->oldweight);
                     } else if (r->oldweight > 0.0) {
                         add_w(c->weight, r->oldidx, r->oldweight);
                         add_w(c->weight_im, r->oldidx, r->oldweight_im);
@@ -772,7 +787,8 @@ static float orc_badouel(oray* r, octx* c) {
                     r->oldweight_im += ww_im;
                 }
                 if (r->faceid == -2 || !r->isend) {
-                    add_w(c->weight, newidx, r->oldweight);
+                    if (rf) add_w(c->weight, newidx, r->oldweight);
+                    else depo_w(c, r, newidx, r->oldweight);
                     if (rf) add_w(c->weight_im, newidx, r->oldweight_im);
                     r->oldweight = 0.f;
                     r->oldweight_im = 0.f;
@@ -782,7 +798,7 @@ static float orc_badouel(oray* r, octx* c) {
                  * (src/mmc_raytrace.c:1674-1690) */
                 const int* ee = m->elem + 4 * (size_t)eid;
                 ww *= 1.f / 3.f;
-                for (int i = 0; i < 3; i++) add_w(c->weight, ee[O_out[faceidx][i]] - 1 + tshift, ww);
+                for (int i = 0; i < 3; i++) depo_w(c, r, (size_t)(ee[O_out[faceidx][i]] - 1 + tshift), ww);
             }
         }
     }
@@ -881,7 +897,7 @@ static float orc_plucker(oray* r, octx* c) {
             r->photontimer += r->Lmove * rc;
             int g = replay_gate(c, r->photontimer);
             int tshift = (g < cfg->maxgate - 1 ? (g > 0 ? g : 0) : cfg->maxgate - 1) * m->ne;
-            add_w(c->weight, (size_t)eid + tshift, ww);
+            depo_w(c, r, (size_t)eid + tshift, ww);
         } else if (Lp0 > EPS) {
             r->photontimer += r->Lmove * rc;
             float ww = currweight - r->weight;
@@ -897,7 +913,7 @@ static float orc_plucker(oray* r, octx* c) {
                     for (int i = 0; i < 4; i++)
                         baryout[i] = (1.f - ratio) * r->bary0[i] + ratio * baryout[i];
                 for (int i = 0; i < 4; i++)
-                    add_w(c->weight, (size_t)(ee[i] - 1 + tshift), ww * (r->bary0[i] + baryout[i]));
+                    depo_w(c, r, (size_t)(ee[i] - 1 + tshift), ww * (r->bary0[i] + baryout[i]));
             }
             if (r->isend) {
                 memcpy(r->bary0, baryout, sizeof(float) * 4);
@@ -1387,7 +1403,19 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
 
     if (orc_launch(&r, c, ran)) return -1;
     float w0 = r.weight;
-    ppath[reclen - 2] = r.weight;
+    if (cfg->srcnum > 1) {
+        /* shared patterns record the launch pixel in the weight column
+         * (src/mmc_raytrace.c:1824) and tally each pattern's launched energy */
+        int32_t px = (int32_t)r.posidx;
+        memcpy(&ppath[reclen - 2], &px, 4);
+        if (c->srcweight)
+            for (int k = 0; k < cfg->srcnum; k++) {
+#pragma omp atomic
+                c->srcweight[k] += cfg->srcpattern[(size_t)r.posidx * cfg->srcnum + k];
+            }
+    } else {
+        ppath[reclen - 2] = r.weight;
+    }
     const int rf = cfg->omega > 0.f && !cfg->replayweight;
     r.wmag = r.weight;
     /* replay with weighted path / scattering outputs: the detected weight
@@ -1536,6 +1564,11 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
         }
     }
     *absorbw += r.Eabsorb;
+    if (cfg->srcnum > 1 && c->srcweight)
+        for (int k = 0; k < cfg->srcnum; k++) {
+#pragma omp atomic
+            c->srcweight[cfg->srcnum + k] += (double)r.Eabsorb * cfg->srcpattern[(size_t)r.posidx * cfg->srcnum + k];
+        }
 
     if (rec) {
         rec->p[0] = r.p0.x; rec->p[1] = r.p0.y; rec->p[2] = r.p0.z;
@@ -1570,6 +1603,7 @@ int orc_run(const orc_cfg* cfg, const orc_mesh* mesh, orc_out* out) {
     base.m = mesh;
     base.weight = out->weight;
     base.weight_im = out->weight_im;
+    base.srcweight = out->srcweight;
     base.rtstep = 1.f / cfg->tstep;
     base.framelen = (cfg->method == ORC_RT_BLBADOUELGRID)
                         ? cfg->griddim[0] * cfg->griddim[1] * cfg->griddim[2]

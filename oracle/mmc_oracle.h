@@ -135,6 +135,8 @@ typedef struct {
     double* weight_im;       /* RF: imaginary part of the field, or NULL */
     int32_t* ties;           /* reccount x 2 per recorded photon: tie-decided ray-tet tests, fixphoton
                                 nudges; or NULL */
+    double* srcweight;       /* srcnum > 1: launched energy of each pattern, then absorbed energy of
+                                each pattern (2*srcnum, +=), or NULL */
 } orc_out;
 
 /* RNG (src/mmc_rand_xorshift128p.c:55-83, src/mmc_rand_common.h:50-51) */

@@ -51,7 +51,7 @@ class OOut(C.Structure):
     _fields_ = [("weight", f64p), ("launchweight", C.c_double), ("absorbweight", C.c_double),
                 ("raytet", C.c_double), ("detdata", f32p), ("detphoton", i64p), ("maxdet", C.c_int),
                 ("detected", C.c_int), ("reclen", C.c_int), ("rec", C.c_void_p), ("runtime_ms", C.c_double),
-                ("error", C.c_int), ("weight_im", f64p), ("ties", i32p)]
+                ("error", C.c_int), ("weight_im", f64p), ("ties", i32p), ("srcweight", f64p)]
 
 
 PHOTON_DTYPE = np.dtype([
@@ -185,7 +185,7 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     c.issaveexit = int(sim.issaveexit) if c.issavedet else 0
     c.ismomentum = int(getattr(sim, "ismomentum", 0) > 0)
     c.srctype = SOURCES[sim.srctype]
-    c.srcnum = 1
+    c.srcnum = int(sim.srcnum)
     pos = list(sim.srcpos) + [0.0] * (4 - len(sim.srcpos))
     dirv = list(sim.srcdir) + [0.0] * (4 - len(sim.srcdir))
     c.srcpos[:], c.srcdir[:] = pos[:4], dirv[:4]
@@ -206,7 +206,10 @@ def make_cfg(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None):
     else:
         c.e0 = int(sim.e0) if sim.e0 > 0 else max(int(mesh.initelem(pos)[0]), 1)
     if sim.srcpattern is not None:
-        pat = np.ascontiguousarray(sim.srcpattern, np.float32).ravel()
+        pat = np.asarray(sim.srcpattern, np.float32)
+        if pat.ndim == 3 and sim.srctype == "pattern":     # (srcnum, ny, nx) -> pixel-major, pattern fastest
+            pat = np.transpose(pat, (1, 2, 0))
+        pat = np.ascontiguousarray(pat).ravel()
         keep.append(pat)
         c.srcpattern = p(pat, C.c_float)
     else:
@@ -261,12 +264,15 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
         datalen = int(np.prod(list(cfg.griddim)))
     else:
         datalen = mesh.nn if cfg.basisorder else mesh.ne
-    field = np.zeros((cfg.maxgate, datalen), np.float64)
+    S = int(cfg.srcnum)
+    field = np.zeros((cfg.maxgate, datalen, S) if S > 1 else (cfg.maxgate, datalen), np.float64)
+    srcweight = np.zeros((2, S), np.float64) if S > 1 else None
     rf = cfg.omega > 0 and getattr(sim, "replayweight", None) is None   # RF is off for replays
     field_im = np.zeros_like(field) if rf else None
     out = OOut()
     out.weight = p(field, C.c_double)
     out.weight_im = p(field_im, C.c_double) if rf else C.cast(None, f64p)
+    out.srcweight = p(srcweight, C.c_double) if srcweight is not None else C.cast(None, f64p)
     reclen = (2 + (cfg.ismomentum > 0)) * mesh.prop + 6 * cfg.issaveexit + 2
     det = np.zeros((max(maxdet, 1), reclen), np.float32)
     detph = np.zeros(max(maxdet, 1), np.int64)
@@ -293,6 +299,7 @@ def run(mesh: OracleMesh, sim, photonseed=None, nthreads=0, records=None, maxdet
     # per recorded photon: ray-tet tests decided by a tie rule, fixphoton nudges
     r.ties = ties
     r.field_im = field_im
+    r.srcweight = srcweight
     r.normalizer = None
     if normalize:
         r.normalizer = L.orc_normalize(C.byref(cfg), C.byref(mesh.c), p(field, C.c_double),

@@ -193,3 +193,70 @@ def test_effective_reflection_coefficient():
     # published values (Haskell et al. 1994, n_out = 1): n=1.33 -> 0.431, n=1.4 -> 0.493
     assert abs(L.orc_getreff(1.33, 1.0) - 0.431) < 0.01
     assert abs(L.orc_getreff(1.4, 1.0) - 0.493) < 0.01
+
+
+def _exact_containing(node, elem, p):
+    """Elements containing p by the signs of the four sub-volumes computed in
+    double precision (the first-match rule of mesh_initelem over them):
+    returns a (ne,) array of the smallest signed sub-volume per element."""
+    v = node[elem - 1].astype(np.float64)            # (ne, 4, 3)
+    q = np.asarray(p, np.float64)
+    # the sub-volume opposite node i: replace node i by p in the determinant
+    def vol(a, b, c, d):
+        return np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)
+    full = vol(v[:, 0], v[:, 1], v[:, 2], v[:, 3])
+    sub = []
+    for i in range(4):
+        w = v.copy()
+        w[:, i] = q
+        sub.append(vol(w[:, 0], w[:, 1], w[:, 2], w[:, 3]) * np.sign(full))
+    return np.min(np.stack(sub, 1), axis=1)
+
+
+def test_initelem_first_match_and_barycentrics_by_brute_force():
+    """orc_initelem (mesh_initelem, src/mmc_mesh.c:1060-1090) against an
+    exhaustive double-precision search: (1) points inside a jittered mesh lie
+    in exactly one element by a clear margin, which must be the one found;
+    (2) lattice nodes and edge midpoints of an unjittered lattice (all
+    products exact in FP32) lie in several elements, and the first match in
+    element order must be the lowest-index element containing them;
+    barycentrics are non-negative, sum to 1 and reconstruct the point."""
+    rng = np.random.default_rng(5)
+    # (1) interior points of a jittered mesh
+    node, elem = M.meshgen.t6_lattice(6, 6, 4, 1.0)
+    node = M.meshgen.jitter_interior(node, 0.25, 3, (0.0, 0.0, 0.0), (6.0, 6.0, 4.0))
+    et = np.ones(len(elem), np.int32)
+    med = np.array([[0, 0, 1, 1], [0.01, 1.0, 0.9, 1.37]], np.float32)
+    om = O.OracleMesh(node, elem, et, med, nodal=False)
+    checked = 0
+    for _ in range(300):
+        p = rng.uniform((0.1, 0.1, 0.1), (5.9, 5.9, 3.9)).astype(np.float32)
+        m = _exact_containing(node, elem, p)
+        inside = np.flatnonzero(m >= -1e-9)
+        if len(inside) != 1 or m[inside[0]] < 1e-4:
+            continue                                 # near a face: the FP32 test may differ
+        e, b = om.initelem(p)
+        assert e == inside[0] + 1
+        assert np.all(b >= 0) and abs(b.sum() - 1) < 1e-5
+        # barycentrics follow the element's nodes after the orientation fix of
+        # mesh_getvolume (the oracle mesh's element order)
+        rec = (b[:, None] * node[om.elem[e - 1] - 1]).sum(axis=0)
+        assert np.abs(rec - p).max() < 1e-4
+        checked += 1
+    assert checked > 200
+    # (2) shared lattice points: first match in element order
+    node, elem = M.meshgen.t6_lattice(4, 4, 3, 1.0)
+    et = np.ones(len(elem), np.int32)
+    om = O.OracleMesh(node, elem, et, med, nodal=False)
+    pts = [node[i] for i in rng.choice(len(node), 40, replace=False)]
+    pts += [0.5 * (node[elem[k, 0] - 1] + node[elem[k, j] - 1]) for k in rng.choice(len(elem), 40) for j in (1, 3)]
+    nshared = 0
+    for p in pts:
+        p = np.asarray(p, np.float32)
+        m = _exact_containing(node, elem, p)
+        first = int(np.flatnonzero(m >= 0)[0]) + 1
+        e, b = om.initelem(p)
+        assert e == first
+        assert np.all(b >= 0) and abs(b.sum() - 1) < 1e-6
+        nshared += (m >= 0).sum() > 1
+    assert nshared > 40

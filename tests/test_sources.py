@@ -142,6 +142,43 @@ def test_pencil_array_and_pattern3d():
     assert np.array_equal(ph["w0"][clear], pat[iz, iy, ix][clear])
 
 
+def test_shared_patterns_weight_each_pattern():
+    # photon sharing (srcnum > 1, src/mmc_raytrace.c:410-423, 1586-1612,
+    # 1812-1835) in an absorbing box without scattering: each photon flies
+    # straight up 10 mm, so its launch pixel is read back from its exit
+    # (x, y), and pattern k's launched energy is the sum of its values at
+    # those pixels and its absorbed energy that sum times 1 - exp(-10 mua).
+    # x and y do not change on the way up, so the pixel is exact away from
+    # its edges.  Slot k of the field (energy / mua per element for fluence
+    # output) equals the run with pattern k alone.
+    node, elem, etype, med, c = clear_box()
+    mua = 0.05
+    med[1] = (mua, 0.0, 1.0, 1.0)
+    srcelem = (np.where(c[:, 2] < 1.0)[0] + 1).astype(np.int32)
+    om = O.OracleMesh(node, elem, etype, med, nodal=False, isreflect=0, srcelem=srcelem)
+    pats = np.random.default_rng(9).uniform(0.2, 1.0, (3, 4, 5)).astype(np.float32)
+    kw = dict(nphoton=N, seed=5, method="S", srctype="pattern", srcpos=(-8.0, -6.0, 0.0), srcdir=(0.0, 0.0, 1.0),
+              srcparam1=(15.0, 0, 0, 5), srcparam2=(0, 12.0, 0, 4), tstart=0.0, tend=1e-8, tstep=1e-8,
+              outputtype="f", isreflect=0, minenergy=0.0, nout=1.0)
+    seeds = M.photon_seeds(5, 0, N)
+    sh = O.run(om, M.SimConfig(srcpattern=pats, **kw), photonseed=seeds, records=(0, N), math="portable")
+    ph = sh.photons
+    assert (ph["reason"] == 1).all() and np.allclose(ph["p"][:, 2], 10.0, atol=1e-4)
+    fx, fy = (ph["p"][:, 0] + 8.0) / 3.0, (ph["p"][:, 1] + 6.0) / 3.0
+    clear = (np.abs(fx - np.round(fx)) > 1e-5) & (np.abs(fy - np.round(fy)) > 1e-5)
+    assert clear.all()                                   # no photon on a pixel edge at this seed
+    ix, iy = np.floor(fx).astype(int), np.floor(fy).astype(int)
+    assert sh.field.shape == (1, om.ne, 3) and sh.srcweight.shape == (2, 3)
+    for k in range(3):
+        w = pats[k, iy, ix].astype(np.float64)
+        assert abs(sh.srcweight[0, k] / w.sum() - 1) < 1e-6
+        assert abs(sh.srcweight[1, k] / (w.sum() * -np.expm1(-10 * mua)) - 1) < 1e-5
+        assert abs(sh.field[..., k].sum() * mua / sh.srcweight[1, k] - 1) < 1e-5   # fluence: energy / mua
+        one = O.run(om, M.SimConfig(srcpattern=pats[k], **kw), photonseed=seeds, math="portable")
+        assert abs(one.launchweight / sh.srcweight[0, k] - 1) < 1e-6
+        assert np.abs(sh.field[..., k] - one.field).max() <= 1e-6 * np.abs(one.field).max()
+
+
 def test_line_slit_hyperboloid_focus():
     ph = run_clear("line", srcpos=(-3.0, 0.0, 5.0), srcdir=(1.0, 0.0, 0.0), p1=(6, 0, 0, 0), zrange=(4.0, 6.0),
                    allow_timeout=True)
