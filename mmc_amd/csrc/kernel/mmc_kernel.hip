@@ -564,7 +564,11 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     const int slot_pw = slot_w0 + 3;
     const int slot_pl = slot_pw + a.maxsrc;
     const int slot_pa = slot_pl + a.maxsrc;
-    const int nslots = slot_w0 + 3 + (a.maxsrc > 1 ? 3 * a.maxsrc : 0);
+    /* with wide-field detector elements: the record's last column as the
+     * reference builds it (launch weight, or pixel bits, plus what lands
+     * there from detector elements) */
+    const int slot_wc = slot_w0 + 3 + (a.maxsrc > 1 ? 3 * a.maxsrc : 0);
+    const int nslots = slot_wc + a.isextdet;
 
     for (int i = tid; i < nstage; i += nthr) s_hot[i] = a.hot[i];
     for (int i = tid; i < nstage * nmed; i += nthr) s_med[i] = a.media[i];
@@ -634,6 +638,21 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     /* the lane's simulation index into the tables, and its hot parameters */
     auto sidx = [&]() -> int { return sim; };
     auto hot_of = [&]() -> SimHot { return STAGE ? s_hot[sim] : a.hot[sim]; };
+    /* partial path of a step in medium ctype (src/mmc_raytrace.c:1869,
+     * 1954, 1969): record column prop-1+ctype.  For a wide-field detector
+     * element (ctype = prop+1) that is column 2 prop, which the reference
+     * leaves where it lands: the first momentum column, else the exit
+     * columns (rewritten when the photon leaves), else the launch-weight
+     * column. */
+    auto add_path = [&](float L) {
+        if (ctype <= a.prop) {
+            *LS(ppbase + a.prop - 1 + ctype) += L;
+        } else if (hot_of().flags & kFlagMom) {
+            *LS(ppbase + 2 * a.prop) += L;
+        } else if (!a.cold[sim].issaveexit) {
+            *LS(slot_wc) += L;
+        }
+    };
 
     /* deposit v at value loc of the simulation's field (offset foff); with
      * shared patterns the value has nsrc slots and slot k receives v times
@@ -850,6 +869,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     *LS(slot_ns) = 0.f;
                     if (a.savedet)
                         for (int k = 0; k < (2 + a.mom) * a.prop; k++) *LS(ppbase + k) = 0.f;
+                    if (a.isextdet) *LS(slot_wc) = (nsrc > 1) ? __int_as_float(posidx) : w;
                     /* replay with weighted path / scattering outputs counts the
                      * detected weight as launched energy (src/mmc_raytrace.c:1815);
                      * a pattern source first scales it by the launch pixel's
@@ -1282,13 +1302,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         faceid = -1;
                     }
                 }
-                /* (paths in wide-field detector elements, label prop+1, are
-                 * not partial paths of a tissue) */
-                if (!next_iter && savedet && Lmove > 0.f && faceid >= 0 && ctype > 0 && ctype <= a.prop)
-                    *LS(ppbase + a.prop - 1 + ctype) += Lmove;
+                if (!next_iter && savedet && Lmove > 0.f && faceid >= 0 && ctype > 0) add_path(Lmove);
             } else {
-                if (savedet && Lmove > 0.f && ctype > 0 && ctype <= a.prop)
-                    *LS(ppbase + a.prop - 1 + ctype) += Lmove;
+                if (savedet && Lmove > 0.f && ctype > 0) add_path(Lmove);
                 if (ctx == CTX_INNER) fixcount = 0;
                 if (undef) {
                     if (fixcount++ < MAX_TRIAL) {
@@ -1477,7 +1493,10 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                     if (hot.flags & kFlagTraj) savetraj();
                     if (savedet && ctype > 0) *LS(ppbase + ctype - 1) += 1.f;
                     /* momentum transfer (src/mmc_raytrace.c:2073-2075) */
-                    if ((hot.flags & kFlagMom) && ctype > 0) *LS(ppbase + 2 * a.prop - 1 + ctype) += 1.f - ctheta;
+                    if ((hot.flags & kFlagMom) && ctype > 0) {
+                        if (ctype <= a.prop) *LS(ppbase + 2 * a.prop - 1 + ctype) += 1.f - ctheta;
+                        else if (!a.cold[sidx()].issaveexit) *LS(slot_wc) += 1.f - ctheta;
+                    }
                     *LS(slot_ns) += 1.f;
                     ctx = CTX_OUTER;
                     continue;
@@ -1524,7 +1543,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
                         }
                         /* shared patterns record the launch pixel instead of
                          * the launch weight (src/mmc_raytrace.c:1824) */
-                        row[col] = (nsrc > 1) ? __int_as_float(posidx) : w0;
+                        row[col] = a.isextdet ? *LS(slot_wc) : (nsrc > 1) ? __int_as_float(posidx) : w0;
                         if (a.detphoton) a.detphoton[slot] = gid;
                     }
                 }

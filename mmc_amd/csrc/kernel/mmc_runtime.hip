@@ -929,7 +929,7 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     for (int i = 0; i < nsim; i++) anymom |= mom_of(&sims[i], m);
     a.mom = anymom;
 
-    const int nslots = kLaneSlots + (2 + anymom) * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0);
+    const int nslots = kLaneSlots + (2 + anymom) * m->prop + 3 + (maxsrc > 1 ? 3 * maxsrc : 0) + m->isextdet;
     /* per-simulation tables go to LDS while they take at most 16 KiB, so that
      * large batches keep 4+ workgroups per CU */
     const size_t simtab = (sizeof(SimHot) + sizeof(float4) * (size_t)nmed) * nsim;

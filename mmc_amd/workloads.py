@@ -72,15 +72,21 @@ def slab_batch_sims(nsim: int = 256, seed: int = 7, prop: int = 5, lo=1e5, hi=1e
     return sims
 
 
-def shells_mesh(h: float = 1.0, window=(70.0, 130.0)):
-    """The large jittered mesh with the launch candidates of the planar
-    source: the elements of the top cell layer under the source window."""
+def shells_arrays(h: float = 1.0, window=(70.0, 130.0)):
+    """(node, elem, etype, med, srcelem) of the large jittered mesh; srcelem
+    are the launch candidates of the planar source: the elements of the top
+    cell layer under the source window."""
     n = int(round(200 / h))
     node, elem, et, med = meshgen.shells_box(n, n, n // 2, h)
     c = meshgen.centroids(node, elem)
     lo, hi = window[0] - 2 * h, window[1] + 2 * h
     top = (c[:, 2] < h) & (c[:, 0] > lo) & (c[:, 0] < hi) & (c[:, 1] > lo) & (c[:, 1] < hi)
     srcelem = (np.nonzero(top)[0] + 1).astype(np.int32)
+    return node, elem, et, med, srcelem
+
+
+def shells_mesh(h: float = 1.0, window=(70.0, 130.0)):
+    node, elem, et, med, srcelem = shells_arrays(h, window)
     return TetMesh(node, elem, et, med, srcelem=srcelem).prepare(nodal=True)
 
 

@@ -1422,9 +1422,13 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
      * is the launched energy (src/mmc_raytrace.c:1815-1817) */
     *launchw += (cfg->outputtype == ORC_OT_WL || cfg->outputtype == ORC_OT_WP) ? c->rw : r.weight;
 
+/* partial paths (src/mmc_raytrace.c:1869, 1954, 1969): column prop-1+type;
+ * for a wide-field detector element (type prop+1) that is column 2 prop,
+ * left where it lands as the reference does (the first momentum column, an
+ * exit column rewritten at the exit, or the launch-weight column) */
 #define PPATH_ADD(cond)                                                            \
     do {                                                                          \
-        if (cfg->issavedet && (cond) && m->type[r.eid - 1] <= prop)                 \
+        if (cfg->issavedet && (cond))                                              \
             ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove;                        \
     } while (0)
 
@@ -1445,10 +1449,7 @@ static int orc_onephoton(int64_t id, octx* c, uint64_t ran[2], odet* det, double
         }
         /* the reference reads type[eid-1] before testing faceid>=0; with
          * eid=ID_UNDEFINED faceid is -1, so no path is added */
-        /* paths in wide-field detector elements (label prop+1) are not
-         * partial paths of a tissue */
-        if (cfg->issavedet && r.Lmove > 0.f && r.faceid >= 0 && m->type[r.eid - 1] > 0 &&
-            m->type[r.eid - 1] <= prop)
+        if (cfg->issavedet && r.Lmove > 0.f && r.faceid >= 0 && m->type[r.eid - 1] > 0)
             ppath[prop - 1 + m->type[r.eid - 1]] += r.Lmove;
         if (r.faceid == -2) break;
 

@@ -129,3 +129,60 @@ def test_oracle_pattern_replay_weights_take_the_launch_pixel():
     wl = O.run(om, M.SimConfig(**base), photonseed=seeds[ids], math="portable")
     assert abs(wl.field.sum() / (rw * w0 * ppath).sum() - 1) < 1e-5
     assert wl.launchweight == pytest.approx(float((rw.astype(np.float64) * w0).sum()), rel=1e-6)
+
+
+def _chords(node, elem, p0, d, tmax):
+    """FP64 length of the ray p0 + t d, 0 <= t <= tmax, inside each tetrahedron
+    (intersection of the four half-spaces, brute force over all elements)."""
+    P = node[elem - 1].astype(np.float64)                          # (ne, 4, 3)
+    lo, hi = np.zeros(len(elem)), np.full(len(elem), tmax)
+    for f in range(4):
+        a, b, c = (P[:, (f + k) % 4] for k in (1, 2, 3))
+        n = np.cross(b - a, c - a)
+        n *= np.sign(((P[:, f] - a) * n).sum(1))[:, None]           # inward: towards the fourth node
+        num, den = ((a - p0) * n).sum(1), (d * n).sum(1)            # inside: den * t >= num
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = num / den
+        lo = np.where(den > 0, np.maximum(lo, t), lo)
+        hi = np.where(den < 0, np.minimum(hi, t), hi)
+        lo = np.where((den == 0) & (num > 0), np.inf, lo)
+    return np.maximum(hi - lo, 0.0)
+
+
+def test_oracle_jacobian_replay_closed_form():
+    """J replay of a non-scattering pencil beam (src/mmc_raytrace.c:1523-1527,
+    1568-1570, mmc_fluence_deposit at :61-63): a step of length L in an
+    element of absorption mua deposits rw e^(-dmua L) / mua, and, where
+    mua = 0, (rw e^(-dmua L) + w) L with w the photon's weight there, dmua =
+    1e-4 /mm (src/mmc_mesh.h:73).  The chord of the beam in every element
+    comes from an FP64 ray-tetrahedron clip over all elements."""
+    node, elem = M.meshgen.t6_lattice(8, 8, 10, 1.0, origin=(-4.0, -4.0, 0.0))
+    c = M.meshgen.centroids(node, elem)
+    etype = np.where(c[:, 2] < 5.0, 1, 2).astype(np.int32)
+    mua1 = 0.02
+    med = np.array([[0, 0, 1, 1], [mua1, 0.0, 1.0, 1.0], [0.0, 0.0, 1.0, 1.0]], np.float32)
+    srcelem = (np.where(c[:, 2] < 1.0)[0] + 1).astype(np.int32)
+    om = O.OracleMesh(node, elem, etype, med, nodal=False, isreflect=0, srcelem=srcelem)
+    d = np.array([0.15, 0.1, 1.0])
+    d /= np.linalg.norm(d)
+    p0 = np.array([0.31, -0.17, 0.0])
+    L = _chords(node, elem, p0, d, 10.0 / d[2])
+    hit = L > 0
+    assert L[hit].min() > 1e-3 and abs(L.sum() - 10.0 / d[2]) < 1e-9      # no edge grazed
+    n = 16
+    rw = np.random.default_rng(4).uniform(0.5, 2.0, n).astype(np.float32)
+    cfg = M.SimConfig(nphoton=n, seed=5, method="S", srctype="pencil", srcpos=tuple(p0), srcdir=tuple(d),
+                      tstart=0.0, tend=1e-8, tstep=1e-8, outputtype="j", isreflect=0, minenergy=0.0, nout=1.0,
+                      replayweight=rw, replaytime=np.full(n, 1e-10, np.float32))
+    o = O.run(om, cfg, photonseed=M.photon_seeds(5, 0, n), math="portable")
+    f = o.field[0]
+    assert np.count_nonzero(f) == hit.sum() and np.all(f[hit] > 0)
+    mua = med[etype, 0].astype(np.float64)
+    # the weight where the beam enters each element of the clear layer: all
+    # of the absorbing layer's 5 / d_z mm behind it
+    w = np.where(mua > 0, 0.0, np.exp(-mua1 * 5.0 / d[2]))
+    e = np.exp(-1e-4 * L)
+    want = np.where(mua > 0, rw.sum(dtype=np.float64) * e / np.where(mua > 0, mua, 1.0),
+                    (rw.sum(dtype=np.float64) * e + n * w) * L)
+    assert (mua[hit] > 0).sum() > 10 and (mua[hit] == 0).sum() > 10
+    assert np.allclose(f[hit], want[hit], rtol=2e-5, atol=0)

@@ -9,6 +9,10 @@ inside the tissue (the tissue/detector face refracts and reflects exactly as
 the tissue/outside face with the same outside index), so the photons the
 oracle detects through the layer are exactly those that leave the bare box
 through its top face, with the same scattering counts and partial paths.
+The path a photon then travels through detector elements lands in record
+column 2 prop + 1, as in the reference (src/mmc_raytrace.c:1869 with type
+prop+1): without momentum or exit columns that is the launch-weight column,
+which becomes 1 + the straight flight through the layer.
 """
 import numpy as np
 import pytest
@@ -58,24 +62,32 @@ def test_detector_layer_detects_the_photons_leaving_through_the_top():
     assert np.all(with_det.etype[od.photons["detid"][det] - 1] == 2)
     assert od.detected == det.sum() and len(od.detdata) == det.sum()
     # records: detector id, scattering count, partial path (tissue 1) and
-    # launch weight; the detector element's path is not a tissue path
+    # launch weight plus the path through the detector layer, which the
+    # photon crosses in a straight line from where it left the bare box
     rec = od.detdata[np.argsort(od.detphoton)]
     ids = np.sort(od.detphoton)
     assert np.array_equal(rec[:, 0].astype(np.int64), od.photons["detid"][ids])
     assert np.array_equal(rec[:, 1], ob.photons["nscat"][ids])
-    assert np.all(rec[:, 3] == 1.0)
+    flight = np.linalg.norm(od.photons["p"][ids].astype(np.float64) - ob.photons["p"][ids], axis=1)
+    assert np.median(flight) > 1.0                       # most leave through the top, some the sides
+    assert np.allclose(rec[:, 3], 1.0 + flight, rtol=0, atol=1e-4)
     # the photons' paths through the tissue are unchanged by the layer
     assert np.array_equal(od.photons["nscat"], ob.photons["nscat"])
     assert np.array_equal(od.photons["w"][det], ob.photons["w"][top])
 
 
 @pytest.mark.gpu
-def test_gpu_detector_elements_match_the_oracle():
+@pytest.mark.parametrize("cols", ["plain", "momentum", "exit"])
+def test_gpu_detector_elements_match_the_oracle(cols):
+    # the detector-layer path lands in the weight, first momentum or (then
+    # rewritten) first exit column
     arrays = box(6, True)
     tm = M.TetMesh(*arrays).prepare(nodal=False)
     om = O.OracleMesh(*arrays, nodal=False)
     assert tm.isextdet == 1
     cfg = sim(20000)
+    cfg.ismomentum = int(cols == "momentum")
+    cfg.issaveexit = int(cols == "exit")
     g = M.run(tm, cfg, records=(0, cfg.nphoton))
     o = O.run(om, cfg, photonseed=M.photon_seeds(cfg.seed, 0, cfg.nphoton), records=(0, cfg.nphoton),
               maxdet=cfg.nphoton, math="portable")
