@@ -45,8 +45,8 @@ def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None, nodal=False):
     the same workload, with every core this process may run on."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as O
-    node, elem, et, med = mesh_arrays
-    om = O.OracleMesh(node, elem, et, med, nodal=nodal)
+    node, elem, et, med = mesh_arrays[:4]
+    om = O.OracleMesh(node, elem, et, med, nodal=nodal, srcelem=mesh_arrays[4] if len(mesh_arrays) > 4 else None)
     avail = len(os.sched_getaffinity(0))
     threads = threads or avail
     import copy
@@ -70,19 +70,30 @@ def cpu_baseline(mesh_arrays, sim, budget_s=15.0, threads=None, nodal=False):
 
 def bench_plain(args):
     """The plain drop-in path on a device-resident mesh: the slab workload
-    (default) or, with --workload cube, the 60 mm cube of 2 mm cells with
-    Plucker nodal fluence (for profiling that kernel)."""
+    (default) or, for profiling the Plucker nodal kernel, --workload cube
+    (the 60 mm cube of 2 mm cells) or --workload shells (the 24.0M-tet
+    jittered shells mesh, one 64x64 pattern request, 100 gates)."""
     import mmc_amd as M
     from mmc_amd import workloads as W
     from mmc_amd import meshgen
     t = time.time()
     cube = args.workload == "cube"
-    arrays = meshgen.cube60(2.0) if cube else meshgen.slab_layers(1.0)
-    mesh = M.TetMesh(*arrays).prepare(nodal=cube)
+    shells = args.workload == "shells"
+    nodal = cube or shells
+    if shells:
+        arrays = W.shells_arrays(1.0)
+        mesh = M.TetMesh(*arrays[:4], srcelem=arrays[4]).prepare(nodal=True)
+    else:
+        arrays = meshgen.cube60(2.0) if cube else meshgen.slab_layers(1.0)
+        mesh = M.TetMesh(*arrays).prepare(nodal=cube)
     res = M.ResidentMesh(mesh)
     log(f"{args.workload} mesh: {mesh.ne} tets, prep+upload {time.time() - t:.1f} s")
-    sim = W.cube60_sim(args.photons) if cube else W.slab_sim(args.photons)
-    bytes_per_step = BYTES_PER_STEP_NODAL if cube else BYTES_PER_STEP_ELEM
+    if shells:
+        sim = W.shells_sims(8, photons=8 * args.photons)[0]
+        sim.e0 = int(mesh.srcelem[0])
+    else:
+        sim = W.cube60_sim(args.photons) if cube else W.slab_sim(args.photons)
+    bytes_per_step = BYTES_PER_STEP_NODAL if nodal else BYTES_PER_STEP_ELEM
     for i in range(args.warmup):
         r = res.run_batch([sim], want_field=False)[0]
         log(f"warmup {i}: {r.kernel_ms:.1f} ms kernel, {r.total_ms:.1f} ms total")
@@ -110,8 +121,12 @@ def bench_plain(args):
         with open(cfile) as f:
             counters = json.load(f)
     traffic = counters["hbm_bytes_per_step"] * steps_per_launch if counters else None
-    kname = "mmc::transport<1, false, 4, 1>" if cube else "mmc::transport<0, false, 4, 1>"
-    if cube:
+    kname = "mmc::transport<1, false, 4, 1>" if nodal else "mmc::transport<0, false, 4, 1>"
+    if shells:
+        workload = (f"shells: jittered 200x200x100 mm box, 1 mm cells x6 = {mesh.ne / 1e6:.1f}M tets, 5 tissues in "
+                    f"nested ellipsoidal shells, one 64x64 pattern over 60x60 mm, {args.photons:.0e} photons/step, "
+                    "100 gates x 0.05 ns, nodal fluence (Plucker), 32 detectors")
+    elif cube:
         workload = (f"cube: 60 mm, 2 mm cells x6 = {mesh.ne} tets, mua 0.005, mus 1.0, g 0.01, n 1.37, pencil at "
                     f"(30,30,0), {args.photons:.0e} photons/step, 1 gate 0-5 ns, nodal fluence (Plucker)")
     else:
@@ -327,8 +342,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=100_000_000)
-    ap.add_argument("--workload", default="slab", choices=["slab", "cube"],
-                    help="plain-path workload: the slab (default) or the 60 mm cube (Plucker nodal)")
+    ap.add_argument("--workload", default="slab", choices=["slab", "cube", "shells"],
+                    help="plain-path workload: the slab (default), the 60 mm cube or the 24.0M-tet shells "
+                         "mesh with one pattern request (both Plucker nodal)")
     ap.add_argument("--mode", default="all", choices=["plain", "batch", "serve", "all", "large"],
                     help="all (default): the plain slab steps, then the 256-simulation batch and the "
                          "served request stream")
@@ -363,7 +379,8 @@ def main():
     if args.mode == "large":
         out["large"] = bench_large(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds, nodal=args.workload == "cube")
+        out["cpu_baseline"] = cpu_baseline(arrays, sim, budget_s=args.cpu_seconds,
+                                           nodal=args.workload in ("cube", "shells"))
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

@@ -110,6 +110,21 @@ __device__ __forceinline__ const double* dk() {
     return (const double*)p;
 }
 
+
+/* The launch arguments, through an opaque copy of the kernel-argument
+ * segment's address (the by-value KArgs parameter sits at its start).  The
+ * transport loop takes a fresh copy every iteration, so the arguments are
+ * reloaded (scalar loads) within the iteration that uses them instead of
+ * being held in scalar registers across the loop: held, they overflowed the
+ * scalar register file into lanes of vector registers (147 spilled scalar
+ * registers, restored by v_readlane in the loop). */
+typedef const __attribute__((address_space(4))) KArgs* kargs_ptr;
+__device__ __forceinline__ const KArgs* kargs() {
+    kargs_ptr p = (kargs_ptr)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return (const KArgs*)p;
+}
+
 /* true when the C library may round e to the far neighbour of f = (float)e:
  * e within D |o - f| of the midpoint (every operation here is exact) */
 __device__ __forceinline__ bool lm_near(double e, float f, float D) {
@@ -526,7 +541,10 @@ __device__ __forceinline__ uint32_t xcd_id() {
  * dual-grid voxel deposits (src/mmc_raytrace.c:1615-1673).
  */
 template <int TR, bool COMP, int OCC, int TAB>
-__global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
+__global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
+    (void)karg;                                   /* read through kargs() */
+    const KArgs* ka = kargs();
+#define a (*ka)
     constexpr bool NODAL = TR == kTrNodal;        /* Plucker tracer, nodal deposits */
     /* branch-less Badouel tracer with the linear (nodal) basis: a third of
      * each step's deposit to each node of the face the ray leaves through */
@@ -712,6 +730,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
     };
 
     while (true) {
+        ka = kargs();
         /* ------------------------------------------------ refill dead lanes */
         /* Lanes whose photon ended take the next id of their private range;
          * lanes with an empty range are refilled together: one atomic per
@@ -1576,6 +1595,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs a) {
 /* FP32 -> FP64 accumulation of a finished field (the final reduction): the
  * nrep copies (and their compensation buffers) summed in copy order; zeroes
  * the FP32 buffers for reuse. */
+#undef a
+
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
                              uint64_t repstride) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
