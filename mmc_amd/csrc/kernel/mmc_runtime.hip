@@ -945,9 +945,8 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
         return (v + 15) & ~(size_t)15;
     };
     while (lds_for(block) > kLdsMax && block > 64 && !(tune && tune->block > 0)) block /= 2;
-    const size_t shmem = lds_for(block);
-    if (shmem > kLdsMax)
-        return set_err(-1, "batch of %d simulations needs %zu B of LDS (%d media%s)", nsim, shmem, m->prop,
+    if (lds_for(block) > kLdsMax)
+        return set_err(-1, "batch of %d simulations needs %zu B of LDS (%d media%s)", nsim, lds_for(block), m->prop,
                        savedet ? " with detector records" : "");
 
     /* the register budget of the kernel variant follows the requested
@@ -976,7 +975,25 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
     int dev = 0, ncu = 0, maxbpc = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&maxbpc, kfn, block, shmem));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&maxbpc, kfn, block, lds_for(block)));
+    /* When the per-lane LDS slots (detector records of many media, pattern
+     * weights and energies of shared patterns) leave room for fewer than
+     * the register-limited number of waves per CU, smaller workgroups pack
+     * the LDS more tightly: take the workgroup size with the most resident
+     * waves (the larger on ties).  A caller that fixed the workgroup size
+     * or the workgroups per CU keeps them. */
+    if (!(tune && tune->block > 0) && want == 0) {
+        for (int b = block / 2; b >= 64; b /= 2) {
+            if (lds_for(b) > kLdsMax) continue;
+            int n = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kfn, b, lds_for(b)));
+            if ((int64_t)n * b > (int64_t)maxbpc * block) {
+                block = b;
+                maxbpc = n;
+            }
+        }
+    }
+    const size_t shmem = lds_for(block);
     int bpc = want > 0 ? std::min(want, std::max(maxbpc, 1)) : std::max(maxbpc, 1);
     int grid = ncu * bpc;
 
