@@ -125,13 +125,22 @@ __device__ __forceinline__ const KArgs* kargs() {
     return (const KArgs*)p;
 }
 
-/* true when the C library may round e to the far neighbour of f = (float)e:
- * e within D |o - f| of the midpoint (every operation here is exact) */
+/* true when the C library may round e to the far neighbour o of f =
+ * (float)e: e within D |o - f| of the midpoint.  With r = e - f and u = |o -
+ * f| (one ulp of f, half of it when stepping towards zero from a power of
+ * two; a power of two either way) that is |r| >= (0.5 - D) u, and every
+ * operation is exact.  (Checked against the midpoint form on all 2^23
+ * generator outputs of every draw and thresholds on the boundary.) */
 __device__ __forceinline__ bool lm_near(double e, float f, float D) {
-    const double fd = (double)f;
-    if (e == fd) return false;
-    const double od = (double)mmc_fnext(f, e);
-    return fabs(e - 0.5 * (fd + od)) <= (double)D * fabs(od - fd);
+    const double r = e - (double)f;
+    if (r == 0.0) return false;
+    const uint32_t b = __float_as_uint(f) & 0x7FFFFFFFu;
+    int ex = (int)(b >> 23);
+    const bool away = (r > 0.0) == (f > 0.f);
+    const int half = (!away && (b & 0x7FFFFFu) == 0u && ex > 1) ? 1 : 0;
+    ex = ex > 1 ? ex : 1;
+    const double u = __longlong_as_double((long long)(ex - 150 - half + 1023) << 52);
+    return fabs(r) >= (0.5 - (double)D) * u;
 }
 __device__ __forceinline__ float lm_pick(float f, double e, uint32_t bit) { return bit ? mmc_fnext(f, e) : f; }
 
