@@ -138,10 +138,15 @@ def bench_plain(args):
             "kernel": kname, "kernel_ms": k_avg,
             "bytes_per_step": bytes_per_step, "steps_per_launch": steps_per_launch}
     if counters:
+        # traffic: the HBM bytes per step that FETCH_SIZE/WRITE_SIZE measured
+        # for this kernel and workload (the committed counter summary) times
+        # this run's steps per launch
+        roof["traffic_source"] = (f"{counters['hbm_bytes_per_step']:.1f} measured HBM bytes per step "
+                                  f"({counters['source']}) x steps per launch of this run")
         # the kernel issues vector instructions nearly every cycle: the
         # HBM fraction above is the algorithmic-bytes view, the limiter is
         # VALU issue (figures from the committed counter summary)
-        roof["limiter"] = "valu-issue"
+        roof["limiter"] = "valu-issue" if counters.get("valu_issue_frac", 0) > 0.8 else "latency/divergence"
         roof["counters"] = {k: counters[k] for k in ("source", "valu_issue_frac", "active_lane_frac",
                                                     "valu_per_wave_step", "l2_hit_frac", "hbm_bytes_per_step")
                             if k in counters}
