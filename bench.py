@@ -33,6 +33,12 @@ BYTES_PER_STEP_ELEM = 92.0   # 64 B face planes + 16 B face neighbours + 4 B lab
 # Plucker nodal step: 48 B node coordinates + 16 B node ids + 16 B neighbours
 # + 4 x 8 B nodal FP32 atomic RMW + 16 B next-element ids
 BYTES_PER_STEP_NODAL = 128.0
+# the same accesses counted in whole 128-byte cache lines, each access in
+# its own line: Badouel planes, neighbours, the deposit's line; Plucker node
+# coordinates (48 B may straddle two lines), node ids, neighbours, four nodal
+# deposits
+LINE_BYTES_PER_STEP_ELEM = 3 * 128.0
+LINE_BYTES_PER_STEP_NODAL = 8 * 128.0
 
 
 def log(*a):
@@ -136,7 +142,9 @@ def bench_plain(args):
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
             "kernel": kname, "kernel_ms": k_avg,
-            "bytes_per_step": bytes_per_step, "steps_per_launch": steps_per_launch}
+            "bytes_per_step": bytes_per_step,
+            "line_bytes_per_step": LINE_BYTES_PER_STEP_NODAL if nodal else LINE_BYTES_PER_STEP_ELEM,
+            "steps_per_launch": steps_per_launch}
     if counters:
         # traffic: the HBM bytes per step that FETCH_SIZE/WRITE_SIZE measured
         # for this kernel and workload (the committed counter summary) times
