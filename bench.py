@@ -127,7 +127,7 @@ def bench_plain(args):
         with open(cfile) as f:
             counters = json.load(f)
     traffic = counters["hbm_bytes_per_step"] * steps_per_launch if counters else None
-    kname = "mmc::transport<1, false, 4, 1>" if nodal else "mmc::transport<0, false, 4, 1>"
+    kname = "mmc::transport<1, false, 3, 1>" if nodal else "mmc::transport<0, false, 4, 1>"
     if shells:
         workload = (f"shells: jittered 200x200x100 mm box, 1 mm cells x6 = {mesh.ne / 1e6:.1f}M tets, 5 tissues in "
                     f"nested ellipsoidal shells, one 64x64 pattern over 60x60 mm, {args.photons:.0e} photons/step, "

@@ -18,7 +18,9 @@ from .sim import ResidentMesh, SimConfig, tune_struct
 _cache: Dict[Tuple, dict] = {}
 
 BLOCKS = (256,)
-BLOCKS_PER_CU = (0, 5)          # 0 = the default register budget (4 per CU); 5 selects the kernel compiled for 5
+# 0 = the default register budget (4 workgroups per CU; 3 for the Plucker
+# kernel); 3 and 5 select the kernels compiled for 3 and 5 per CU
+BLOCKS_PER_CU = (0, 3, 5)
 CHUNKS = (1, 8)
 
 

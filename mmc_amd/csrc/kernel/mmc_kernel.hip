@@ -1648,6 +1648,7 @@ __global__ void fill_zero_u64(unsigned long long* p, int n) {
     template __global__ void transport<kTrGridRF, true, OCC, TAB>(KArgs);   \
     template __global__ void transport<kTrElemNodal, false, OCC, TAB>(KArgs); \
     template __global__ void transport<kTrElemNodal, true, OCC, TAB>(KArgs);
+MMC_INSTANTIATE(3, kTabLds)
 MMC_INSTANTIATE(4, kTabLds)
 MMC_INSTANTIATE(5, kTabLds)
 MMC_INSTANTIATE(4, kTabGlobal)
