@@ -195,12 +195,13 @@ def test_shared_patterns_equal_one_run_per_pattern(method):
     assert np.linalg.norm(shared.field[..., 2] - one.field) <= 1e-5 * np.linalg.norm(one.field)
 
 
-@pytest.mark.parametrize("method", ["S", "P"])
-def test_shared_patterns_match_oracle(method):
+@pytest.mark.parametrize("method,basis", [("S", 0), ("S", 1), ("P", 1)])
+def test_shared_patterns_match_oracle(method, basis):
     # photon sharing against the oracle's own restatement: same photons, and
     # every pattern's slot of the field and its launched/absorbed energies
-    # (element output under Badouel, nodal output under Plucker)
-    nodal = method == "P"
+    # (element and exit-face-node output under Badouel, nodal output under
+    # Plucker)
+    nodal = basis == 1
     node, elem, et, med = M.meshgen.cube60(4.0)
     base = M.TetMesh(node, elem, et, med).prepare(nodal=nodal)
     c = M.meshgen.centroids(base.node, base.elem)
@@ -210,7 +211,7 @@ def test_shared_patterns_match_oracle(method):
     pats = np.random.default_rng(6).uniform(0.1, 1.0, (4, 6, 8)).astype(np.float32)
     cfg = M.SimConfig(nphoton=20000, method=method, srctype="pattern", srcpos=(20.0, 20.0, 0.0), srcdir=(0, 0, 1),
                       srcparam1=(20.0, 0, 0, 8), srcparam2=(0, 20.0, 0, 6), srcpattern=pats, tend=5e-9,
-                      tstep=1e-9, seed=41, e0=int(srcelem[0]))
+                      tstep=1e-9, seed=41, e0=int(srcelem[0]), basisorder=basis)
     g = M.run(tm, cfg, records=(0, cfg.nphoton))
     o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), math="portable")
     assert compare_photons(g.photons, o.photons).mean() >= 0.999
