@@ -197,8 +197,10 @@ typedef struct mmc_hip_tune {
     int32_t blocks_per_cu;   /* resident workgroups per CU */
     int32_t chunk;           /* photons claimed per lane refill round (1 = per photon) */
     int32_t compensated;     /* 1: TwoSum-compensated FP32 atomics (hi+lo buffers) */
-    int32_t replicas;        /* FP32 field copies, one per XCD group (0 = automatic:
-                                8 when they fit the device-memory budget, else 4, 2, 1) */
+    int32_t replicas;        /* FP32 field copies: 1..8 (one per XCD group) or a multiple of 8
+                                up to 64 (several per XCD); 0 = automatic: up to 64 while
+                                they take at most 256 MiB, else 8 when they fit the
+                                device-memory budget, else 4, 2, 1 */
 } mmc_hip_tune;
 
 typedef struct mmc_hip_meshres* mmc_hip_mesh_handle;

@@ -17,6 +17,7 @@ constexpr int kBlock = 256;          /* default threads per workgroup (4 waves) 
 constexpr int kMaxProp = 4000;       /* media labels 1..kMaxProp per mesh (MAX_PROP, src/mmc_const.h:43) */
 constexpr int kPackedProp = 15;      /* labels that fit the 4 low bits of a packed face neighbour */
 constexpr int kLaneSlots = 8;        /* per-lane LDS accumulators (energy, Kahan terms) */
+constexpr int kMaxReplicas = 64;     /* FP32 field copies: 1..8, or 8 per die up to 64 */
 constexpr int kMaxSrc = 8;           /* patterns shared by one photon (srcnum) */
 
 /* Per-simulation parameters read on every step or crossing; staged in LDS

@@ -614,7 +614,12 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
     /* Each XCD has its own L2: with one field copy per XCD no cache line is
      * shared between dies, so the atomics stay local to one L2 instead of
      * contending across the fabric (all copies are summed in FP64 at the end). */
-    const uint64_t rep = a.nrep > 1 ? (uint64_t)(xcd_id() % (uint32_t)a.nrep) : 0;
+    /* this workgroup's field copy: its die's, and with more than 8 copies
+     * (a multiple of 8) one of the die's nrep/8, taken by workgroup number */
+    const uint32_t nrepu = (uint32_t)a.nrep;
+    const uint64_t rep = nrepu <= 1 ? 0
+                         : nrepu <= 8 ? (uint64_t)(xcd_id() % nrepu)
+                                      : (uint64_t)(xcd_id() % 8u + 8u * ((blockIdx.x >> 3) % (nrepu >> 3)));
     float* const fcopy = a.field + rep * a.repstride;
     float* const lcopy = COMP ? a.field_lo + rep * a.repstride : nullptr;
     const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));

@@ -279,8 +279,9 @@ class _ResultBuf:
 
 
 def tune_struct(block=0, blocks_per_cu=0, compensated=0, chunk=1, replicas=0) -> TuneC:
-    """Launch parameters; replicas = FP32 field copies (0: one per XCD when
-    they fit the device-memory budget)."""
+    """Launch parameters; replicas = FP32 field copies (0: automatic, up to 8
+    per XCD for small fields, one per XCD when they fit the device-memory
+    budget)."""
     t = TuneC()
     t.block, t.blocks_per_cu, t.chunk, t.compensated = int(block), int(blocks_per_cu), int(chunk), int(compensated)
     t.replicas = int(replicas)
