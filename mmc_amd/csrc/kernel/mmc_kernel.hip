@@ -1298,7 +1298,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                                 npend = true;
                             }
                         }
-                        if (!isend && npend) flush_nodal();
+                        /* (the sums of the nodes shared with the next
+                         * element follow the photon across the face; see
+                         * the crossing below) */
                         if (isend) {
                             b0 = bo[0]; b1 = bo[1]; b2 = bo[2]; b3 = bo[3];
                         } else if (nb_id(nexteid, a.lblshift) > 0 && faceidx < 4) {
@@ -1424,6 +1426,27 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                     if (oldtype != 0 && ntype == 0 && !a.isextdet) eid = 0;
                 }
                 if (eid > 0) {
+                    if (NODAL && npend && eid != oldeid) {
+                        /* Plucker nodal sums across the face: the three nodes
+                         * the next element shares keep their sums (in the next
+                         * element's node order, which cee takes when its
+                         * geometry is loaded); the node left behind is
+                         * deposited.  Same sums per node as a deposit at every
+                         * crossing, a quarter of the atomics. */
+                        const int4 nn = a.elem[eid - 1];
+                        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+                        const int on[4] = {cee.x, cee.y, cee.z, cee.w};
+                        const float ov[4] = {na0, na1, na2, na3};
+                        #pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            if (nn.x == on[i]) m0 = ov[i];
+                            else if (nn.y == on[i]) m1 = ov[i];
+                            else if (nn.z == on[i]) m2 = ov[i];
+                            else if (nn.w == on[i]) m3 = ov[i];
+                            else if (ov[i] != 0.f) depo(nfoff, nloc + (uint32_t)(on[i] - 1), ov[i]);
+                        }
+                        na0 = m0; na1 = m1; na2 = m2; na3 = m3;
+                    }
                     ctype = ntype;
                     ctx = CTX_INNER;
                     continue;
