@@ -1070,7 +1070,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                         if (opp != 1) na1 += w3;
                         if (opp != 2) na2 += w3;
                         if (opp != 3) na3 += w3;
-                        if (faceid == -2 || !isend) flush_nodal();
+                        /* (at a crossing the sums of the exit face's nodes,
+                         * which the next element shares, follow the photon;
+                         * see the crossing below) */
                     } else if (!GRID) {
                         uint32_t newidx = (uint32_t)(e + tshift);
                         if (oldidx == 0xFFFFFFFFu) oldidx = newidx;
@@ -1426,8 +1428,8 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                     if (oldtype != 0 && ntype == 0 && !a.isextdet) eid = 0;
                 }
                 if (eid > 0) {
-                    if (NODAL && npend && eid != oldeid) {
-                        /* Plucker nodal sums across the face: the three nodes
+                    if ((NODAL || BNODE) && npend && eid != oldeid) {
+                        /* Nodal sums across the face: the three nodes
                          * the next element shares keep their sums (in the next
                          * element's node order, which cee takes when its
                          * geometry is loaded); the node left behind is
