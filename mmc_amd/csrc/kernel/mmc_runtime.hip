@@ -968,12 +968,11 @@ static int32_t slot_enqueue(mmc_hip_slotres* sl, mmc_hip_mesh_handle m, const mm
           : tracer == kTrGridRF ? MMC_KFN(kTrGridRF, O, S)                              \
           : tracer == kTrElemRF ? MMC_KFN(kTrElemRF, O, S)                              \
                                 : MMC_KFN(kTrElem, O, S);
-    /* 3 workgroups per CU leave the Plucker kernel 168 VGPRs and no spills,
-     * which pays where its mesh reads miss the caches: 1.35x on the 24.0M-tet
-     * shells mesh (1.9 GB of node coordinates, ids and neighbours) and level,
-     * within the run-to-run spread, on the 162k-tet cube, whose mesh stays in
-     * the last-level cache.  The Badouel kernels run best at 4 (the 3-per-CU
-     * build is 13 % slower on the slab). */
+    /* 3 workgroups per CU leave the Plucker kernel 168 VGPRs and no spills:
+     * against 4 per CU, 1.07x with one pattern and 1.27x with 8 shared
+     * patterns on the 24.0M-tet shells mesh, 1.09x on the 162k-tet cube.  The
+     * Badouel kernels run best at 4 (the 3-per-CU build is 13 % slower on the
+     * slab). */
     const bool three = want == 3 || (want == 0 && tracer == kTrNodal);
     if (!stage) {
         MMC_PICK(4, kTabGlobal)
