@@ -57,6 +57,29 @@ def test_cube_plucker_nodal_per_photon(cube):
     assert abs(g.normalizer / o.normalizer - 1) < 1e-4
 
 
+@pytest.mark.parametrize("method", ["P", "S"])
+def test_nodal_output_across_refracting_layers(method):
+    # nodal output (Plucker, and Badouel with its exit-face nodes) in a cube
+    # whose upper half has another refractive index: at the internal face
+    # photons either reflect (and stay in their element) or refract into the
+    # next, and the nodal sums follow them only in the second case
+    node, elem = M.meshgen.t6_lattice(15, 15, 15, 2.0)
+    c = M.meshgen.centroids(node, elem)
+    et = np.where(c[:, 2] < 14.0, 1, 2).astype(np.int32)
+    med = np.array([[0, 0, 1, 1], [0.01, 1.0, 0.5, 1.37], [0.02, 2.0, 0.7, 1.6]], np.float32)
+    tm = M.TetMesh(node, elem, et, med).prepare(nodal=True)
+    om = O.OracleMesh(node, elem, et, med, nodal=True)
+    cfg = M.SimConfig(nphoton=20000, method=method, basisorder=1, srcpos=(15.1, 14.9, 0.0), srcdir=(0, 0, 1),
+                      tend=5e-9, tstep=1e-9, seed=61)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), math="portable")
+    assert compare_photons(g.photons, o.photons).mean() >= 0.999
+    assert g.field.shape == o.field.shape == (5, om.nn)
+    rel = np.linalg.norm(g.field - o.field) / np.linalg.norm(o.field)
+    print(f"nodal field vs oracle: {rel:.2e}", flush=True)
+    assert rel < 1e-4
+
+
 def test_slab_badouel_element_per_photon_and_detectors(slab):
     _, tm, om = slab
     cfg = W.slab_sim(20000, seed=99)
