@@ -244,7 +244,7 @@ def bench_serve(args, deadline):
     reqs = reqs[:n]
     log(f"serving {n} of {asked} requests ({sum(r.cfg.nphoton for r in reqs):.3g} photons; offered load "
         f"{offered:.3g} photons/ms)")
-    srv = S.Server(meshes, arena_bytes=16 << 30)
+    srv = S.Server(meshes, arena_bytes=16 << 30, policy=args.serve_policy)
     served = srv.serve(reqs)
     srv.close()
     log(f"served: {served}")
@@ -254,7 +254,7 @@ def bench_serve(args, deadline):
     return {"workload": f"Poisson 200 requests/s, meshes cube 162k / slab 3.0M / jittered 5.2M tets, photons "
                         f"log-uniform [1e5, {hi:.0e}], 1-50 gates, per-tissue media as the batch workload, "
                         "element fluence",
-            "requests_asked": asked, "requests": n,
+            "requests_asked": asked, "requests": n, "policy": args.serve_policy,
             "offered_photons_per_ms": offered, "capacity_photons_per_ms": served["photons_per_ms"],
             "served": served,
             "plain_sample": {"every": every, **plain},
@@ -367,6 +367,9 @@ def main():
     ap.add_argument("--serve-plain-every", type=int, default=5,
                     help="the plain drop-in runs every n-th request of the served stream")
     ap.add_argument("--serve-max-photons", type=float, default=1e8)
+    ap.add_argument("--serve-policy", default="srpt", choices=["srpt", "fair"],
+                    help="serving order: fewest photons left first (srpt) or first come with equal "
+                         "shares (fair)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sims", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -293,7 +293,8 @@ def test_field_copies_sum_to_the_fp64_accumulation(cube, replicas):
     assert abs(g.fieldsum - o.field.sum()) <= 1e-6 * abs(o.field.sum())
 
 
-def test_served_results_equal_results_alone():
+@pytest.mark.parametrize("policy", ["fair", "srpt"])
+def test_served_results_equal_results_alone(policy):
     from mmc_amd.serve import Request, Server
     arrays = M.meshgen.slab_layers(4.0)
     mesh = M.TetMesh(*arrays).prepare(nodal=False)
@@ -305,7 +306,7 @@ def test_served_results_equal_results_alone():
     reqs = [Request(rid=i, mesh=k, cfg=c, arrival=0.001 * i, want_field=True) for i, (k, c) in enumerate(cfgs)]
     # compensated (TwoSum) accumulation on both sides, so that the fields can be
     # compared far below the FP32 atomic-ordering noise (~1e-4 relative)
-    srv = Server(meshes, arena_bytes=1 << 28, slice_photons=60000, compensated=True)
+    srv = Server(meshes, arena_bytes=1 << 28, slice_photons=60000, compensated=True, policy=policy)
     srv.sched.min_chunk = 20000          # several chunks per request
     stats = srv.serve(reqs)
     srv.close()

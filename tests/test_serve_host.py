@@ -91,3 +91,36 @@ def test_rf_request_reserves_its_imaginary_block():
     assert rf.flen == 2 * plain.flen == 4000
     s.admit(0.2)
     assert plain.off >= rf.off + rf.flen
+
+
+def test_srpt_admits_and_runs_the_fewest_photons_first():
+    """Shortest remaining work first: admission by photon count (any request
+    whose field fits, not only the head of the queue) and launches filled
+    with the requests that have the fewest photons left."""
+    s = Scheduler(arena_floats=11_000, datalen={"a": 1000, "b": 1000}, slice_photons=1000, min_chunk=100,
+                  policy="srpt")
+    reqs = [mkreq(0, "a", 5, 5000, 0.0), mkreq(1, "b", 6, 300, 0.1), mkreq(2, "a", 3, 700, 0.2),
+            mkreq(3, "a", 1, 200, 0.3)]
+    for r in reqs:
+        s.arrive(r)
+    admitted = s.admit(0.4)
+    # fields: 5000, 6000, 3000, 1000 floats (64-float aligned); smallest
+    # photon counts first: 3 (200), 1 (300), 2 (700) fit in 11000 floats,
+    # 0 (5000 photons) waits
+    assert [r.rid for r in admitted] == [3, 1, 2]
+    assert [r.rid for r in s.waiting] == [0]
+    mesh, chunks = s.plan()
+    # request 3 has the fewest photons left: mesh a, then 2 on the same mesh
+    assert mesh == "a" and [(r.rid, n) for r, n in chunks] == [(3, 200), (2, 700)]
+    for r, n in chunks:
+        r.assigned += n
+    mesh, chunks = s.plan()
+    assert mesh == "b" and [(r.rid, n) for r, n in chunks] == [(1, 300)]
+
+
+def test_unknown_policy_is_refused():
+    try:
+        Scheduler(arena_floats=10, datalen={"a": 1}, policy="lifo")
+    except ValueError:
+        return
+    raise AssertionError("unknown policy accepted")
