@@ -707,6 +707,22 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
         if (na3 != 0.f) depo(nfoff, nloc + (uint32_t)(cee.w - 1), na3);
         npend = false;
     };
+    /* the geometry and neighbours of element e into the cache registers
+     * (Badouel: the 4 planes; Plucker: the 12 node coordinates in cNX..cNZ
+     * and the node ids) */
+    auto load_elem = [&](int e) {
+        if (!NODAL) {
+            const float4* P = a.planes + 4 * (size_t)e;
+            cNX = P[0]; cNY = P[1]; cNZ = P[2]; cDD = P[3];
+            if (BNODE) cee = a.elem[e];
+        } else {
+            const float4* Q = a.enode + 3 * (size_t)e;
+            cNX = Q[0]; cNY = Q[1]; cNZ = Q[2];
+            cee = a.elem[e];
+        }
+        cfnb = a.facenb[e];
+        cache_e = e;
+    };
     /* one trajectory record: photon id, position, weight, element
      * (savedebugdata, src/mmc_raytrace.c:1739-1756) */
     auto savetraj = [&]() {
@@ -938,22 +954,9 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
             /* ============================ ray-tetrahedron step ============ */
             const int e = eid - 1;
             /* the element's geometry and neighbours stay in registers while
-             * the photon scatters inside it; reloaded after a crossing
-             * (Badouel: the 4 planes; Plucker: the 12 node coordinates in
-             * cNX..cNZ and the node ids) */
-            if (e != cache_e) {
-                if (!NODAL) {
-                    const float4* P = a.planes + 4 * (size_t)e;
-                    cNX = P[0]; cNY = P[1]; cNZ = P[2]; cDD = P[3];
-                    if (BNODE) cee = a.elem[e];
-                } else {
-                    const float4* Q = a.enode + 3 * (size_t)e;
-                    cNX = Q[0]; cNY = Q[1]; cNZ = Q[2];
-                    cee = a.elem[e];
-                }
-                cfnb = a.facenb[e];
-                cache_e = e;
-            }
+             * the photon scatters inside it; a crossing loads the next
+             * element's, so only a photon's first element is loaded here */
+            if (e != cache_e) load_elem(e);
             const int4 fnb = cfnb;
             const float4 m = med(ctype);
             const float mua = m.x, mus = m.y, nidx = m.w;
@@ -1428,16 +1431,21 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                     if (oldtype != 0 && ntype == 0 && !a.isextdet) eid = 0;
                 }
                 if (eid > 0) {
-                    if ((NODAL || BNODE) && npend && eid != oldeid) {
+                  if (eid != oldeid) {
+                    /* the next element's geometry is fetched here, so the
+                     * loads overlap the bookkeeping up to its first step */
+                    const int4 oe = cee;
+                    load_elem(eid - 1);
+                    if ((NODAL || BNODE) && npend) {
                         /* Nodal sums across the face: the three nodes
                          * the next element shares keep their sums (in the next
                          * element's node order, which cee takes when its
                          * geometry is loaded); the node left behind is
                          * deposited.  Same sums per node as a deposit at every
                          * crossing, a quarter of the atomics. */
-                        const int4 nn = a.elem[eid - 1];
+                        const int4 nn = cee;
                         float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
-                        const int on[4] = {cee.x, cee.y, cee.z, cee.w};
+                        const int on[4] = {oe.x, oe.y, oe.z, oe.w};
                         const float ov[4] = {na0, na1, na2, na3};
                         #pragma unroll
                         for (int i = 0; i < 4; i++) {
@@ -1449,6 +1457,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
                         }
                         na0 = m0; na1 = m1; na2 = m2; na3 = m3;
                     }
+                  }
                     ctype = ntype;
                     ctx = CTX_INNER;
                     continue;
