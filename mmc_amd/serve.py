@@ -319,7 +319,12 @@ class Server:
             while pending and pending[0].arrival <= now():
                 r = pending.popleft()
                 self.sched.arrive(r, self.meshes[r.mesh])
-            self.sched.admit(now())
+            for r in self.sched.admit(now()):
+                if r.cfg.nphoton == 0:
+                    # nothing to run: its field is its (cleared) arena region
+                    self._collect(r)
+                    self.sched.complete(r, now())
+                    done.append(r)
             if len(self.inflight) < len(self.slots):
                 mesh, chunks = self.sched.plan()
                 if chunks:

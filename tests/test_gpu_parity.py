@@ -450,3 +450,45 @@ def test_invalid_settings_fail_with_a_message(slab):
                           M.SimConfig(nphoton=100, omega=2e9, srcpos=(50.0, 50.0, 0.0))])
     finally:
         rm.release()
+
+
+def test_zero_photons_plain_batched_and_served(slab):
+    """Empty inputs: a simulation of zero photons returns an empty field and
+    zero energies through the plain path; inside a batch it leaves the other
+    simulations as they run alone; through the server it completes (with an
+    empty field) beside requests that do run."""
+    import copy
+    from mmc_amd.serve import Request, Server
+    _, mesh, _ = slab
+    sims = W.slab_batch_sims(3, seed=5, lo=3000, hi=6000)
+    empty = copy.copy(sims[1])
+    empty.nphoton = 0
+    print("case: zero photons, plain path", flush=True)
+    r = M.run(mesh, empty)
+    assert r.launchweight == 0 and r.absorbweight == 0 and r.raytet == 0 and r.detected == 0
+    assert not r.field.any()
+
+    print("case: zero photons inside a batch of three", flush=True)
+    t = M.sim.tune_struct(compensated=1)
+    rm = M.ResidentMesh(mesh)
+    batch = rm.run_batch([sims[0], empty, sims[2]], tune=t)
+    alone = [rm.run_batch([s], tune=t)[0] for s in (sims[0], sims[2])]
+    rm.release()
+    assert batch[1].launchweight == 0 and batch[1].raytet == 0 and not batch[1].field.any()
+    for b, a in zip((batch[0], batch[2]), alone):
+        assert b.launchweight == a.launchweight and b.raytet == a.raytet
+        assert abs(b.absorbweight - a.absorbweight) <= 1e-6 * a.launchweight
+        assert np.linalg.norm(b.field - a.field) <= 1e-6 * np.linalg.norm(a.field)
+
+    print("case: zero photons through the server", flush=True)
+    reqs = [Request(rid=0, mesh="slab", cfg=sims[0], arrival=0.0, want_field=True),
+            Request(rid=1, mesh="slab", cfg=empty, arrival=0.0, want_field=True),
+            Request(rid=2, mesh="slab", cfg=sims[2], arrival=0.001, want_field=True)]
+    srv = Server({"slab": mesh}, arena_bytes=1 << 27, slice_photons=4000, compensated=True, policy="srpt")
+    stats = srv.serve(reqs)
+    srv.close()
+    assert stats["requests"] == 3
+    assert reqs[1].finished >= 0 and reqs[1].launches == 0 and reqs[1].fieldsum == 0 and not reqs[1].field.any()
+    for q, a in zip((reqs[0], reqs[2]), alone):
+        assert q.launchweight == a.launchweight and q.raytet == a.raytet
+        assert np.linalg.norm(q.field - a.field) <= 1e-6 * np.linalg.norm(a.field)

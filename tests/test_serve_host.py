@@ -124,3 +124,24 @@ def test_unknown_policy_is_refused():
     except ValueError:
         return
     raise AssertionError("unknown policy accepted")
+
+
+def test_requests_without_photons_are_admitted_but_never_planned():
+    """A zero-photon request takes its (empty) field region and is completed
+    by the server at admission; no launch ever carries a chunk of it."""
+    for policy in ("fair", "srpt"):
+        s = Scheduler(arena_floats=10_000, datalen={"a": 100}, slice_photons=1000, min_chunk=100, policy=policy)
+        reqs = [mkreq(0, "a", 1, 0, 0.0), mkreq(1, "a", 1, 1500, 0.0)]
+        for r in reqs:
+            s.arrive(r)
+        assert sorted(r.rid for r in s.admit(0.0)) == [0, 1]
+        seen = []
+        while True:
+            mesh, chunks = s.plan()
+            if not chunks:
+                break
+            assert all(n > 0 for _, n in chunks)
+            for r, n in chunks:
+                r.assigned += n
+                seen.append(r.rid)
+        assert set(seen) == {1} and reqs[1].assigned == 1500
