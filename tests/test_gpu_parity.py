@@ -425,6 +425,7 @@ def test_invalid_settings_fail_with_a_message(slab):
     cases = [
         (tm, dict(method="G", gridstep=0.0), "voxel edge"),
         (tm, dict(method="G", gridstep=1e-4), "too large"),
+        (tm, dict(tstart=0.0, tend=6e-9, tstep=1e-12), "exceeds 2"),     # 375k elements x 6000 gates
         (cube, dict(method="P", omega=1e9), "RF"),
         (tm, dict(omega=-1.0), "omega"),
         (tm, dict(srctype="pattern", srcparam1=(10, 0, 0, 4), srcparam2=(0, 10, 0, 4),
