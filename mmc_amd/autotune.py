@@ -38,6 +38,8 @@ def sample_batch(sims: List[SimConfig], total: int) -> List[SimConfig]:
 
 
 def autotune(rm: ResidentMesh, sims: List[SimConfig], sample_photons: int = 6_000_000, verbose=False) -> dict:
+    if not sims:
+        raise ValueError("autotune needs at least one simulation")
     key = (rm.mesh.ne, sims[0].method, min(len(sims), 512) // 16)
     if key in _cache:
         return _cache[key]

@@ -348,11 +348,14 @@ class Server:
 
 
 def summarize(done: List[Request], wall_s: float, launches: int, kernel_ms: float) -> dict:
+    if not done:
+        return {"requests": 0, "photons": 0, "wall_ms": wall_s * 1e3, "photons_per_ms": 0.0, "p50_ms": 0.0,
+                "p99_ms": 0.0, "mean_ms": 0.0, "launches": launches, "kernel_ms": kernel_ms}
     lat = np.array([r.latency for r in done]) * 1e3
     photons = sum(r.cfg.nphoton for r in done)
     first = min(r.arrival for r in done)
     return {"requests": len(done), "photons": photons, "wall_ms": wall_s * 1e3,
-            "photons_per_ms": photons / ((wall_s - first) * 1e3),
+            "photons_per_ms": photons / (max(wall_s - first, 1e-9) * 1e3),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "launches": launches, "kernel_ms": kernel_ms}
 

@@ -145,3 +145,13 @@ def test_requests_without_photons_are_admitted_but_never_planned():
                 r.assigned += n
                 seen.append(r.rid)
         assert set(seen) == {1} and reqs[1].assigned == 1500
+
+
+def test_statistics_of_an_empty_stream():
+    from mmc_amd.serve import summarize
+    st = summarize([], 0.5, 0, 0.0)
+    assert st["requests"] == 0 and st["photons"] == 0 and st["photons_per_ms"] == 0.0
+    r = mkreq(0, "a", 1, 0, 0.0)
+    r.admitted, r.finished = 0.0, 0.0
+    st = summarize([r], 0.0, 0, 0.0)
+    assert st["requests"] == 1 and st["photons"] == 0 and st["p50_ms"] == 0.0
