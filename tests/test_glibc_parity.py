@@ -16,9 +16,9 @@ from the same xorshift128+ state, so:
   decided, and fixphoton nudges -- are counted by the oracle and reported, and
   photons with ties are held to the same bar.
 
-Workloads: the 60 mm cube of 2 mm cells (Plucker, nodal, roulette) and the
+Workloads: the 60 mm cube of 2 mm cells (Plucker, nodal, roulette), the
 full 100x100x50 mm slab of 1 mm cells (3.0M tets; branch-less Badouel,
-element output, 16 detectors).
+element output, 16 detectors), and every source type on a scattering cube.
 """
 import numpy as np
 import pytest
@@ -27,6 +27,7 @@ import pyoracle as O
 
 import mmc_amd as M
 from mmc_amd import workloads as W
+from test_sources import CASES, source_case
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +91,31 @@ def test_full_slab_badouel_matches_glibc_per_photon_and_detectors(slab_full):
     prop = len(arrays[3]) - 1
     assert np.array_equal(gd[:, :1 + prop], od[:, :1 + prop])          # detector id, scattering counts
     assert np.allclose(gd[:, 1 + prop:], od[:, 1 + prop:], rtol=1e-6, atol=0)   # partial paths, w0
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_every_source_type_matches_glibc_per_photon(name):
+    # the launch of each source type (src/mmc_raytrace.c:2346-2592) followed
+    # by transport, against the oracle with the host C library's libm
+    tm, om, cfg = source_case(name)
+    g = M.run(tm, cfg, records=(0, cfg.nphoton))
+    o = O.run(om, cfg, photonseed=M.photon_seeds(cfg.seed, 0, cfg.nphoton), records=(0, cfg.nphoton),
+              math="glibc")
+    same, close, wok, tied = report(g.photons, o.photons, o.ties, f"source {name}")
+    # The z-Gaussian zenith sqrt(-2 log u1) (1 - 2 u2) sigma is a function of
+    # two draws, so the agreement tables (functions of one generator output)
+    # do not cover its sine and cosine: the kernel rounds them correctly
+    # through double, where glibc's sincosf misrounds now and then, and 0.2 %
+    # of the photons drift apart by more than 1e-4 mm after the first
+    # scattering (none leaves through another element).  The reference's own
+    # z-Gaussian takes u2 from an unseeded thread-private state, ran0
+    # (src/mmc_host.c:227,265,280-283), so it has no reproducible photons to
+    # match anyway.  The launch direction, not a tie, decides which photons
+    # drift, so the photons with tie cases (a few hundred) hold the same
+    # share up to sampling: 6 of 280 differ.
+    zg = name == "zgaussian"
+    assert same.mean() >= 0.999 and close.mean() >= (0.995 if zg else 0.999)
+    assert wok.mean() >= 0.999
+    assert (g.photons["reason"] == o.photons["reason"])[same & close].all()
+    if tied.any():
+        assert (same & close)[tied].mean() >= (0.95 if zg else 0.999)

@@ -225,9 +225,9 @@ CASES = {
 }
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_gpu_sources_per_photon(name):
+def source_case(name):
+    """The scattering cube (4 mm cells, source elements below z = 8 mm) and
+    the simulation of source case `name`: (device mesh, oracle mesh, cfg)."""
     kw = CASES[name]
     st = "planar" if name == "planar-focus" else name
     node, elem, et, med = meshgen.cube60(4.0)
@@ -247,6 +247,13 @@ def test_gpu_sources_per_photon(name):
                       srcparam1=kw["p1"], srcparam2=kw.get("p2", (0, 0, 0, 0)), tend=5e-9, tstep=5e-10, seed=23,
                       srcpattern=(np.random.default_rng(4).random((3, 8, 8)).astype(np.float32)
                                   if st == "pattern3d" else None))
+    return tm, om, cfg
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gpu_sources_per_photon(name):
+    tm, om, cfg = source_case(name)
     print(f"source {name}", flush=True)
     g = M.run(tm, cfg, records=(0, cfg.nphoton))
     o = O.run(om, cfg, photonseed=M.photon_seeds(cfg.seed, 0, cfg.nphoton), records=(0, cfg.nphoton),
