@@ -415,6 +415,10 @@ def test_forty_media_labels_match_the_oracle_per_photon():
     gi, oi = np.argsort(g.detphoton), np.argsort(o.detphoton)
     assert np.array_equal(g.detdata[gi], o.detdata[oi])
     assert abs(g.absorbweight - o.absorbweight) <= 1e-6 * o.launchweight
+    # and against the reference's arithmetic (the host C library's libm)
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), maxdet=cfg.nphoton, math="glibc")
+    assert compare_photons(g.photons, o.photons).mean() >= 0.999
+    assert abs(g.detected - o.detected) <= 0.001 * cfg.nphoton
 
 
 def test_invalid_settings_fail_with_a_message(slab):

@@ -97,3 +97,8 @@ def test_gpu_detector_elements_match_the_oracle(cols):
     gi, oi = np.argsort(g.detphoton), np.argsort(o.detphoton)
     assert np.array_equal(g.detphoton[gi], o.detphoton[oi])
     assert np.allclose(g.detdata[gi], o.detdata[oi], rtol=1e-5, atol=1e-5)
+    # and against the reference's arithmetic (the host C library's libm)
+    o = O.run(om, cfg, photonseed=M.photon_seeds(cfg.seed, 0, cfg.nphoton), records=(0, cfg.nphoton),
+              maxdet=cfg.nphoton, math="glibc")
+    assert (g.photons["eid"] == o.photons["eid"]).mean() >= 0.999
+    assert (g.photons["detid"] == o.photons["detid"]).mean() >= 0.999
