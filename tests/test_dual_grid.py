@@ -161,6 +161,13 @@ def test_gpu_grid_matches_the_oracle_on_the_slab(out):
     print(f"grid field rel L2 {rel:.3e}, normalizer {g.normalizer:.6e} vs {o.normalizer:.6e}", flush=True)
     assert rel < 1e-4
     assert abs(g.normalizer / o.normalizer - 1) < 1e-6
+    # and against the reference's arithmetic (the host C library's libm)
+    o = O.run(om, cfg, photonseed=M.photon_seeds(cfg.seed, 0, cfg.nphoton), records=(0, cfg.nphoton),
+              math="glibc", normalize=True)
+    same = (g.photons["eid"] == o.photons["eid"]) & (g.photons["reason"] == o.photons["reason"])
+    rel = np.linalg.norm(g.field - o.field) / np.linalg.norm(o.field)
+    print(f"against the glibc oracle: same photons {same.mean():.5f}, grid field rel L2 {rel:.3e}", flush=True)
+    assert same.mean() >= 0.999 and rel < 1e-4
 
 
 @pytest.mark.gpu
