@@ -356,6 +356,11 @@ def test_replay_outputs_match_the_oracle(slab, out):
     assert gr.launchweight == pytest.approx(o.launchweight, rel=1e-6)
     assert np.linalg.norm(gr.field - o.field) <= 1e-5 * np.linalg.norm(o.field)
     assert abs(gr.normalizer / o.normalizer - 1) < 1e-6
+    # and against the reference's arithmetic (the host C library's libm)
+    o = O.run(om, rcfg, photonseed=rcfg.photonseed, math="glibc", normalize=True)
+    rel = np.linalg.norm(gr.field - o.field) / np.linalg.norm(o.field)
+    print(f"replay {out} against the glibc oracle: field rel L2 {rel:.3e}", flush=True)
+    assert gr.launchweight == pytest.approx(o.launchweight, rel=1e-6) and rel <= 1e-4
 
 
 @pytest.mark.parametrize("out", ["l", "p"])
