@@ -243,6 +243,12 @@ def test_shared_patterns_match_oracle(method, basis):
         rel = np.linalg.norm(g.field[..., k] - o.field[..., k]) / np.linalg.norm(o.field[..., k])
         assert rel < 1e-3, (k, rel)
     assert np.allclose(g.srcweight, o.srcweight, rtol=1e-3, atol=0)
+    # and against the reference's arithmetic (the host C library's libm)
+    o = O.run(om, cfg, photonseed=seeds(cfg), records=(0, cfg.nphoton), math="glibc")
+    assert compare_photons(g.photons, o.photons).mean() >= 0.999
+    for k in range(4):
+        rel = np.linalg.norm(g.field[..., k] - o.field[..., k]) / np.linalg.norm(o.field[..., k])
+        assert rel < 1e-3, (k, rel)
 
 
 def test_batch_result_equals_result_alone(slab):
