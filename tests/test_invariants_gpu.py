@@ -14,6 +14,9 @@
   independent runs is tested against its own scatter (a t test).
 * Time of flight: a detected photon's time equals the sum over media of its
   partial path times n / c0 (the partial paths of its detection record).
+* Exit state: a detected photon's saved exit position lies on the detector
+  surface inside its detector, and its direction is a unit vector pointing
+  out of the mesh.
 """
 import numpy as np
 import pytest
@@ -65,3 +68,26 @@ def test_detected_time_of_flight_is_path_times_index():
     rel = np.abs(t_rec - t_path) / t_path
     print(f"{g.detected} detected photons: time vs path x n / c0 max rel {rel.max():.2e}", flush=True)
     assert rel.max() < 1e-4
+
+
+def test_exit_positions_and_directions_of_detected_photons():
+    # examples/regression/exitangle: with the exit state saved (-x 1), a
+    # detected photon left through the surface z = 0 that holds the
+    # detectors, inside its detector's disk, moving outward with a unit
+    # direction
+    arrays = meshgen.slab_layers(2.0)
+    tm = M.TetMesh(*arrays).prepare(nodal=False)
+    prop = len(arrays[3]) - 1
+    cfg = W.slab_sim(20000, seed=62)
+    cfg.issaveexit = 1
+    g = M.run(tm, cfg)
+    assert g.detected > 50
+    d = g.detdata.astype(np.float64)
+    p, v = d[:, 1 + 2 * prop:4 + 2 * prop], d[:, 4 + 2 * prop:7 + 2 * prop]
+    det = np.asarray(cfg.detpos, np.float64)[d[:, 0].astype(int) - 1]
+    print(f"{g.detected} detected: |z| max {np.abs(p[:, 2]).max():.1e}, vz max {v[:, 2].max():.3f}, "
+          f"| |v| - 1 | max {np.abs(np.linalg.norm(v, axis=1) - 1).max():.1e}", flush=True)
+    assert np.abs(p[:, 2]).max() < 1e-4
+    assert (v[:, 2] < 0).all()
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-5)
+    assert (((p[:, :2] - det[:, :2]) ** 2).sum(1) < det[:, 3] ** 2 + 1e-4).all()
