@@ -1642,7 +1642,7 @@ __global__ void __launch_bounds__(kBlock, OCC) transport(KArgs karg) {
 
 /* FP32 -> FP64 accumulation of a finished field (the final reduction): the
  * nrep copies (and their compensation buffers) summed in copy order; zeroes
- * the FP32 buffers for reuse. */
+ * the FP32 buffers for reuse.  out may be null when only the sum is wanted. */
 #undef a
 
 __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, double* sum, int nrep,
@@ -1661,7 +1661,7 @@ __global__ void field_reduce(float* f, float* lo, double* out, uint64_t n, doubl
                 lo[k] = 0.f;
             }
         }
-        out[i] = v;
+        if (out) out[i] = v;
         part += v;
     }
     /* wave reduction, one FP64 atomic per wave */

@@ -19,6 +19,8 @@
 #include <mutex>
 #include <string>
 #include <chrono>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "mmc_hip.h"
@@ -121,6 +123,33 @@ static int ensure_libm_tables() {
     if (rc) return rc;
     g_tab_dev[dev] = true;
     return 0;
+}
+
+/* dst[k] += src[k] over a field copied back from the device: split over up
+ * to 8 host threads when the field is large (a memory-bound loop that one
+ * thread runs at a fraction of the host's bandwidth); serial when threads
+ * cannot be started. */
+static void add_into(double* dst, const double* src, uint64_t n) {
+    const uint64_t kMinPerThread = 1ull << 22;
+    uint64_t nt = std::min<uint64_t>(std::max(1u, std::thread::hardware_concurrency()), 8);
+    nt = std::min<uint64_t>(nt, (n + kMinPerThread - 1) / kMinPerThread);
+    auto part = [dst, src, n, nt](uint64_t t) {
+        const uint64_t b = n * t / nt, e = n * (t + 1) / nt;
+        for (uint64_t k = b; k < e; k++) dst[k] += src[k];
+    };
+    if (nt > 1) {
+        std::vector<std::thread> th;
+        uint64_t started = 0;
+        try {
+            for (uint64_t t = 1; t < nt; t++, started++) th.emplace_back(part, t);
+        } catch (...) {
+        }
+        part(0);
+        for (auto& x : th) x.join();
+        for (uint64_t t = started + 1; t < nt; t++) part(t);   /* parts whose thread did not start */
+        return;
+    }
+    for (uint64_t k = 0; k < n; k++) dst[k] += src[k];
 }
 
 template <typename T>
@@ -1104,12 +1133,17 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
         int dev = 0, ncu = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        /* the FP64 staging buffer exists only when a field goes to the host;
+         * otherwise the reduction yields the sums alone */
         uint64_t maxn = 1;
-        for (int i = 0; i < nsim; i++)
+        bool host = false;
+        for (int i = 0; i < nsim; i++) {
             maxn = std::max<uint64_t>(maxn, (uint64_t)sl->datalen[i] * sl->maxgate[i] * sl->rfmul[i]);
+            host = host || res[i].field || res[i].field_im;
+        }
         DevBuf d64, dsum;
         int rc;
-        if ((rc = dalloc((double**)&d64.p, (size_t)maxn, nullptr)) ||
+        if ((host && (rc = dalloc((double**)&d64.p, (size_t)maxn, nullptr))) ||
             (rc = dalloc((double**)&dsum.p, (size_t)nsim + 1, nullptr)))
             return rc;
         HIPCHK(hipMemsetAsync(dsum.p, 0, sizeof(double) * (nsim + 1), st));
@@ -1128,10 +1162,11 @@ static int32_t slot_finish(mmc_hip_slotres* sl, mmc_hip_result* res) {
                 HIPCHK(hipGetLastError());
                 double* dst = part ? r.field_im : r.field;
                 if (dst) {
-                    std::vector<double> hbuf(nre);
-                    HIPCHK(hipMemcpyAsync(hbuf.data(), d64.p, sizeof(double) * nre, hipMemcpyDeviceToHost, st));
+                    std::unique_ptr<double[]> hbuf(new (std::nothrow) double[nre]);
+                    if (!hbuf) return set_err(-2, "host staging of %llu field values failed", (unsigned long long)nre);
+                    HIPCHK(hipMemcpyAsync(hbuf.get(), d64.p, sizeof(double) * nre, hipMemcpyDeviceToHost, st));
                     HIPCHK(hipStreamSynchronize(st));
-                    for (uint64_t k = 0; k < nre; k++) dst[k] += hbuf[k];
+                    add_into(dst, hbuf.get(), nre);
                 }
             }
         }
@@ -1343,16 +1378,16 @@ extern "C" int32_t mmc_hip_arena_collect(mmc_hip_arena_handle a, uint64_t off, u
     hipLaunchKernelGGL(field_reduce, dim3(std::max(rb, 1)), dim3(256), 0, a->st, a->field + off,
                        a->lo ? a->lo + off : nullptr, (double*)a->d64.p, n, (double*)a->dsum.p, a->nrep, a->n);
     HIPCHK(hipGetLastError());
-    std::vector<double> hbuf;
+    std::unique_ptr<double[]> hbuf;
     double s = 0.0;
     if (host) {
-        hbuf.resize(n);
-        HIPCHK(hipMemcpyAsync(hbuf.data(), a->d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, a->st));
+        hbuf.reset(new (std::nothrow) double[n]);
+        if (!hbuf) return set_err(-2, "host staging of %llu field values failed", (unsigned long long)n);
+        HIPCHK(hipMemcpyAsync(hbuf.get(), a->d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, a->st));
     }
     HIPCHK(hipMemcpyAsync(&s, a->dsum.p, sizeof(double), hipMemcpyDeviceToHost, a->st));
     HIPCHK(hipStreamSynchronize(a->st));
-    if (host)
-        for (uint64_t k = 0; k < n; k++) host[k] += hbuf[k];
+    if (host) add_into(host, hbuf.get(), n);
     if (sum) *sum = s;
     return 0;
 }
